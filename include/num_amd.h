@@ -1,0 +1,170 @@
+/*
+ * num_amd.h -- C ABI of libnum_amd.so: CSR SpMV, multi-RHS SpMV, BLAS-1 and
+ * CG on one MI355X (gfx950), the drop-in native layer under the Python module
+ * num_amd.kokkos_spmv.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  "d_" pointers are device memory
+ *    (hipMalloc / torch tensors), "h_" pointers host memory.  Index arrays are
+ *    int32 (reference src/crs_matrix.hpp:22-24: Ordinal = Offset = int),
+ *    values float64 (Scalar = double) unless the name says f32.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Kernel entry points are asynchronous on that stream; the Python layer
+ *    synchronises before returning, as every reference call ends in a
+ *    Kokkos::fence (src/spmv_kernel.hpp:160,179; src/vec_ops.hpp:25,41,57).
+ *  - Return 0 on success, else a NUM_E* code; num_last_error() gives the
+ *    thread-local message.  NUM_EINVAL is the reference's
+ *    std::invalid_argument (Python ValueError), the others its
+ *    std::runtime_error (Python RuntimeError).
+ *  - No entry point falls back to the CPU.
+ */
+#ifndef NUM_AMD_H
+#define NUM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUM_OK 0
+#define NUM_EINVAL 1   /* bad argument: shape, size, pointer, range        */
+#define NUM_ERUNTIME 2 /* numerical breakdown, state error                  */
+#define NUM_EHIP 3     /* HIP runtime error                                 */
+#define NUM_ENOMEM 4   /* allocation failed                                 */
+
+typedef struct num_csr num_csr;
+
+/* CGResult (src/cg_solver.hpp:22-26) plus what this build adds. */
+typedef struct {
+    int32_t iters;        /* loop SpMVs; 0 on early exit (cg_solver.hpp:158-159) */
+    int32_t converged;    /* 1 if sqrt(r.r)/||b|| < tol (strict <)              */
+    int32_t breakdown;    /* 1 if |p.Ap| < 1e-300 stopped the solve             */
+    int32_t pad_;
+    double residual;      /* sqrt(r.r)/||b|| of the recursive residual          */
+} num_cg_result;
+
+/* Mixed-precision refined CG (no reference counterpart). */
+typedef struct {
+    int32_t outer;        /* FP64 refinement steps (residual evaluations - 1) */
+    int32_t inner_total;  /* FP32 CG iterations summed over all steps          */
+    int32_t converged;    /* 1 if the FP64 true residual reached tol          */
+    int32_t breakdown;
+    double true_residual; /* ||b - A x|| / ||b|| in FP64 at exit              */
+} num_mixed_result;
+
+/* ---- runtime ----------------------------------------------------------- */
+/* init(device_id) of src/bindings.cpp:53-66: selects the device; idempotent. */
+int num_init(int32_t device_id);
+/* finalize() of src/bindings.cpp:68-70. */
+int num_finalize(void);
+const char* num_last_error(void);
+int32_t num_abi_version(void);
+
+/* ---- CSR handle (src/crs_matrix.hpp:37-73, CrsMatrixHandle) -------------- */
+/* Borrow device CSR arrays owned by the caller (they must outlive the handle)
+ * and build the row-block schedule from the host copy of rowptr.
+ * Rejects nrows<=0 or ncols<=0 (crs_matrix.hpp:52-53) and, unlike the
+ * reference, a non-monotone rowptr or rowptr[0] != 0. */
+int num_csr_create(const int32_t* h_rowptr, const int32_t* d_rowptr,
+                   const int32_t* d_colind, const double* d_vals, int32_t nrows,
+                   int32_t ncols, num_csr** out);
+/* Copy host CSR arrays to the device (the CrsMatrixHandle constructor itself,
+ * crs_matrix.hpp:45-72); the handle owns the copies.  Also checks every
+ * column index is in [0, ncols). */
+int num_csr_create_host(const int32_t* h_rowptr, const int32_t* h_colind,
+                        const double* h_vals, int32_t nrows, int32_t ncols,
+                        num_csr** out);
+/* Attach an FP32 copy of the values (same pattern) for the FP32 inner solve. */
+int num_csr_attach_f32(num_csr* A, const float* d_vals32);
+int num_csr_destroy(num_csr* A);
+/* nrows / ncols / nnz read-only attributes (src/bindings.cpp:79-88). */
+int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz);
+/* Row-block schedule: at most rows_per_block rows and nnz_per_block
+ * non-zeros per workgroup (rows longer than nnz_per_block get a workgroup of
+ * their own).  Rebuilds the schedule; used by the autotuner. */
+int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block);
+int num_csr_schedule_info(const num_csr* A, int32_t* nblocks, int32_t* rows_per_block,
+                          int32_t* nnz_per_block, int32_t* long_rows, int32_t* max_row_nnz);
+
+/* ---- SpMV (src/spmv_kernel.hpp:118-180) ---------------------------------- */
+/* y = A x.  Replaces both launch_spmv_kokkoskernels (spmv_kernel.hpp:170-180)
+ * and launch_spmv_custom (spmv_kernel.hpp:118-161): one row-binned kernel. */
+int num_spmv(const num_csr* A, const double* d_x, double* d_y, void* stream);
+/* y[r] = (A x)[r] for rows r in [row_begin, row_end) only (interior/boundary
+ * split of one rank's slab; no reference counterpart, the reference runs the
+ * whole local SpMV after the halo, src/dist_spmv.hpp:157-162). */
+int num_spmv_rows(const num_csr* A, int32_t row_begin, int32_t row_end,
+                  const double* d_x, double* d_y, void* stream);
+/* Slab SpMV without an x_ext copy: rows [row_begin, row_end) of a rank's
+ * local matrix whose columns >= nrows are ghosts; column c reads
+ * x_own[c] if c < nrows else x_ghost[c - nrows].  Replaces x_ext_fill_local +
+ * local SpMV of distributed_spmv (src/dist_spmv.hpp:118-163). */
+int num_spmv_split(const num_csr* A, int32_t row_begin, int32_t row_end,
+                   const double* d_x_own, const double* d_x_ghost, double* d_y, void* stream);
+/* FP32 y = A32 x (needs num_csr_attach_f32). */
+int num_spmv_f32(const num_csr* A, const float* d_x, float* d_y, void* stream);
+/* Host round trip: upload, y = A x, download (spmv_host_roundtrip,
+ * src/spmv_kernel.hpp:186-217). */
+int num_spmv_host(const int32_t* h_rowptr, const int32_t* h_colind, const double* h_vals,
+                  int32_t nrows, int32_t ncols, const double* h_x, double* h_y);
+
+/* ---- multi-RHS SpMV (src/batch_spmv_kernel.hpp:120-174) ------------------ */
+/* Y = A X, X (ncols, k) and Y (nrows, k) row-major (LayoutRight). */
+int num_batch_spmv(const num_csr* A, const double* d_X, double* d_Y, int32_t k,
+                   void* stream);
+/* batch_spmv_host_roundtrip (src/batch_spmv_kernel.hpp:179-209). */
+int num_batch_spmv_host(const int32_t* h_rowptr, const int32_t* h_colind,
+                        const double* h_vals, int32_t nrows, int32_t ncols, int32_t k,
+                        const double* h_X, double* h_Y);
+
+/* ---- BLAS-1 (src/vec_ops.hpp:16-100) ------------------------------------- */
+int num_axpy(int64_t n, double alpha, const double* d_x, const double* d_y, double* d_z,
+             void* stream);                                    /* z = a x + y   */
+int num_axpy2(int64_t n, double alpha, const double* d_x, double beta, const double* d_y,
+              double* d_z, void* stream);                      /* z = a x + b y */
+int num_scale(int64_t n, double alpha, double* d_x, void* stream);   /* x *= a  */
+int num_fill(int64_t n, double val, double* d_x, void* stream);      /* x = v   */
+int num_copy(int64_t n, const double* d_src, double* d_dst, void* stream);
+/* Bytes of device workspace num_dot needs for length n. */
+int64_t num_dot_workspace_bytes(int64_t n);
+/* *d_out = sum x_i y_i, deterministic two-stage reduction, result left on the
+ * device (vec_local_dot, vec_ops.hpp:47-59, minus its host read-back). */
+int num_dot(int64_t n, const double* d_x, const double* d_y, double* d_out, void* d_work,
+            void* stream);
+/* *d_out = sqrt(sum x_i^2) (vec_local_norm2, vec_ops.hpp:64-69). */
+int num_norm2(int64_t n, const double* d_x, double* d_out, void* d_work, void* stream);
+
+/* ---- CG (src/cg_solver.hpp:133-196, cg_solve_local) ---------------------- */
+int64_t num_cg_workspace_bytes(const num_csr* A);
+/* Solves A x = b; x holds x0 on entry.  Same operation order, ||b||=0 -> 1
+ * rule, strict < test and iteration count as the reference; scalars stay on
+ * the device.  Returns NUM_ERUNTIME (result filled, breakdown=1) when
+ * |p.Ap| < 1e-300, the guard of cg_solve_distributed (cg_solver.hpp:84-85). */
+int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol,
+                 int32_t max_iter, void* d_work, int64_t work_bytes, num_cg_result* res,
+                 void* stream);
+
+/* ---- mixed-precision refined CG (no reference counterpart) -------------- */
+int64_t num_cg_mixed_workspace_bytes(const num_csr* A);
+/* FP64 outer loop r = b - A x, FP32 inner CG (FP64-accumulated dots) on
+ * A d = r to relative tolerance inner_tol, x += d in FP64; stops when the
+ * FP64 true residual ||b - A x||/||b|| < tol.  Needs num_csr_attach_f32. */
+int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double tol,
+                       int32_t max_outer, double inner_tol, int32_t inner_max,
+                       void* d_work, int64_t work_bytes, num_mixed_result* res,
+                       void* stream);
+
+/* ---- halo pack / scatter of one rank's share (src/halo_exchange.hpp) --- */
+/* buf[i] = x[idx[i]] for i < count (halo_pack, halo_exchange.hpp:222-235). */
+int num_halo_pack(const int32_t* d_idx, int32_t count, const double* d_x, double* d_buf,
+                  void* stream);
+/* x_ghost[offset + i] = buf[i] for i < count (halo_scatter_<r>,
+ * halo_exchange.hpp:287-300). */
+int num_halo_scatter(const double* d_buf, int32_t count, double* d_x_ghost, int32_t offset,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUM_AMD_H */

@@ -1,0 +1,103 @@
+// common.hpp -- error plumbing, launch constants and wave/block reductions
+// shared by every translation unit of libnum_amd.so (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/num_amd.h"
+
+namespace numamd {
+
+// ---------------------------------------------------------------- errors ---
+// Thread-local last-error text (num_last_error()).  Return codes follow
+// include/num_amd.h: NUM_EINVAL maps to ValueError in Python, the others to
+// RuntimeError.
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+const char* last_error();
+
+#define NUM_HIP(expr)                                                               \
+    do {                                                                            \
+        hipError_t e__ = (expr);                                                    \
+        if (e__ != hipSuccess)                                                      \
+            return ::numamd::fail(NUM_EHIP, "%s failed: %s (%s:%d)", #expr,         \
+                                  hipGetErrorString(e__), __FILE__, __LINE__);      \
+    } while (0)
+
+#define NUM_LAUNCH_CHECK()                                                          \
+    do {                                                                            \
+        hipError_t e__ = hipGetLastError();                                         \
+        if (e__ != hipSuccess)                                                      \
+            return ::numamd::fail(NUM_EHIP, "kernel launch failed: %s (%s:%d)",     \
+                                  hipGetErrorString(e__), __FILE__, __LINE__);      \
+    } while (0)
+
+// ------------------------------------------------------------ constants ---
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kRedBlock = 256;     // block size of elementwise / reduction kernels
+constexpr int kMaxRedBlocks = 2048;  // cap on per-launch partial sums
+
+// Workgroup index -> logical tile so that each XCD (private L2) walks one
+// contiguous stretch of rows: hardware block b runs on XCD b % 8, and gets
+// logical tile (b % 8) * per + b / 8.  Returns -1 for the padding blocks.
+__device__ __forceinline__ int xcd_tile(int b, int ntiles, int per)
+{
+    const int t = (b % kNumXcd) * per + b / kNumXcd;
+    return t < ntiles ? t : -1;
+}
+inline int xcd_per(int ntiles) { return (ntiles + kNumXcd - 1) / kNumXcd; }
+inline int xcd_grid(int ntiles) { return xcd_per(ntiles) * kNumXcd; }
+
+// ------------------------------------------------------------ reductions ---
+// Deterministic butterfly sum over a whole wave64: every lane ends with the
+// same value, combined in a fixed pattern independent of timing.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum; result valid in thread 0.  `red` must hold
+// blockDim.x / 64 doubles.  Ends with a barrier so `red` can be reused.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < BS / kWave; ++i) s += red[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Two independent block sums in one pass (r.r and b.b of the CG set-up).
+template <int BS>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) { red[w] = a; red[BS / kWave + w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0;
+#pragma unroll
+        for (int i = 0; i < BS / kWave; ++i) { sa += red[i]; sb += red[BS / kWave + i]; }
+        a = sa;
+        b = sb;
+    }
+    __syncthreads();
+}
+
+}  // namespace numamd

@@ -1,0 +1,60 @@
+// csr.hpp -- the CSR handle behind num_csr* and its row-block schedule.
+#pragma once
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+namespace numamd {
+
+constexpr int kSpmvBS = 256;  // threads per row-block workgroup
+
+// Row-block schedule over rows [row_begin, row_end): consecutive rows packed
+// greedily into blocks of at most rows_per_block rows and nnz_per_block
+// non-zeros; a row with more than nnz_per_block non-zeros is a block alone.
+struct Schedule {
+    int32_t row_begin = 0, row_end = 0;
+    int32_t rows_per_block = 0, nnz_per_block = 0;
+    int32_t npt = 0;             // LDS slots per thread the kernel is built for
+    int32_t nblocks = 0;
+    int32_t long_rows = 0;       // single-row blocks longer than nnz_per_block
+    int32_t* d_blk_row = nullptr;  // device [2 * (nblocks + 1)]: [first row, lanes] pairs
+    std::vector<int32_t> h_blk_row;
+};
+
+}  // namespace numamd
+
+struct num_csr {
+    int device = 0;
+    int32_t nrows = 0, ncols = 0;
+    int64_t nnz = 0;
+    const int32_t* rowptr = nullptr;
+    const int32_t* colind = nullptr;
+    const double* vals = nullptr;
+    const float* vals32 = nullptr;
+    bool owns = false;                  // rowptr/colind/vals allocated here
+    std::vector<int32_t> h_rowptr;      // host copy, drives scheduling
+    int32_t max_row_nnz = 0;
+    numamd::Schedule sched;             // whole-matrix schedule
+    mutable std::mutex mu;              // guards range_sched
+    mutable std::map<std::pair<int32_t, int32_t>, numamd::Schedule> range_sched;
+};
+
+namespace numamd {
+
+// Builds `s` for rows [r0, r1) of A and uploads it.  Returns NUM_OK or error.
+int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
+                   int32_t nnz_per_block, Schedule& s);
+void free_schedule(Schedule& s);
+// Schedule for a row range (cached per range); whole range -> A->sched.
+int range_schedule(const num_csr* A, int32_t r0, int32_t r1, const Schedule** out);
+int check_device(const num_csr* A);
+
+// Launch y = A x over a schedule (T = double or float).
+template <typename T>
+int launch_spmv(const num_csr* A, const Schedule& s, const T* vals, const T* x, T* y,
+                hipStream_t stream);
+
+}  // namespace numamd

@@ -1,0 +1,334 @@
+// runtime.hip -- errors, device selection, CSR handle life cycle and the
+// row-block scheduler.
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "csr.hpp"
+
+namespace numamd {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char* last_error() { return g_err; }
+
+int check_device(const num_csr* A)
+{
+    int dev = -1;
+    NUM_HIP(hipGetDevice(&dev));
+    if (dev != A->device)
+        return fail(NUM_EINVAL, "matrix lives on device %d but device %d is current", A->device,
+                    dev);
+    return NUM_OK;
+}
+
+static int pick_npt(int32_t nnz_per_block)
+{
+    const int need = (nnz_per_block + kSpmvBS - 1) / kSpmvBS;
+    if (need <= 4) return 4;
+    if (need <= 8) return 8;
+    return 16;
+}
+
+// Lanes per row in the reduce phase: 1 (sequential, bit-identical to the
+// row loop) while the longest row has <= 32 entries, else enough lanes for
+// ~16 entries each, capped by the wave width and by BS / rows.
+static int32_t reduce_lanes(int32_t maxlen, int32_t nrows_blk)
+{
+    if (maxlen <= 32) return 1;
+    int32_t want = 1;
+    while (want * 16 < maxlen && want < kWave) want *= 2;
+    int32_t cap = 1;
+    while (cap * 2 * std::max(nrows_blk, 1) <= kSpmvBS && cap * 2 <= kWave) cap *= 2;
+    return std::min(want, cap);
+}
+
+void free_schedule(Schedule& s)
+{
+    if (s.d_blk_row) (void)hipFree(s.d_blk_row);
+    s.d_blk_row = nullptr;
+    s.h_blk_row.clear();
+    s.nblocks = 0;
+}
+
+int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
+                   int32_t nnz_per_block, Schedule& s)
+{
+    if (rows_per_block < 1 || rows_per_block > kSpmvBS)
+        return fail(NUM_EINVAL, "rows_per_block must be in [1, %d], got %d", kSpmvBS,
+                    rows_per_block);
+    if (nnz_per_block < 1 || nnz_per_block > 16 * kSpmvBS)
+        return fail(NUM_EINVAL, "nnz_per_block must be in [1, %d], got %d", 16 * kSpmvBS,
+                    nnz_per_block);
+    if (r0 < 0 || r1 > A->nrows || r0 > r1)
+        return fail(NUM_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, A->nrows);
+    const int32_t* rp = A->h_rowptr.data();
+    // entries: [first row, lanes per row] per block, then [r1, 0]
+    std::vector<int32_t> blk;
+    blk.reserve(2 * ((size_t)(r1 - r0) / (size_t)std::max(1, rows_per_block / 2) + 8));
+    int32_t longs = 0;
+    int32_t r = r0;
+    while (r < r1) {
+        const int32_t start = r;
+        const int32_t len = rp[r + 1] - rp[r];
+        if (len > nnz_per_block) {
+            ++r;
+            ++longs;
+            blk.push_back(start);
+            blk.push_back(reduce_lanes(len, 1));
+            continue;
+        }
+        int64_t nz = 0;
+        int32_t maxlen = 0;
+        while (r < r1 && r - start < rows_per_block) {
+            const int32_t l = rp[r + 1] - rp[r];
+            if (nz + l > nnz_per_block) break;
+            nz += l;
+            maxlen = std::max(maxlen, l);
+            ++r;
+        }
+        blk.push_back(start);
+        blk.push_back(reduce_lanes(maxlen, r - start));
+    }
+    blk.push_back(r1);
+    blk.push_back(0);
+    Schedule ns;
+    ns.row_begin = r0;
+    ns.row_end = r1;
+    ns.rows_per_block = rows_per_block;
+    ns.nnz_per_block = nnz_per_block;
+    ns.npt = pick_npt(nnz_per_block);
+    ns.nblocks = (int32_t)blk.size() / 2 - 1;
+    ns.long_rows = longs;
+    const size_t bytes = blk.size() * sizeof(int32_t);
+    NUM_HIP(hipMalloc(&ns.d_blk_row, bytes));
+    hipError_t e = hipMemcpy(ns.d_blk_row, blk.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(ns.d_blk_row);
+        return fail(NUM_EHIP, "schedule upload failed: %s", hipGetErrorString(e));
+    }
+    ns.h_blk_row = std::move(blk);
+    free_schedule(s);
+    s = std::move(ns);
+    return NUM_OK;
+}
+
+int range_schedule(const num_csr* A, int32_t r0, int32_t r1, const Schedule** out)
+{
+    if (r0 == 0 && r1 == A->nrows) {
+        *out = &A->sched;
+        return NUM_OK;
+    }
+    std::lock_guard<std::mutex> lk(A->mu);
+    auto key = std::make_pair(r0, r1);
+    auto it = A->range_sched.find(key);
+    if (it == A->range_sched.end()) {
+        Schedule s;
+        int rc = build_schedule(A, r0, r1, A->sched.rows_per_block, A->sched.nnz_per_block, s);
+        if (rc) return rc;
+        it = A->range_sched.emplace(key, std::move(s)).first;
+    }
+    *out = &it->second;
+    return NUM_OK;
+}
+
+static int validate_rowptr(const int32_t* h_rowptr, int32_t nrows, int32_t ncols)
+{
+    if (nrows <= 0 || ncols <= 0)
+        return fail(NUM_EINVAL, "CrsMatrixHandle: non-positive dimensions");
+    if (!h_rowptr) return fail(NUM_EINVAL, "rowptr is NULL");
+    if (h_rowptr[0] != 0) return fail(NUM_EINVAL, "rowptr[0] must be 0, got %d", h_rowptr[0]);
+    for (int32_t i = 0; i < nrows; ++i)
+        if (h_rowptr[i + 1] < h_rowptr[i])
+            return fail(NUM_EINVAL, "rowptr decreases at row %d", i);
+    return NUM_OK;
+}
+
+static int finish_create(num_csr* A)
+{
+    int32_t mx = 0;
+    for (int32_t i = 0; i < A->nrows; ++i)
+        mx = std::max(mx, A->h_rowptr[i + 1] - A->h_rowptr[i]);
+    A->max_row_nnz = mx;
+    return build_schedule(A, 0, A->nrows, kSpmvBS, 8 * kSpmvBS, A->sched);
+}
+
+}  // namespace numamd
+
+using namespace numamd;
+
+extern "C" {
+
+const char* num_last_error(void) { return last_error(); }
+
+int32_t num_abi_version(void) { return 1; }
+
+int num_init(int32_t device_id)
+{
+    int count = 0;
+    NUM_HIP(hipGetDeviceCount(&count));
+    if (device_id < 0 || device_id >= count)
+        return fail(NUM_EINVAL, "device_id %d out of range (%d devices)", device_id, count);
+    NUM_HIP(hipSetDevice(device_id));
+    return NUM_OK;
+}
+
+int num_finalize(void) { return NUM_OK; }
+
+int num_csr_create(const int32_t* h_rowptr, const int32_t* d_rowptr, const int32_t* d_colind,
+                   const double* d_vals, int32_t nrows, int32_t ncols, num_csr** out)
+{
+    if (!out) return fail(NUM_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = validate_rowptr(h_rowptr, nrows, ncols);
+    if (rc) return rc;
+    const int64_t nnz = h_rowptr[nrows];
+    if (!d_rowptr || (nnz > 0 && (!d_colind || !d_vals)))
+        return fail(NUM_EINVAL, "device CSR pointer is NULL");
+    num_csr* A = new (std::nothrow) num_csr();
+    if (!A) return fail(NUM_ENOMEM, "out of host memory");
+    if (hipGetDevice(&A->device) != hipSuccess) {
+        delete A;
+        return fail(NUM_EHIP, "hipGetDevice failed");
+    }
+    A->nrows = nrows;
+    A->ncols = ncols;
+    A->nnz = nnz;
+    A->rowptr = d_rowptr;
+    A->colind = d_colind;
+    A->vals = d_vals;
+    A->h_rowptr.assign(h_rowptr, h_rowptr + nrows + 1);
+    rc = finish_create(A);
+    if (rc) {
+        delete A;
+        return rc;
+    }
+    *out = A;
+    return NUM_OK;
+}
+
+int num_csr_create_host(const int32_t* h_rowptr, const int32_t* h_colind, const double* h_vals,
+                        int32_t nrows, int32_t ncols, num_csr** out)
+{
+    if (!out) return fail(NUM_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = validate_rowptr(h_rowptr, nrows, ncols);
+    if (rc) return rc;
+    const int64_t nnz = h_rowptr[nrows];
+    if (nnz > 0 && (!h_colind || !h_vals)) return fail(NUM_EINVAL, "host CSR pointer is NULL");
+    for (int64_t j = 0; j < nnz; ++j)
+        if (h_colind[j] < 0 || h_colind[j] >= ncols)
+            return fail(NUM_EINVAL, "column index %d at position %lld outside [0, %d)",
+                        h_colind[j], (long long)j, ncols);
+    num_csr* A = new (std::nothrow) num_csr();
+    if (!A) return fail(NUM_ENOMEM, "out of host memory");
+    if (hipGetDevice(&A->device) != hipSuccess) {
+        delete A;
+        return fail(NUM_EHIP, "hipGetDevice failed");
+    }
+    A->nrows = nrows;
+    A->ncols = ncols;
+    A->nnz = nnz;
+    A->owns = true;
+    int32_t* rp = nullptr;
+    int32_t* ci = nullptr;
+    double* v = nullptr;
+    hipError_t e = hipMalloc(&rp, sizeof(int32_t) * (size_t)(nrows + 1));
+    if (e == hipSuccess) e = hipMalloc(&ci, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1));
+    if (e == hipSuccess) e = hipMalloc(&v, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1));
+    if (e == hipSuccess)
+        e = hipMemcpy(rp, h_rowptr, sizeof(int32_t) * (size_t)(nrows + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz > 0)
+        e = hipMemcpy(ci, h_colind, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz > 0)
+        e = hipMemcpy(v, h_vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
+    A->rowptr = rp;
+    A->colind = ci;
+    A->vals = v;
+    if (e != hipSuccess) {
+        num_csr_destroy(A);
+        return fail(e == hipErrorOutOfMemory ? NUM_ENOMEM : NUM_EHIP, "CSR upload failed: %s",
+                    hipGetErrorString(e));
+    }
+    A->h_rowptr.assign(h_rowptr, h_rowptr + nrows + 1);
+    rc = finish_create(A);
+    if (rc) {
+        num_csr_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return NUM_OK;
+}
+
+int num_csr_attach_f32(num_csr* A, const float* d_vals32)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    if (!d_vals32 && A->nnz > 0) return fail(NUM_EINVAL, "d_vals32 is NULL");
+    A->vals32 = d_vals32;
+    return NUM_OK;
+}
+
+int num_csr_destroy(num_csr* A)
+{
+    if (!A) return NUM_OK;
+    free_schedule(A->sched);
+    for (auto& kv : A->range_sched) free_schedule(kv.second);
+    if (A->owns) {
+        if (A->rowptr) (void)hipFree(const_cast<int32_t*>(A->rowptr));
+        if (A->colind) (void)hipFree(const_cast<int32_t*>(A->colind));
+        if (A->vals) (void)hipFree(const_cast<double*>(A->vals));
+    }
+    delete A;
+    return NUM_OK;
+}
+
+int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    if (nrows) *nrows = A->nrows;
+    if (ncols) *ncols = A->ncols;
+    if (nnz) *nnz = A->nnz;
+    return NUM_OK;
+}
+
+int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    int rc = build_schedule(A, 0, A->nrows, rows_per_block, nnz_per_block, A->sched);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(A->mu);
+    for (auto& kv : A->range_sched) free_schedule(kv.second);
+    A->range_sched.clear();
+    return NUM_OK;
+}
+
+int num_csr_schedule_info(const num_csr* A, int32_t* nblocks, int32_t* rows_per_block,
+                          int32_t* nnz_per_block, int32_t* long_rows, int32_t* max_row_nnz)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    if (nblocks) *nblocks = A->sched.nblocks;
+    if (rows_per_block) *rows_per_block = A->sched.rows_per_block;
+    if (nnz_per_block) *nnz_per_block = A->sched.nnz_per_block;
+    if (long_rows) *long_rows = A->sched.long_rows;
+    if (max_row_nnz) *max_row_nnz = A->max_row_nnz;
+    return NUM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,567 @@
+"""Drop-in replacement for the reference's ``kokkos_spmv`` module on MI355X.
+
+Same names, arguments, defaults, return values and exceptions as the nanobind
+module of src/bindings.cpp:38-290, with PyTorch-ROCm tensors (``device='cuda'``,
+float64, contiguous) in place of the CUDA ndarrays:
+
+    init, finalize, rank, size, barrier            bindings.cpp:53-76
+    CrsMatrix (nrows, ncols, nnz, repr)            bindings.cpp:79-88
+    DistCrsMatrix (global_nrows, local_row_start,  bindings.cpp:90-94
+                   local_row_end)
+    upload_csr, distribute_csr                     bindings.cpp:97-148
+    spmv(A, x, y, kokkoskernels=True)              bindings.cpp:152-183
+    batch_spmv(A, X, Y)                            bindings.cpp:186-215
+    dist_spmv(A_dist, x_local, y_local)            bindings.cpp:218-236
+    cg_solve(A_dist, b, x, tol, max_iter)          bindings.cpp:239-270
+    cg_solve_local(A, b, x, tol, max_iter)         bindings.cpp:273-290
+
+plus the vector ops of src/vec_ops.hpp (axpy, axpy2, dot, norm2, copy, fill,
+scale), which the reference keeps C++-only.  Every call runs hand-written HIP
+kernels from libnum_amd.so and, like every reference call (each ends in a
+Kokkos::fence), returns after the device work is complete; pass
+``sync=False`` to leave the work queued on the current stream.
+
+Errors: wrong shapes raise ValueError (the reference's std::invalid_argument),
+wrong dtype/device/rank raise TypeError (nanobind's ndarray signature
+mismatch), numerical breakdown raises RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call
+
+_state = {"initialized": False, "device": 0}
+
+
+# ---------------------------------------------------------------- runtime --
+
+def init(device_id: int = 0):
+    """Select the GPU; returns (rank, size).  Idempotent (bindings.cpp:53-66)."""
+    lib = _lib.load()
+    if not torch.cuda.is_available():
+        raise RuntimeError("num_amd: no ROCm GPU is visible (torch.cuda.is_available() "
+                           "is False); there is no CPU fallback")
+    torch.cuda.set_device(int(device_id))
+    _lib.check(lib.num_init(int(device_id)), "init")
+    _state["initialized"] = True
+    _state["device"] = int(device_id)
+    return rank(), size()
+
+
+def finalize() -> None:
+    """bindings.cpp:68-70; frees nothing the handles still own."""
+    _lib.check(_lib.load().num_finalize(), "finalize")
+
+
+def _dist():
+    import torch.distributed as dist
+    return dist if dist.is_available() and dist.is_initialized() else None
+
+
+def rank() -> int:
+    d = _dist()
+    return d.get_rank() if d else 0
+
+
+def size() -> int:
+    d = _dist()
+    return d.get_world_size() if d else 1
+
+
+def barrier() -> None:
+    d = _dist()
+    if d:
+        d.barrier()
+
+
+def _ensure_init():
+    if not _state["initialized"]:
+        init(torch.cuda.current_device() if torch.cuda.is_available() else 0)
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _sync():
+    torch.cuda.current_stream().synchronize()
+
+
+# ------------------------------------------------------------ validation --
+
+def _dev_tensor(t, name: str, ndim: int, dtype=torch.float64) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor on the GPU, got {type(t).__name__}")
+    if t.device.type != "cuda":
+        raise TypeError(f"{name}: expected a GPU tensor, got device {t.device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: expected dtype {dtype}, got {t.dtype}")
+    if t.dim() != ndim:
+        raise TypeError(f"{name}: expected {ndim}-D tensor, got {t.dim()}-D")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous (row-major)")
+    return t
+
+
+def _host_array(a, name: str, dtype) -> np.ndarray:
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cpu":
+            raise TypeError(f"{name}: expected a host array")
+        a = a.numpy()
+    a = np.asarray(a)
+    if a.ndim != 1:
+        raise TypeError(f"{name}: expected a 1-D array")
+    if a.dtype != dtype:
+        raise TypeError(f"{name}: expected dtype {np.dtype(dtype).name}, got {a.dtype}")
+    return np.ascontiguousarray(a)
+
+
+def _ptr(t: torch.Tensor):
+    return C.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------- CrsMatrix --
+
+class CrsMatrix:
+    """Device CSR matrix (src/crs_matrix.hpp:37-73).  Owns its device arrays
+    (torch tensors) and the native handle with its row-block schedule."""
+
+    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_nrows", "_ncols",
+                 "_nnz", "_h_rowptr", "__weakref__")
+
+    def __init__(self):
+        raise TypeError("CrsMatrix objects are created by upload_csr()")
+
+    @classmethod
+    def _from_device(cls, h_rowptr: np.ndarray, rowptr: torch.Tensor, colind: torch.Tensor,
+                     values: torch.Tensor, nrows: int, ncols: int) -> "CrsMatrix":
+        self = object.__new__(cls)
+        self._h = None
+        self._rowptr, self._colind, self._values = rowptr, colind, values
+        self._values32 = None
+        self._h_rowptr = h_rowptr
+        h = C.c_void_p()
+        call("num_csr_create", h_rowptr.ctypes.data_as(C.c_void_p), _ptr(rowptr), _ptr(colind),
+             _ptr(values), int(nrows), int(ncols), C.byref(h))
+        self._h = h
+        r, c, z = C.c_int32(), C.c_int32(), C.c_int64()
+        call("num_csr_shape", h, C.byref(r), C.byref(c), C.byref(z))
+        self._nrows, self._ncols, self._nnz = r.value, c.value, z.value
+        return self
+
+    @property
+    def nrows(self) -> int:
+        return self._nrows
+
+    @property
+    def ncols(self) -> int:
+        return self._ncols
+
+    @property
+    def nnz(self) -> int:
+        return self._nnz
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __repr__(self) -> str:
+        return f"<CrsMatrix nrows={self._nrows} ncols={self._ncols} nnz={self._nnz}>"
+
+    def ensure_f32(self) -> None:
+        """Attach an FP32 copy of the values (mixed-precision inner solve)."""
+        if self._values32 is None:
+            self._values32 = self._values.to(torch.float32)
+            call("num_csr_attach_f32", self._h, _ptr(self._values32))
+
+    def set_schedule(self, rows_per_block: int, nnz_per_block: int) -> None:
+        call("num_csr_set_schedule", self._h, int(rows_per_block), int(nnz_per_block))
+
+    def schedule_info(self) -> dict:
+        v = [C.c_int32() for _ in range(5)]
+        call("num_csr_schedule_info", self._h, *[C.byref(x) for x in v])
+        keys = ("nblocks", "rows_per_block", "nnz_per_block", "long_rows", "max_row_nnz")
+        return {k: x.value for k, x in zip(keys, v)}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().num_csr_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def upload_csr(rowptr, colind, values, nrows: int, ncols: int) -> CrsMatrix:
+    """Copy a host CSR matrix to the GPU (bindings.cpp:97-123).
+
+    rowptr int32[nrows+1], colind int32[nnz], values float64[nnz] (host).
+    Raises ValueError for non-positive dimensions (crs_matrix.hpp:52-53) and,
+    beyond the reference, for inconsistent lengths or out-of-range columns.
+    """
+    _ensure_init()
+    nrows, ncols = int(nrows), int(ncols)
+    rp = _host_array(rowptr, "rowptr", np.int32)
+    ci = _host_array(colind, "colind", np.int32)
+    vv = _host_array(values, "values", np.float64)
+    if nrows <= 0 or ncols <= 0:
+        raise ValueError("CrsMatrixHandle: non-positive dimensions")
+    if rp.shape[0] < nrows + 1:
+        raise ValueError(f"rowptr has {rp.shape[0]} entries, needs nrows+1 = {nrows + 1}")
+    rp = rp[: nrows + 1]
+    nnz = int(rp[-1])
+    if rp[0] != 0 or np.any(np.diff(rp) < 0):
+        raise ValueError("rowptr must start at 0 and be non-decreasing")
+    if ci.shape[0] < nnz or vv.shape[0] < nnz:
+        raise ValueError(f"colind/values shorter than nnz = rowptr[nrows] = {nnz}")
+    ci, vv = ci[:nnz], vv[:nnz]
+    if nnz and (int(ci.min()) < 0 or int(ci.max()) >= ncols):
+        raise ValueError(f"column index outside [0, {ncols})")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    d_rp = torch.from_numpy(rp).to(dev)
+    d_ci = torch.from_numpy(ci).to(dev) if nnz else torch.zeros(1, dtype=torch.int32, device=dev)
+    d_v = torch.from_numpy(vv).to(dev) if nnz else torch.zeros(1, dtype=torch.float64, device=dev)
+    return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, nrows, ncols)
+
+
+# ------------------------------------------------------------------ SpMV --
+
+def spmv(A: CrsMatrix, x, y, kokkoskernels: bool = True, *, sync: bool = True) -> None:
+    """y = A @ x in place (bindings.cpp:152-183).  ``kokkoskernels`` is
+    accepted for compatibility; both settings run the same row-binned kernel."""
+    x = _dev_tensor(x, "x", 1)
+    y = _dev_tensor(y, "y", 1)
+    if x.shape[0] != A.ncols:
+        raise ValueError("spmv: x length != A.ncols")
+    if y.shape[0] != A.nrows:
+        raise ValueError("spmv: y length != A.nrows")
+    call("num_spmv", A._h, _ptr(x), _ptr(y), _stream())
+    if sync:
+        _sync()
+
+
+def batch_spmv(A: CrsMatrix, X, Y, *, sync: bool = True) -> None:
+    """Y = A @ X with X (ncols, k), Y (nrows, k) row-major (bindings.cpp:186-215)."""
+    X = _dev_tensor(X, "X", 2)
+    Y = _dev_tensor(Y, "Y", 2)
+    k = X.shape[1]
+    if X.shape[0] != A.ncols:
+        raise ValueError("batch_spmv: X.shape[0] != A.ncols")
+    if Y.shape[0] != A.nrows or Y.shape[1] != k:
+        raise ValueError("batch_spmv: Y must be (A.nrows, k)")
+    call("num_batch_spmv", A._h, _ptr(X), _ptr(Y), int(k), _stream())
+    if sync:
+        _sync()
+
+
+# ---------------------------------------------------------------- vec ops --
+
+def _same_len(*ts):
+    n = ts[0].shape[0]
+    for t in ts[1:]:
+        if t.shape[0] != n:
+            raise ValueError("vector lengths differ")
+    return n
+
+
+def axpy(alpha: float, x, y, z, *, sync: bool = True) -> None:
+    """z = alpha*x + y (vec_ops.hpp:16-26); z may alias x or y."""
+    x, y, z = (_dev_tensor(t, nm, 1) for t, nm in ((x, "x"), (y, "y"), (z, "z")))
+    n = _same_len(x, y, z)
+    call("num_axpy", n, float(alpha), _ptr(x), _ptr(y), _ptr(z), _stream())
+    if sync:
+        _sync()
+
+
+def axpy2(alpha: float, x, beta: float, y, z, *, sync: bool = True) -> None:
+    """z = alpha*x + beta*y (vec_ops.hpp:31-42)."""
+    x, y, z = (_dev_tensor(t, nm, 1) for t, nm in ((x, "x"), (y, "y"), (z, "z")))
+    n = _same_len(x, y, z)
+    call("num_axpy2", n, float(alpha), _ptr(x), float(beta), _ptr(y), _ptr(z), _stream())
+    if sync:
+        _sync()
+
+
+def _dot_call(x, y, sqrt: bool) -> float:
+    n = x.shape[0]
+    ws = torch.empty(int(_lib.load().num_dot_workspace_bytes(n)) // 8 + 1,
+                     dtype=torch.float64, device=x.device)
+    out = ws[-1:]
+    if sqrt:
+        call("num_norm2", n, _ptr(x), _ptr(out), _ptr(ws), _stream())
+    else:
+        call("num_dot", n, _ptr(x), _ptr(y), _ptr(out), _ptr(ws), _stream())
+    return float(out.item())
+
+
+def dot(x, y) -> float:
+    """sum_i x_i y_i (vec_ops.hpp:47-59), deterministic two-stage reduction."""
+    x = _dev_tensor(x, "x", 1)
+    y = _dev_tensor(y, "y", 1)
+    _same_len(x, y)
+    return _dot_call(x, y, False)
+
+
+def norm2(x) -> float:
+    """||x||_2 (vec_ops.hpp:64-69)."""
+    x = _dev_tensor(x, "x", 1)
+    return _dot_call(x, x, True)
+
+
+def copy(src, dst, *, sync: bool = True) -> None:
+    """dst = src (vec_ops.hpp:74-79)."""
+    src = _dev_tensor(src, "src", 1)
+    dst = _dev_tensor(dst, "dst", 1)
+    n = _same_len(src, dst)
+    call("num_copy", n, _ptr(src), _ptr(dst), _stream())
+    if sync:
+        _sync()
+
+
+def fill(x, val: float, *, sync: bool = True) -> None:
+    """x[i] = val (vec_ops.hpp:84-88)."""
+    x = _dev_tensor(x, "x", 1)
+    call("num_fill", x.shape[0], float(val), _ptr(x), _stream())
+    if sync:
+        _sync()
+
+
+def scale(x, alpha: float, *, sync: bool = True) -> None:
+    """x *= alpha (vec_ops.hpp:93-100)."""
+    x = _dev_tensor(x, "x", 1)
+    call("num_scale", x.shape[0], float(alpha), _ptr(x), _stream())
+    if sync:
+        _sync()
+
+
+# -------------------------------------------------------------------- CG --
+
+def _cg_result(res) -> dict:
+    return {"iters": int(res.iters), "residual": float(res.residual),
+            "converged": bool(res.converged)}
+
+
+def cg_solve_local(A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000) -> dict:
+    """Single-GPU CG (bindings.cpp:273-290, cg_solver.hpp:133-196).
+
+    x is the initial guess and is overwritten with the solution.  Returns
+    {'iters', 'residual', 'converged'}.  Raises RuntimeError when p'Ap
+    underflows 1e-300 (the guard of cg_solver.hpp:84-85, applied here too).
+    """
+    b = _dev_tensor(b, "b", 1)
+    x = _dev_tensor(x, "x", 1)
+    if A.nrows != A.ncols:
+        raise ValueError("cg_solve_local: A must be square")
+    if b.shape[0] != A.nrows or x.shape[0] != A.nrows:
+        raise ValueError("cg_solve_local: b and x must have length A.nrows")
+    lib = _lib.load()
+    nbytes = int(lib.num_cg_workspace_bytes(A._h))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    res = _lib.CgResult()
+    rc = lib.num_cg_solve(A._h, _ptr(b), _ptr(x), float(tol), int(max_iter), _ptr(ws), nbytes,
+                          C.byref(res), _stream())
+    _sync()
+    _lib.check(rc, "cg_solve_local")
+    return _cg_result(res)
+
+
+def cg_solve_mixed(A: CrsMatrix, b, x, tol: float = 1e-10, max_outer: int = 50,
+                   inner_tol: float = 1e-4, inner_max: int = 100000) -> dict:
+    """FP32-inner / FP64-refined CG (no reference counterpart).
+
+    Outer loop in FP64 on the GPU: r = b - A x, stop when
+    ||r||/||b|| < tol (strict), else solve A d = r with FP32 CG (FP32 matrix
+    copy and vectors, FP64-accumulated dots) to relative tolerance
+    ``inner_tol`` and update x += d in FP64.  Returns {'outer', 'inner_total',
+    'residual' (FP64 true residual), 'converged', 'iters' (= inner_total)}.
+    """
+    b = _dev_tensor(b, "b", 1)
+    x = _dev_tensor(x, "x", 1)
+    if A.nrows != A.ncols:
+        raise ValueError("cg_solve_mixed: A must be square")
+    if b.shape[0] != A.nrows or x.shape[0] != A.nrows:
+        raise ValueError("cg_solve_mixed: b and x must have length A.nrows")
+    A.ensure_f32()
+    lib = _lib.load()
+    nbytes = int(lib.num_cg_mixed_workspace_bytes(A._h))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    res = _lib.MixedResult()
+    rc = lib.num_cg_solve_mixed(A._h, _ptr(b), _ptr(x), float(tol), int(max_outer),
+                                float(inner_tol), int(inner_max), _ptr(ws), nbytes,
+                                C.byref(res), _stream())
+    _sync()
+    _lib.check(rc, "cg_solve_mixed")
+    return {"outer": int(res.outer), "inner_total": int(res.inner_total),
+            "iters": int(res.inner_total), "residual": float(res.true_residual),
+            "converged": bool(res.converged)}
+
+
+# ------------------------------------------------------ one rank's share --
+
+class DistCrsMatrix:
+    """One rank's share of the 1-D row partition (crs_matrix.hpp:101-114).
+
+    ``local`` is the remapped local matrix ([own | ghost] columns,
+    dist_spmv.hpp:62-82); ``ghost_ids`` the global id of every ghost slot in
+    slot order; ``send_ids`` / ``send_idx`` the rows this rank sends each
+    neighbour (halo_exchange.hpp:154-191); ``row_runs`` the interior and
+    boundary row ranges of the split SpMV.
+    """
+
+    def __init__(self):
+        raise TypeError("DistCrsMatrix objects are created by distribute_csr()")
+
+    @property
+    def global_nrows(self) -> int:
+        return self._global_nrows
+
+    @property
+    def local_row_start(self) -> int:
+        return self._start
+
+    @property
+    def local_row_end(self) -> int:
+        return self._end
+
+    @property
+    def n_ghost(self) -> int:
+        return int(self.ghost_ids.shape[0])
+
+    def ghost_values(self, x_global) -> torch.Tensor:
+        """The ghost block a halo exchange would deliver: x_global[ghost_ids]."""
+        if isinstance(x_global, torch.Tensor):
+            return x_global.index_select(0, self._ghost_idx_dev)
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(x_global)[self.ghost_ids])).to(
+            self.local._values.device)
+
+    def __repr__(self) -> str:
+        return (f"<DistCrsMatrix rank={self.rank}/{self.size} rows=[{self._start}, {self._end}) "
+                f"ghosts={self.n_ghost}>")
+
+
+def _runs(mask: np.ndarray):
+    """Maximal runs of True in mask -> [(begin, end)]."""
+    m = np.concatenate([[False], mask, [False]]).astype(np.int8)
+    d = np.diff(m)
+    return list(zip(np.nonzero(d == 1)[0].tolist(), np.nonzero(d == -1)[0].tolist()))
+
+
+def distribute_csr(rowptr, colind, values, global_nrows: int, global_ncols: int, *,
+                   rank: int | None = None, size: int | None = None) -> DistCrsMatrix:
+    """Build this rank's local matrix exactly as distribute_csr does
+    (dist_spmv.hpp:27-100; ghost order of halo_exchange.hpp:62-75,106-128).
+
+    ``rank``/``size`` default to this process's rank and world size; passing
+    them builds any rank's share on this one GPU (one GPU's share of an
+    N-slab run at its real shape).
+    """
+    from .generators import slab_share
+    _ensure_init()
+    rank = globals()["rank"]() if rank is None else int(rank)
+    size = globals()["size"]() if size is None else int(size)
+    if not (0 <= rank < size):
+        raise ValueError(f"rank {rank} outside [0, {size})")
+    rp = _host_array(rowptr, "rowptr", np.int32)[: int(global_nrows) + 1]
+    ci = _host_array(colind, "colind", np.int32)
+    vv = _host_array(values, "values", np.float64)
+    sh = slab_share(rp, ci, vv, int(global_nrows), int(global_ncols), rank, size)
+    self = object.__new__(DistCrsMatrix)
+    self.rank, self.size = rank, size
+    self._global_nrows = int(global_nrows)
+    self._start, self._end = sh.local_row_start, sh.local_row_end
+    self.local = upload_csr(sh.local.rowptr, sh.local.colind, sh.local.values, sh.local.nrows,
+                            sh.local.ncols)
+    self.ghost_ids = sh.ghost_ids
+    self.recv = sh.recv
+    dev = self.local._values.device
+    self._ghost_idx_dev = torch.from_numpy(sh.ghost_ids).to(dev)
+    n_local = sh.local.nrows
+    boundary = np.zeros(n_local, dtype=bool)
+    boundary[sh.boundary_rows] = True
+    self.interior_runs = _runs(~boundary)
+    self.boundary_runs = _runs(boundary)
+    self.interior_rows = sh.interior_rows
+    self.boundary_rows = sh.boundary_rows
+    # rows every neighbour needs from us, by rank ascending, ids ascending
+    send = []
+    j0, j1 = int(rp[self._start]), int(rp[self._end])
+    cols_all = ci[: int(rp[-1])]
+    owner_rows = np.repeat(np.arange(int(global_nrows), dtype=np.int64), np.diff(rp))
+    mine = (cols_all >= self._start) & (cols_all < self._end)
+    mine[j0:j1] = False
+    if mine.any():
+        from .generators import row_split
+        starts = np.array([row_split(int(global_nrows), r, size)[0] for r in range(size)])
+        req_rank = np.searchsorted(starts, owner_rows[mine], side="right") - 1
+        req_col = cols_all[mine].astype(np.int64)
+        for r in np.unique(req_rank):
+            ids = np.unique(req_col[req_rank == r])
+            send.append((int(r), ids))
+    self.send = send
+    idx = (np.concatenate([ids for _, ids in send]) - self._start).astype(np.int32) \
+        if send else np.zeros(0, np.int32)
+    self.send_idx = torch.from_numpy(idx).to(dev)
+    return self
+
+
+def dist_spmv(A_dist: DistCrsMatrix, x_local, y_local, x_ghost=None, *,
+              sync: bool = True) -> None:
+    """y_local = (A x)[own rows] (bindings.cpp:218-236, dist_spmv.hpp:118-163).
+
+    Runs the interior rows (no ghost columns) on x_local, then the boundary
+    rows on [x_local | x_ghost] without building x_ext.  With more than one
+    rank the ghost block comes from the halo exchange, which is not run on one
+    GPU: pass ``x_ghost`` (e.g. ``A_dist.ghost_values(x_global)``).
+    """
+    A = A_dist.local
+    x_local = _dev_tensor(x_local, "x_local", 1)
+    y_local = _dev_tensor(y_local, "y_local", 1)
+    if x_local.shape[0] != A.nrows or y_local.shape[0] != A.nrows:
+        raise ValueError("dist_spmv: x_local and y_local must have the local row count")
+    ng = A.ncols - A.nrows
+    if ng > 0:
+        if x_ghost is None:
+            raise RuntimeError("dist_spmv: this share has ghost columns; the halo exchange "
+                               "needs the other ranks - pass x_ghost")
+        x_ghost = _dev_tensor(x_ghost, "x_ghost", 1)
+        if x_ghost.shape[0] != ng:
+            raise ValueError(f"dist_spmv: x_ghost must have {ng} entries")
+    st = _stream()
+    for b, e in A_dist.interior_runs:
+        call("num_spmv_rows", A._h, int(b), int(e), _ptr(x_local), _ptr(y_local), st)
+    gp = _ptr(x_ghost) if ng > 0 else C.c_void_p(0)
+    for b, e in A_dist.boundary_runs:
+        call("num_spmv_split", A._h, int(b), int(e), _ptr(x_local), gp, _ptr(y_local), st)
+    if sync:
+        _sync()
+
+
+def halo_pack(A_dist: DistCrsMatrix, x_local, send_buf=None, *, sync: bool = True):
+    """send_buf[i] = x_local[send_idx[i]] (halo_exchange.hpp:222-235)."""
+    x_local = _dev_tensor(x_local, "x_local", 1)
+    n = int(A_dist.send_idx.shape[0])
+    if send_buf is None:
+        send_buf = torch.empty(n, dtype=torch.float64, device=x_local.device)
+    send_buf = _dev_tensor(send_buf, "send_buf", 1)
+    if send_buf.shape[0] < n:
+        raise ValueError("halo_pack: send_buf too small")
+    call("num_halo_pack", _ptr(A_dist.send_idx), n, _ptr(x_local), _ptr(send_buf), _stream())
+    if sync:
+        _sync()
+    return send_buf
+
+
+def cg_solve(A_dist: DistCrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000) -> dict:
+    """Distributed CG (bindings.cpp:239-270).  On one process (size 1) this is
+    cg_solve_local on the whole matrix; a share of a multi-rank partition needs
+    the other ranks' halo and allreduce and raises RuntimeError."""
+    if A_dist.size != 1:
+        raise RuntimeError("cg_solve: the share of a multi-rank partition needs the halo "
+                           "exchange and allreduce of the other ranks")
+    return cg_solve_local(A_dist.local, b, x, tol, max_iter)

@@ -1,0 +1,167 @@
+"""GPU parity for cg_solve_local and the mixed-precision refined solver.
+
+Known-answer tests follow tests/test_cg.cpp [1]-[7], [9] of the reference;
+parity against the oracle (the reference's CG sequence restated in C) is:
+same convergence flag, iteration count within +-2%, both
+recursive residuals below tol, and solutions within what the conditioning
+allows.
+"""
+import numpy as np
+import pytest
+import torch
+
+from num_amd import generators as g
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _up(ks, A):
+    return ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+
+
+def _solve(ks, A, b, tol, max_iter, x0=None):
+    dA = _up(ks, A)
+    x = _dev(np.zeros(A.nrows) if x0 is None else x0)
+    r = ks.cg_solve_local(dA, _dev(b), x, tol=tol, max_iter=max_iter)
+    return x.cpu().numpy(), r
+
+
+def rel(a, b):
+    d = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (d if d else 1.0)
+
+
+def _csr(rows):
+    rp, ci, v = [0], [], []
+    for row in rows:
+        for c, val in row:
+            ci.append(c)
+            v.append(val)
+        rp.append(len(ci))
+    n = len(rows)
+    return g.CSR(np.array(rp, np.int32), np.array(ci, np.int32), np.array(v, np.float64), n, n)
+
+
+def test_identity_cg(ks):  # [1]
+    N = 50
+    A = _csr([[(i, 1.0)] for i in range(N)])
+    b = np.arange(1, N + 1, dtype=np.float64)
+    x, r = _solve(ks, A, b, 1e-12, 200)
+    assert rel(x, b) < 1e-10 and r["converged"] and r["iters"] <= 2
+
+
+def test_2x2_spd(ks):  # [2]
+    A = _csr([[(0, 4.0), (1, 1.0)], [(0, 1.0), (1, 3.0)]])
+    x, r = _solve(ks, A, np.array([1.0, 2.0]), 1e-14, 100)
+    assert rel(x, np.array([1 / 11, 7 / 11])) < 1e-10 and r["converged"] and r["iters"] <= 2
+
+
+def test_diagonal_spd(ks):  # [3]
+    N = 100
+    d = np.arange(1, N + 1, dtype=np.float64)
+    A = _csr([[(i, d[i])] for i in range(N)])
+    x, r = _solve(ks, A, np.ones(N), 1e-12, 5000)
+    assert rel(x, 1 / d) < 1e-8 and r["converged"]
+    # n distinct eigenvalues -> CG terminates in <= n steps in exact arithmetic
+    assert r["iters"] <= N
+
+
+def test_laplacian_1d_convergence(ks, orc):  # [4]
+    A = g.laplacian_1d(50)
+    x, r = _solve(ks, A, np.ones(50), 1e-10, 500)
+    assert r["converged"] and r["residual"] < 1e-10 and r["iters"] > 0
+    # symmetric b on a symmetric tridiagonal: 25 distinct active eigenvalues
+    assert r["iters"] <= 25 + 2
+    _, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, np.ones(50), tol=1e-10, max_iter=500)
+    assert r["iters"] == ro["iters"]
+
+
+def test_zero_rhs(ks):  # [5]
+    A = g.laplacian_1d(20)
+    x, r = _solve(ks, A, np.zeros(20), 1e-10, 100)
+    assert r["converged"] and r["iters"] == 0 and r["residual"] == 0.0
+    assert np.all(x == 0.0)
+
+
+def test_manufactured_sine(ks):  # [6]
+    N = 500
+    A = g.laplacian_1d(N)
+    xs = np.sin(np.pi * (np.arange(N) + 1.0) / (N + 1.0))
+    b = 2.0 * xs
+    b[1:] -= xs[:-1]
+    b[:-1] -= xs[1:]
+    x, r = _solve(ks, A, b, 1e-10, 2000)
+    assert r["converged"] and rel(x, xs) < 1e-8
+
+
+def test_max_iter_respected(ks):  # [7]
+    A = g.laplacian_1d(50)
+    x, r = _solve(ks, A, np.ones(50), 1e-14, 5)
+    assert r["iters"] <= 5 and not r["converged"]
+    assert r["iters"] == 5
+
+
+def test_repeated_solves_bit_identical(ks):  # [9]
+    A = g.laplacian_1d(30)
+    sols = [_solve(ks, A, np.ones(30), 1e-12, 500)[0] for _ in range(3)]
+    assert np.array_equal(sols[0], sols[1]) and np.array_equal(sols[0], sols[2])
+
+
+def test_breakdown_guard(ks):
+    # singular 2x2 with b in the null space direction: p'Ap == 0 -> RuntimeError
+    A = _csr([[(0, 1.0), (1, -1.0)], [(0, -1.0), (1, 1.0)]])
+    dA = _up(ks, A)
+    with pytest.raises(RuntimeError):
+        ks.cg_solve_local(dA, _dev(np.array([1.0, 1.0])), _dev(np.zeros(2)), 1e-10, 10)
+
+
+def test_warm_start(ks, orc):
+    A = g.laplacian_2d(32)
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    x0 = np.random.default_rng(1).uniform(-1, 1, A.nrows)
+    x, r = _solve(ks, A, b, 1e-10, 2000, x0=x0)
+    xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, x0=x0, tol=1e-10, max_iter=2000)
+    assert r["converged"] and ro["converged"]
+    assert abs(r["iters"] - ro["iters"]) <= max(1, 0.02 * ro["iters"])
+
+
+def test_config_cpu_path_parity(ks, orc):
+    """2-D 256^2 Laplacian, b = U(-1,1) seed 0, x0 = 0, tol 1e-8, 2000 max."""
+    A = g.laplacian_2d(256)
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    x, r = _solve(ks, A, b, 1e-8, 2000)
+    xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-8, max_iter=2000)
+    assert r["converged"] and ro["converged"]
+    assert abs(r["iters"] - ro["iters"]) <= max(1, int(0.02 * ro["iters"]))
+    assert r["residual"] < 1e-8
+    true = np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b)
+    assert true < 2e-8
+    assert rel(x, xo) < 1e-6
+
+
+def test_vs_scipy_cg(ks):
+    import scipy.sparse.linalg as spla
+    A = g.laplacian_3d(20)
+    b = np.ones(A.nrows)
+    x_ref, info = spla.cg(A.to_scipy(), b, rtol=1e-12, maxiter=5000)
+    assert info == 0
+    x, r = _solve(ks, A, b, 1e-10, 5000)
+    assert r["converged"] and rel(x, x_ref) < 1e-6  # tests/test_accuracy.py:193
+
+
+def test_mixed_refined(ks, orc):
+    A = g.diffusion_3d(24, sigma=1.0, seed=3)
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    dA = _up(ks, A)
+    x = _dev(np.zeros(A.nrows))
+    r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+    assert r["converged"] and r["residual"] < 1e-10
+    xh = x.cpu().numpy()
+    true = np.linalg.norm(b - A.to_scipy() @ xh) / np.linalg.norm(b)
+    assert true < 1e-10
+    x64, r64 = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=20000)
+    assert rel(xh, x64) < 1e-7

@@ -1,0 +1,242 @@
+"""GPU parity: SpMV, FP32 SpMV, multi-RHS SpMV and the vector ops through the
+C ABI, against the CPU oracle on the same seeded inputs.
+
+Bar: the row-block kernel sums a row in index order from 0.0 whenever its
+block holds >= 128 rows (all stencil and LCG matrices here), so those results
+must be bit-identical to the oracle's sequential row loop
+(src/spmv_kernel.hpp:90-97).  Matrices with long rows are checked to 1e-12
+relative (the target of the reference's LCG test, tests/test_spmv_kernel.cpp:205).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from num_amd import generators as g
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _up(ks, A):
+    return ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+
+
+def _spmv(ks, dA, x):
+    y = torch.full((dA.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    ks.spmv(dA, _dev(x), y)
+    return y.cpu().numpy()
+
+
+def rel(a, b):
+    d = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (d if d else 1.0)
+
+
+# -- known-answer tests of tests/test_spmv_kernel.cpp ------------------------
+
+def test_tridiag4_kat(ks):
+    # [1] and [2]: A = tridiag(-1, 2, -1) 4x4, x = [1,2,3,4] -> [0,0,0,5]
+    rp = np.array([0, 2, 5, 8, 10], np.int32)
+    ci = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3], np.int32)
+    v = np.array([2, -1, -1, 2, -1, -1, 2, -1, -1, 2], np.float64)
+    dA = ks.upload_csr(rp, ci, v, 4, 4)
+    for kk in (True, False):
+        y = torch.zeros(4, dtype=torch.float64, device="cuda")
+        ks.spmv(dA, _dev(np.array([1.0, 2, 3, 4])), y, kokkoskernels=kk)
+        assert y.cpu().numpy().tolist() == [0.0, 0.0, 0.0, 5.0]
+    assert repr(dA) == "<CrsMatrix nrows=4 ncols=4 nnz=10>"
+
+
+def test_identity_kat(ks):
+    # [3]: identity N=100, x_i = 0.1 i -> y == x exactly
+    N = 100
+    rp = np.arange(N + 1, dtype=np.int32)
+    dA = ks.upload_csr(rp, np.arange(N, dtype=np.int32), np.ones(N), N, N)
+    x = np.arange(N) * 0.1
+    assert np.array_equal(_spmv(ks, dA, x), x)
+
+
+def test_lcg_kat_bitexact(ks, orc):
+    # [5]: LCG N=1e4 matrix, unsorted and repeated columns
+    A, x = g.lcg_matrix()
+    y = _spmv(ks, _up(ks, A), x)
+    y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+    assert np.array_equal(y, y_ref)
+    assert np.max(np.abs(y - y_ref)) / np.max(np.abs(y_ref)) < 1e-12
+
+
+@pytest.mark.parametrize("L", [1, 2, 7, 64, 256])
+def test_laplacian_2d_bitexact(ks, orc, L):
+    A = g.laplacian_2d(L)
+    x = np.random.default_rng(1).uniform(-1, 1, A.ncols)
+    assert np.array_equal(_spmv(ks, _up(ks, A), x), orc.spmv(A.rowptr, A.colind, A.values, x))
+
+
+def test_laplacian_3d_closed_forms(ks):
+    # L * ones = number of missing neighbours; L * sine mode = lambda * mode
+    L = 24
+    A = g.laplacian_3d(L)
+    dA = _up(ks, A)
+    y = _spmv(ks, dA, np.ones(A.ncols))
+    idx = np.arange(A.nrows)
+    c = [idx % L, (idx // L) % L, idx // (L * L)]
+    deg = sum((ci > 0).astype(int) + (ci < L - 1).astype(int) for ci in c)
+    assert np.array_equal(y, (6 - deg).astype(np.float64))
+    k = (1, 2, 3)
+    s = [np.sin(np.pi * kk * (ci + 1) / (L + 1)) for kk, ci in zip(k, c)]
+    mode = s[0] * s[1] * s[2]
+    lam = sum(2 - 2 * np.cos(np.pi * kk / (L + 1)) for kk in k)
+    assert rel(_spmv(ks, dA, mode), lam * mode) < 1e-13
+
+
+def test_zipf_long_rows(ks, orc):
+    A = g.zipf_matrix(n=30000, kmax=4096, seed=7)
+    info = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols).schedule_info()
+    assert info["max_row_nnz"] > 2048 and info["long_rows"] > 0
+    x = np.random.default_rng(3).uniform(-1, 1, A.ncols)
+    y = _spmv(ks, _up(ks, A), x)
+    y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+    assert rel(y, y_ref) < 1e-12
+
+
+@pytest.mark.parametrize("sched", [(256, 2048), (64, 512), (16, 4096), (256, 1024), (1, 1)])
+def test_schedules_agree(ks, orc, sched):
+    A = g.zipf_matrix(n=5000, kmax=600, seed=11)
+    dA = _up(ks, A)
+    dA.set_schedule(*sched)
+    x = np.random.default_rng(4).uniform(-1, 1, A.ncols)
+    assert rel(_spmv(ks, dA, x), orc.spmv(A.rowptr, A.colind, A.values, x)) < 1e-12
+
+
+def test_empty_rows_and_rect(ks, orc):
+    rng = np.random.default_rng(5)
+    nrows, ncols = 3000, 777
+    k = rng.integers(0, 5, nrows)
+    k[::7] = 0
+    rp = np.concatenate([[0], np.cumsum(k)]).astype(np.int32)
+    ci = rng.integers(0, ncols, int(rp[-1])).astype(np.int32)
+    v = rng.uniform(-1, 1, int(rp[-1]))
+    dA = ks.upload_csr(rp, ci, v, nrows, ncols)
+    x = rng.uniform(-1, 1, ncols)
+    y = _spmv(ks, dA, x)
+    assert np.array_equal(y, orc.spmv(rp, ci, v, x))
+    assert np.all(y[::7] == 0.0)
+
+
+def test_spmv_f32(ks, orc):
+    A = g.diffusion_3d(20, seed=3)
+    dA = _up(ks, A)
+    dA.ensure_f32()
+    x = np.random.default_rng(2).uniform(-1, 1, A.ncols).astype(np.float32)
+    y = torch.zeros(A.nrows, dtype=torch.float32, device="cuda")
+    from num_amd import _lib
+    import ctypes as C
+    _lib.call("num_spmv_f32", dA.handle, C.c_void_p(_dev(x).data_ptr()),
+              C.c_void_p(y.data_ptr()), None)
+    torch.cuda.synchronize()
+    y_ref = orc.spmv_f32(A.rowptr, A.colind, A.values.astype(np.float32), x)
+    assert np.array_equal(y.cpu().numpy(), y_ref)
+
+
+def test_shape_errors(ks):
+    A = g.laplacian_2d(4)
+    dA = _up(ks, A)
+    with pytest.raises(ValueError):
+        ks.spmv(dA, torch.zeros(15, dtype=torch.float64, device="cuda"),
+                torch.zeros(16, dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        ks.spmv(dA, torch.zeros(16, dtype=torch.float64, device="cuda"),
+                torch.zeros(17, dtype=torch.float64, device="cuda"))
+    with pytest.raises(TypeError):
+        ks.spmv(dA, torch.zeros(16, dtype=torch.float32, device="cuda"),
+                torch.zeros(16, dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        ks.upload_csr(A.rowptr, A.colind, A.values, 0, 16)
+
+
+# -- multi-RHS (src/batch_spmv_kernel.hpp) -----------------------------------
+
+def test_batch_identity_kat(ks):
+    # [4]: tridiag 4x4, X = I4 -> Y[:, b] = A[:, b]
+    rp = np.array([0, 2, 5, 8, 10], np.int32)
+    ci = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3], np.int32)
+    v = np.array([2, -1, -1, 2, -1, -1, 2, -1, -1, 2], np.float64)
+    dA = ks.upload_csr(rp, ci, v, 4, 4)
+    Y = torch.zeros((4, 4), dtype=torch.float64, device="cuda")
+    ks.batch_spmv(dA, torch.eye(4, dtype=torch.float64, device="cuda"), Y)
+    dense = np.zeros((4, 4))
+    for i in range(4):
+        for j in range(rp[i], rp[i + 1]):
+            dense[i, ci[j]] = v[j]
+    assert np.array_equal(Y.cpu().numpy(), dense)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 7, 16, 33, 64, 100])
+def test_batch_spmv(ks, orc, k):
+    A = g.laplacian_3d(12)
+    dA = _up(ks, A)
+    X = np.random.default_rng(k).standard_normal((A.ncols, k))
+    Y = torch.zeros((A.nrows, k), dtype=torch.float64, device="cuda")
+    ks.batch_spmv(dA, _dev(X), Y)
+    Y_ref = orc.batch_spmv(A.rowptr, A.colind, A.values, X)
+    assert np.array_equal(Y.cpu().numpy(), Y_ref)
+    y1 = orc.spmv(A.rowptr, A.colind, A.values, X[:, 0])
+    assert np.array_equal(Y.cpu().numpy()[:, 0], y1)
+
+
+def test_batch_shape_errors(ks):
+    A = g.laplacian_2d(4)
+    dA = _up(ks, A)
+    with pytest.raises(ValueError):
+        ks.batch_spmv(dA, torch.zeros((15, 3), dtype=torch.float64, device="cuda"),
+                      torch.zeros((16, 3), dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        ks.batch_spmv(dA, torch.zeros((16, 3), dtype=torch.float64, device="cuda"),
+                      torch.zeros((16, 2), dtype=torch.float64, device="cuda"))
+
+
+# -- vector ops (src/vec_ops.hpp) ---------------------------------------------
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 1000, 123457, 3_000_001])
+def test_vec_ops(ks, orc, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n)
+    y = rng.uniform(-1, 1, n)
+    dx, dy = _dev(x), _dev(y)
+    z = torch.empty(n, dtype=torch.float64, device="cuda")
+    ks.axpy(0.37, dx, dy, z)
+    assert np.array_equal(z.cpu().numpy(), orc.axpy(0.37, x, y))
+    ks.axpy2(-1.5, dx, 0.25, dy, z)
+    assert np.array_equal(z.cpu().numpy(), orc.axpy2(-1.5, x, 0.25, y))
+    w = dx.clone()
+    ks.scale(w, 3.0)
+    assert np.array_equal(w.cpu().numpy(), orc.scale(x, 3.0))
+    ks.fill(w, 2.5)
+    assert np.all(w.cpu().numpy() == 2.5)
+    ks.copy(dy, w)
+    assert np.array_equal(w.cpu().numpy(), y)
+    d = ks.dot(dx, dy)
+    exact = math.fsum((x * y).tolist()) if n else 0.0
+    assert abs(d - exact) <= 1e-14 * max(1.0, float(np.abs(x * y).sum()))
+    nn = ks.norm2(dx)
+    assert abs(nn - math.sqrt(math.fsum((x * x).tolist()))) <= 1e-14 * max(1.0, nn)
+
+
+def test_dot_integer_exact(ks):
+    n = 1_000_003
+    x = (np.arange(n) % 17 - 8).astype(np.float64)
+    y = (np.arange(n) % 5 - 2).astype(np.float64)
+    assert ks.dot(_dev(x), _dev(y)) == float(np.dot(x.astype(np.int64), y.astype(np.int64)))
+
+
+def test_axpy_aliasing(ks, orc):
+    x = np.random.default_rng(9).uniform(-1, 1, 5000)
+    y = np.random.default_rng(10).uniform(-1, 1, 5000)
+    dx, dy = _dev(x), _dev(y)
+    ks.axpy(2.0, dx, dy, dy)  # z aliases y (vec_ops.hpp:14)
+    assert np.array_equal(dy.cpu().numpy(), orc.axpy(2.0, x, y))
