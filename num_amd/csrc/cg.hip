@@ -17,7 +17,7 @@
 
 #include "blas1.hpp"
 #include "csr.hpp"
-#include "spmv_device.hpp"
+#include "spmv_launch.hpp"
 
 namespace numamd {
 
@@ -212,23 +212,7 @@ template <typename T, class Epi>
 static void launch_fused_spmv(const num_csr* A, const T* vals, const T* x, const Epi& epi,
                               hipStream_t stream)
 {
-    const Schedule& s = A->sched;
-    const int per = xcd_per(s.nblocks);
-    const int grid = xcd_grid(s.nblocks);
-    switch (s.npt) {
-    case 4:
-        spmv_rowblock_kernel<T, kSpmvBS, 4, Epi><<<grid, kSpmvBS, 0, stream>>>(
-            A->rowptr, A->colind, vals, GatherX<T>{x}, s.d_blk_row, s.nblocks, per, epi);
-        break;
-    case 8:
-        spmv_rowblock_kernel<T, kSpmvBS, 8, Epi><<<grid, kSpmvBS, 0, stream>>>(
-            A->rowptr, A->colind, vals, GatherX<T>{x}, s.d_blk_row, s.nblocks, per, epi);
-        break;
-    default:
-        spmv_rowblock_kernel<T, kSpmvBS, 16, Epi><<<grid, kSpmvBS, 0, stream>>>(
-            A->rowptr, A->colind, vals, GatherX<T>{x}, s.d_blk_row, s.nblocks, per, epi);
-        break;
-    }
+    launch_tiles<T>(A, A->sched, vals, GatherX<T>{x}, epi, stream);
 }
 
 // Pinned status slots for the asynchronous poll (one pair per host thread).

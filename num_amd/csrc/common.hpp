@@ -42,16 +42,34 @@ constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, workgroups dealt round-rob
 constexpr int kRedBlock = 256;     // block size of elementwise / reduction kernels
 constexpr int kMaxRedBlocks = 2048;  // cap on per-launch partial sums
 
-// Workgroup index -> logical tile so that each XCD (private L2) walks one
-// contiguous stretch of rows: hardware block b runs on XCD b % 8, and gets
-// logical tile (b % 8) * per + b / 8.  Returns -1 for the padding blocks.
-__device__ __forceinline__ int xcd_tile(int b, int ntiles, int per)
+// Workgroup index -> tile.  Workgroup b runs on XCD b % 8 (round-robin
+// dispatch); tiles are cut into chunks of `chunk` consecutive tiles and chunk
+// ch goes to XCD ch % 8, so every XCD (with its private L2) walks runs of
+// neighbouring rows while all XCDs stay close together in the matrix.
+// chunk = 1 is the plain hardware order.  Returns -1 for padding workgroups.
+__device__ __forceinline__ int chunk_tile(int b, int ntiles, int chunk)
 {
-    const int t = (b % kNumXcd) * per + b / kNumXcd;
+    const int k = b % kNumXcd;
+    const int q = b / kNumXcd;  // q-th workgroup this XCD receives
+    const int t = ((q / chunk) * kNumXcd + k) * chunk + (q % chunk);
     return t < ntiles ? t : -1;
 }
-inline int xcd_per(int ntiles) { return (ntiles + kNumXcd - 1) / kNumXcd; }
-inline int xcd_grid(int ntiles) { return xcd_per(ntiles) * kNumXcd; }
+// Chunk size used for a launch of `ntiles` tiles and the grid covering it.
+inline int tile_chunk(int ntiles) { return ntiles >= 4096 ? 16 : 1; }
+inline int chunk_grid(int ntiles, int chunk)
+{
+    const int nchunks = (ntiles + chunk - 1) / chunk;
+    return ((nchunks + kNumXcd - 1) / kNumXcd) * kNumXcd * chunk;
+}
+
+// Schedule entry of one SpMV row tile; the entry after the last tile holds
+// {r_end, 0, rowptr[r_end], 0}.
+struct TileEntry {
+    int32_t row;    // first row
+    int32_t lanes;  // lanes per row in the walk phase (power of two <= 64)
+    int32_t nz0;    // rowptr[row]
+    int32_t pad;
+};
 
 // ------------------------------------------------------------ reductions ---
 // Deterministic butterfly sum over a whole wave64: every lane ends with the

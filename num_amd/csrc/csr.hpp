@@ -11,17 +11,21 @@ namespace numamd {
 
 constexpr int kSpmvBS = 256;  // threads per row-block workgroup
 
-// Row-block schedule over rows [row_begin, row_end): consecutive rows packed
-// greedily into blocks of at most rows_per_block rows and nnz_per_block
-// non-zeros; a row with more than nnz_per_block non-zeros is a block alone.
+// LDS entries per thread the tile kernel is instantiated for.
+constexpr int kNptChoices[] = {5, 7, 12, 16};
+constexpr int kNptMax = 16;
+
+// Row-tile schedule over rows [row_begin, row_end): consecutive rows packed
+// greedily into tiles of at most rows_per_block rows and nnz_per_block
+// non-zeros; a row with more than nnz_per_block non-zeros is a tile alone.
 struct Schedule {
     int32_t row_begin = 0, row_end = 0;
     int32_t rows_per_block = 0, nnz_per_block = 0;
-    int32_t npt = 0;             // LDS slots per thread the kernel is built for
-    int32_t nblocks = 0;
-    int32_t long_rows = 0;       // single-row blocks longer than nnz_per_block
-    int32_t* d_blk_row = nullptr;  // device [2 * (nblocks + 1)]: [first row, lanes] pairs
-    std::vector<int32_t> h_blk_row;
+    int32_t npt = 0;             // LDS entries per thread (kNptChoices)
+    int32_t nblocks = 0;         // tiles
+    int32_t long_rows = 0;       // single-row tiles longer than nnz_per_block
+    int32_t chunk = 1;           // XCD chunk size for chunk_tile
+    void* d_tiles = nullptr;     // device TileEntry[nblocks + 1]
 };
 
 }  // namespace numamd
@@ -47,6 +51,9 @@ namespace numamd {
 // Builds `s` for rows [r0, r1) of A and uploads it.  Returns NUM_OK or error.
 int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
                    int32_t nnz_per_block, Schedule& s);
+// Default tile capacity for A: the smallest kNptChoices entry covering the
+// mean row length, times the block size.
+int32_t default_nnz_per_block(const num_csr* A);
 void free_schedule(Schedule& s);
 // Schedule for a row range (cached per range); whole range -> A->sched.
 int range_schedule(const num_csr* A, int32_t r0, int32_t r1, const Schedule** out);

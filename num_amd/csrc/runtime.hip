@@ -41,16 +41,23 @@ int check_device(const num_csr* A)
 
 static int pick_npt(int32_t nnz_per_block)
 {
-    const int need = (nnz_per_block + kSpmvBS - 1) / kSpmvBS;
-    if (need <= 4) return 4;
-    if (need <= 8) return 8;
-    return 16;
+    for (int c : kNptChoices)
+        if (c * kSpmvBS >= nnz_per_block) return c;
+    return kNptMax;
 }
 
-// Lanes per row in the reduce phase: 1 (sequential, bit-identical to the
-// row loop) while the longest row has <= 32 entries, else enough lanes for
-// ~16 entries each, capped by the wave width and by BS / rows.
-static int32_t reduce_lanes(int32_t maxlen, int32_t nrows_blk)
+int32_t default_nnz_per_block(const num_csr* A)
+{
+    const double avg = A->nrows > 0 ? (double)A->nnz / (double)A->nrows : 1.0;
+    for (int c : kNptChoices)
+        if ((double)c >= avg) return c * kSpmvBS;
+    return kNptMax * kSpmvBS;
+}
+
+// Lanes per row in the walk phase: 1 (sequential, bit-identical to the row
+// loop) while the longest row has <= 32 entries, else enough lanes for ~16
+// entries each, capped by the wave width and by BS / rows.
+static int32_t walk_lanes(int32_t maxlen, int32_t nrows_blk)
 {
     if (maxlen <= 32) return 1;
     int32_t want = 1;
@@ -62,9 +69,8 @@ static int32_t reduce_lanes(int32_t maxlen, int32_t nrows_blk)
 
 void free_schedule(Schedule& s)
 {
-    if (s.d_blk_row) (void)hipFree(s.d_blk_row);
-    s.d_blk_row = nullptr;
-    s.h_blk_row.clear();
+    if (s.d_tiles) (void)hipFree(s.d_tiles);
+    s.d_tiles = nullptr;
     s.nblocks = 0;
 }
 
@@ -74,15 +80,14 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
     if (rows_per_block < 1 || rows_per_block > kSpmvBS)
         return fail(NUM_EINVAL, "rows_per_block must be in [1, %d], got %d", kSpmvBS,
                     rows_per_block);
-    if (nnz_per_block < 1 || nnz_per_block > 16 * kSpmvBS)
-        return fail(NUM_EINVAL, "nnz_per_block must be in [1, %d], got %d", 16 * kSpmvBS,
+    if (nnz_per_block < 1 || nnz_per_block > kNptMax * kSpmvBS)
+        return fail(NUM_EINVAL, "nnz_per_block must be in [1, %d], got %d", kNptMax * kSpmvBS,
                     nnz_per_block);
     if (r0 < 0 || r1 > A->nrows || r0 > r1)
         return fail(NUM_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, A->nrows);
     const int32_t* rp = A->h_rowptr.data();
-    // entries: [first row, lanes per row] per block, then [r1, 0]
-    std::vector<int32_t> blk;
-    blk.reserve(2 * ((size_t)(r1 - r0) / (size_t)std::max(1, rows_per_block / 2) + 8));
+    std::vector<TileEntry> tiles;
+    tiles.reserve((size_t)(r1 - r0) / (size_t)std::max(1, rows_per_block / 2) + 8);
     int32_t longs = 0;
     int32_t r = r0;
     while (r < r1) {
@@ -91,8 +96,7 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
         if (len > nnz_per_block) {
             ++r;
             ++longs;
-            blk.push_back(start);
-            blk.push_back(reduce_lanes(len, 1));
+            tiles.push_back({start, walk_lanes(len, 1), rp[start], 0});
             continue;
         }
         int64_t nz = 0;
@@ -104,29 +108,27 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
             maxlen = std::max(maxlen, l);
             ++r;
         }
-        blk.push_back(start);
-        blk.push_back(reduce_lanes(maxlen, r - start));
+        tiles.push_back({start, walk_lanes(maxlen, r - start), rp[start], 0});
     }
-    blk.push_back(r1);
-    blk.push_back(0);
+    tiles.push_back({r1, 0, rp[r1], 0});
     Schedule ns;
     ns.row_begin = r0;
     ns.row_end = r1;
     ns.rows_per_block = rows_per_block;
     ns.nnz_per_block = nnz_per_block;
     ns.npt = pick_npt(nnz_per_block);
-    ns.nblocks = (int32_t)blk.size() / 2 - 1;
+    ns.nblocks = (int32_t)tiles.size() - 1;
     ns.long_rows = longs;
-    const size_t bytes = blk.size() * sizeof(int32_t);
-    NUM_HIP(hipMalloc(&ns.d_blk_row, bytes));
-    hipError_t e = hipMemcpy(ns.d_blk_row, blk.data(), bytes, hipMemcpyHostToDevice);
+    ns.chunk = tile_chunk(ns.nblocks);
+    const size_t bytes = tiles.size() * sizeof(TileEntry);
+    NUM_HIP(hipMalloc(&ns.d_tiles, bytes));
+    hipError_t e = hipMemcpy(ns.d_tiles, tiles.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(ns.d_blk_row);
+        (void)hipFree(ns.d_tiles);
         return fail(NUM_EHIP, "schedule upload failed: %s", hipGetErrorString(e));
     }
-    ns.h_blk_row = std::move(blk);
     free_schedule(s);
-    s = std::move(ns);
+    s = ns;
     return NUM_OK;
 }
 
@@ -167,7 +169,7 @@ static int finish_create(num_csr* A)
     for (int32_t i = 0; i < A->nrows; ++i)
         mx = std::max(mx, A->h_rowptr[i + 1] - A->h_rowptr[i]);
     A->max_row_nnz = mx;
-    return build_schedule(A, 0, A->nrows, kSpmvBS, 8 * kSpmvBS, A->sched);
+    return build_schedule(A, 0, A->nrows, kSpmvBS, default_nnz_per_block(A), A->sched);
 }
 
 }  // namespace numamd

@@ -2,32 +2,15 @@
 // pack/scatter and the host round-trip helpers.
 #include <vector>
 
-#include "csr.hpp"
-#include "spmv_device.hpp"
+#include "spmv_launch.hpp"
 
 namespace numamd {
-
-template <typename T, int NPT, class Epi, class Gather>
-static void launch_rowblock(const num_csr* A, const Schedule& s, const T* vals, const Gather& x,
-                            const Epi& epi, hipStream_t stream)
-{
-    const int per = xcd_per(s.nblocks);
-    spmv_rowblock_kernel<T, kSpmvBS, NPT, Epi, Gather>
-        <<<xcd_grid(s.nblocks), kSpmvBS, 0, stream>>>(A->rowptr, A->colind, vals, x,
-                                                     s.d_blk_row, s.nblocks, per, epi);
-}
 
 template <typename T, class Gather>
 static int launch_spmv_g(const num_csr* A, const Schedule& s, const T* vals, const Gather& x,
                          T* y, hipStream_t stream)
 {
-    if (s.nblocks == 0) return NUM_OK;
-    EpiY<T> epi{y};
-    switch (s.npt) {
-    case 4: launch_rowblock<T, 4>(A, s, vals, x, epi, stream); break;
-    case 8: launch_rowblock<T, 8>(A, s, vals, x, epi, stream); break;
-    default: launch_rowblock<T, 16>(A, s, vals, x, epi, stream); break;
-    }
+    launch_tiles<T>(A, s, vals, x, EpiY<T>{y}, stream);
     NUM_LAUNCH_CHECK();
     return NUM_OK;
 }

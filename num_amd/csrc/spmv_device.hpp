@@ -1,32 +1,38 @@
-// spmv_device.hpp -- row-block CSR SpMV kernel for gfx950, shared by the plain
+// spmv_device.hpp -- row-tile CSR SpMV kernel for gfx950, shared by the plain
 // SpMV entry points and the fused CG kernels.
 //
-// Work unit: one workgroup of BS threads owns a block of consecutive rows
-// [blk_row[2t], blk_row[2t+2]) whose non-zeros fit the block's LDS budget
-// (BS*NPT entries).  Three phases:
-//   1. stream: every thread loads NPT non-zeros at stride BS, so each wave
-//      load instruction reads 64 consecutive values / column indices (fully
-//      coalesced whatever the row lengths), gathers x[col], and stores the
-//      rounded product v*x into LDS;
-//   2. reduce: G lanes per row (blk_row[2t+1]; G = 1 when no row of the block
-//      has more than 32 entries) sum the row's products from LDS; with G = 1
-//      the sum runs in index order from
-//      0.0, the exact arithmetic of the reference's row loop
-//      (src/spmv_kernel.hpp:90-97) -- bit-identical results;
+// Work unit: one workgroup of BS threads owns a tile of consecutive rows
+// [r0, r1) whose non-zeros [j0, j1) fit the tile's LDS budget (BS*NPT
+// entries).  Phases:
+//   1. stage: every thread issues NPT non-temporal loads of values and column
+//      indices at stride BS (each wave instruction reads 64 consecutive
+//      entries: fully coalesced whatever the row lengths) and parks them in
+//      LDS.  All loads are issued before any is used (clamped indices, no
+//      per-element branches), so each thread keeps 2*NPT loads in flight.
+//      The row bounds rowptr[r], rowptr[r+1] are fetched in the same batch.
+//   2. walk: G lanes per row (G = 1 unless the tile holds a row with more
+//      than 32 entries) walk the row's entries in LDS and gather x[col].  For
+//      stencil rows consecutive lanes own consecutive rows, so the gathers for
+//      one neighbour offset are contiguous across the wave.  With G = 1 the
+//      row is summed in index order from 0.0 with separately rounded products,
+//      the arithmetic of the reference's row loop (src/spmv_kernel.hpp:90-97):
+//      bit-identical results.
 //   3. epilogue: a functor consumes (row, value) -- store y, or the fused CG
-//      updates -- and can contribute to a deterministic per-block partial sum.
-// A row longer than the LDS budget gets a workgroup of its own ("long" block)
-// that accumulates at stride BS and reduces with a fixed butterfly.
-// Blocks are mapped XCD-contiguously (common.hpp: xcd_tile).
+//      updates -- and can contribute to a deterministic per-tile partial sum.
+// A row longer than the LDS budget gets a workgroup of its own ("long" tile)
+// that accumulates straight from global memory at stride BS and reduces with
+// a fixed butterfly.
+// Tiles are dealt to the 8 XCDs in chunks of consecutive tiles (common.hpp:
+// chunk_tile), so each XCD's L2 sees neighbouring x windows.
 #pragma once
 #include "common.hpp"
 
 namespace numamd {
 
 template <typename T>
-__device__ __forceinline__ T shfl_xor_w(T v, int off, int width)
+__device__ __forceinline__ T ld_stream(const T* p)
 {
-    return __shfl_xor(v, off, width);
+    return __builtin_nontemporal_load(p);
 }
 
 // x accessors: one vector, or the [own | ghost] pair of a rank's slab where
@@ -48,87 +54,135 @@ struct GatherSplit {
     }
 };
 
-__device__ __forceinline__ int pow2_floor(int v)
-{
-    return v <= 1 ? 1 : (1 << (31 - __clz(v)));
-}
-
-// Kernel body; `tile` is the logical row block.  Returns the two epilogue
-// contributions of this thread (zero for non-leader lanes).
+// Body for one tile.  Leaves this thread's two epilogue contributions in
+// ca/cb (zero for lanes that finish no row).
 template <typename T, int BS, int NPT, class Epi, class Gather>
-__device__ __forceinline__ void spmv_block_body(
+__device__ __forceinline__ void spmv_tile_body(
     int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-    const T* __restrict__ vals, const Gather& x, const int32_t* __restrict__ blk_row,
-    const Epi& epi, T* __restrict__ prod, double& ca, double& cb)
+    const T* __restrict__ vals, const Gather& x, const TileEntry* __restrict__ sched,
+    const Epi& epi, T* __restrict__ lv, int32_t* __restrict__ lc, double& ca, double& cb)
 {
     constexpr int NNZ_MAX = BS * NPT;
     const int t = threadIdx.x;
-    // schedule entries are [first row, lanes per row] pairs, plus a sentinel
-    const int r0 = blk_row[2 * tile];
-    const int G = blk_row[2 * tile + 1];
-    const int r1 = blk_row[2 * tile + 2];
-    const int j0 = rowptr[r0];
-    const int j1 = rowptr[r1];
+    const TileEntry e0 = sched[tile];
+    const TileEntry e1 = sched[tile + 1];
+    const int r0 = e0.row, G = e0.lanes, j0 = e0.nz0;
+    const int r1 = e1.row, j1 = e1.nz0;
     const int nz = j1 - j0;
     ca = 0.0;
     cb = 0.0;
     if (nz > NNZ_MAX) {
-        // long row: one row, whole workgroup, stride-BS partial sums then a
-        // fixed-shape tree (wave butterfly, then waves in order).
+        // long row: one row, whole workgroup; 4 independent loads in flight
+        // per thread, then a fixed-shape tree (wave butterfly, waves in order).
         T acc = T(0);
-        for (int j = j0 + t; j < j1; j += BS) acc += vals[j] * x(colind[j]);
+        int j = j0 + t;
+        for (; j + 3 * BS < j1; j += 4 * BS) {
+            T v[4];
+            int c[4];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc += shfl_xor_w(acc, off, 64);
-        if ((t & (kWave - 1)) == 0) prod[t / kWave] = acc;
+            for (int u = 0; u < 4; ++u) {
+                v[u] = ld_stream(vals + j + u * BS);
+                c[u] = ld_stream(colind + j + u * BS);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u] * x(c[u]);
+        }
+        for (; j < j1; j += BS) acc += vals[j] * x(colind[j]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if ((t & (kWave - 1)) == 0) lv[t / kWave] = acc;
         __syncthreads();
         if (t == 0) {
             T s = T(0);
 #pragma unroll
-            for (int w = 0; w < BS / kWave; ++w) s += prod[w];
+            for (int w = 0; w < BS / kWave; ++w) s += lv[w];
             epi(r0, s, ca, cb);
         }
         return;
     }
-    // 1. stream phase
-#pragma unroll
-    for (int s = 0; s < NPT; ++s) {
-        const int j = j0 + t + s * BS;
-        if (j < j1) prod[j - j0] = vals[j] * x(colind[j]);
-    }
-    __syncthreads();
-    // 2. reduce phase: G lanes per row, chosen by the scheduler from the
-    //    block's longest row (G = 1 whenever every row has <= 32 entries)
+    // row bounds for the walk phase, fetched together with the stage loads
     const int nr = r1 - r0;
     const int lr = t / G;
     const int lane = t & (G - 1);
-    T acc = T(0);
     const bool active = lr < nr;
+    int rb = 0, re = 0;
     if (active) {
-        const int rs = rowptr[r0 + lr] - j0;
-        const int re = rowptr[r0 + lr + 1] - j0;
-        for (int k = rs + lane; k < re; k += G) acc += prod[k];
+        rb = rowptr[r0 + lr] - j0;
+        re = rowptr[r0 + lr + 1] - j0;
+    }
+    // 1. stage
+    if (nz > 0) {
+        const int last = j1 - 1;
+        T v[NPT];
+        int c[NPT];
+#pragma unroll
+        for (int s = 0; s < NPT; ++s) {
+            const int j = min(j0 + t + s * BS, last);
+            v[s] = ld_stream(vals + j);
+            c[s] = ld_stream(colind + j);
+        }
+#pragma unroll
+        for (int s = 0; s < NPT; ++s) {
+            const int k = t + s * BS;
+            if (k < nz) {
+                lv[k] = v[s];
+                lc[k] = c[s];
+            }
+        }
+    }
+    __syncthreads();
+    // 2. walk: lane visits k = rb + lane, rb + lane + G, ...; 8 gathers in
+    //    flight before the (ordered) sums
+    T acc = T(0);
+    if (active) {
+        int k = rb + lane;
+        for (; k + 7 * G < re; k += 8 * G) {
+            T vv[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                vv[u] = lv[k + u * G];
+                xv[u] = x(lc[k + u * G]);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += vv[u] * xv[u];
+        }
+        T vv[7], xv[7];
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
+            const int kk = k + u * G;
+            const bool ok = kk < re;
+            const int ks = ok ? kk : 0;  // slot 0 is always inside the LDS array
+            vv[u] = lv[ks];
+            const int c = lc[ks];
+            xv[u] = x(ok ? c : 0);       // column 0 always exists
+        }
+#pragma unroll
+        for (int u = 0; u < 7; ++u)
+            if (k + u * G < re) acc += vv[u] * xv[u];
     }
     if (G > 1) {
-        for (int off = G >> 1; off > 0; off >>= 1) acc += shfl_xor_w(acc, off, G);
+        for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
     }
     if (active && lane == 0) epi(r0 + lr, acc, ca, cb);
 }
 
-// Generic kernel: per-block partial sums (if Epi::kRed) go to
+// Generic kernel: per-tile partial sums (if Epi::kRed) go to
 // epi.partial_a / partial_b[tile].
 template <typename T, int BS, int NPT, class Epi, class Gather = GatherX<T>>
-__global__ __launch_bounds__(BS) void spmv_rowblock_kernel(
+__global__ __launch_bounds__(BS) void spmv_tile_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-    const T* __restrict__ vals, Gather x, const int32_t* __restrict__ blk_row,
-    int ntiles, int per, Epi epi)
+    const T* __restrict__ vals, Gather x, const TileEntry* __restrict__ sched, int ntiles,
+    int chunk, Epi epi)
 {
-    __shared__ T prod[BS * NPT];
+    __shared__ T lv[BS * NPT];
+    __shared__ int32_t lc[BS * NPT];
     __shared__ double red[2 * BS / kWave];
-    const int tile = xcd_tile(blockIdx.x, ntiles, per);
+    const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
     if (tile < 0) return;
     if (epi.skip()) return;
     double ca, cb;
-    spmv_block_body<T, BS, NPT, Epi, Gather>(tile, rowptr, colind, vals, x, blk_row, epi, prod, ca, cb);
+    spmv_tile_body<T, BS, NPT, Epi, Gather>(tile, rowptr, colind, vals, x, sched, epi, lv, lc,
+                                            ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
         if (threadIdx.x == 0) epi.partial_a[tile] = s;
