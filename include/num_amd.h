@@ -46,8 +46,8 @@ typedef struct {
 
 /* Mixed-precision refined CG (no reference counterpart). */
 typedef struct {
-    int32_t outer;        /* FP64 refinement steps (residual evaluations - 1) */
-    int32_t inner_total;  /* FP32 CG iterations summed over all steps          */
+    int32_t outer;        /* FP64 true-residual evaluations (refinements)      */
+    int32_t inner_total;  /* FP32 CG steps                                     */
     int32_t converged;    /* 1 if the FP64 true residual reached tol          */
     int32_t breakdown;
     double true_residual; /* ||b - A x|| / ||b|| in FP64 at exit              */
@@ -80,12 +80,17 @@ int num_csr_attach_f32(num_csr* A, const float* d_vals32);
 int num_csr_destroy(num_csr* A);
 /* nrows / ncols / nnz read-only attributes (src/bindings.cpp:79-88). */
 int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz);
-/* Row-block schedule: at most rows_per_block rows and nnz_per_block
+/* Row-tile schedule: at most rows_per_block rows (<= 256) and nnz_per_block
  * non-zeros per workgroup (rows longer than nnz_per_block get a workgroup of
- * their own).  Rebuilds the schedule; used by the autotuner. */
-int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block);
+ * their own); tiles dealt to the 8 XCDs in chunks of `chunk` tiles; nt64 /
+ * nt32 = 1 streams the FP64 / FP32 values non-temporally.  Values <= 0 (for
+ * the sizes) or < 0 (chunk, nt64, nt32) pick the defaults.  Rebuilds the
+ * schedule; used by the autotuner. */
+int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block,
+                         int32_t chunk, int32_t nt64, int32_t nt32);
 int num_csr_schedule_info(const num_csr* A, int32_t* nblocks, int32_t* rows_per_block,
-                          int32_t* nnz_per_block, int32_t* long_rows, int32_t* max_row_nnz);
+                          int32_t* nnz_per_block, int32_t* long_rows, int32_t* max_row_nnz,
+                          int32_t* chunk, int32_t* nt64, int32_t* nt32);
 
 /* ---- SpMV (src/spmv_kernel.hpp:118-180) ---------------------------------- */
 /* y = A x.  Replaces both launch_spmv_kokkoskernels (spmv_kernel.hpp:170-180)
@@ -147,13 +152,16 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol,
 
 /* ---- mixed-precision refined CG (no reference counterpart) -------------- */
 int64_t num_cg_mixed_workspace_bytes(const num_csr* A);
-/* FP64 outer loop r = b - A x, FP32 inner CG (FP64-accumulated dots) on
- * A d = r to relative tolerance inner_tol, x += d in FP64; stops when the
- * FP64 true residual ||b - A x||/||b|| < tol.  Needs num_csr_attach_f32. */
+/* CG steps in FP32 (FP32 matrix copy and vectors, dot products accumulated
+ * in FP64) inside FP64 iterative refinement with reliable updates: whenever
+ * the FP32 residual has dropped by `delta` since the last refinement (or
+ * looks converged), x += x32 and the true residual r = b - A x are computed
+ * in FP64 on the GPU, the FP32 residual is re-seeded from r and the search
+ * direction is kept.  Stops when the FP64 true residual ||b - A x||/||b|| <
+ * tol (strict) or after max_iter FP32 steps.  Needs num_csr_attach_f32. */
 int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double tol,
-                       int32_t max_outer, double inner_tol, int32_t inner_max,
-                       void* d_work, int64_t work_bytes, num_mixed_result* res,
-                       void* stream);
+                       int32_t max_iter, double delta, void* d_work, int64_t work_bytes,
+                       num_mixed_result* res, void* stream);
 
 /* ---- halo pack / scatter of one rank's share (src/halo_exchange.hpp) --- */
 /* buf[i] = x[idx[i]] for i < count (halo_pack, halo_exchange.hpp:222-235). */

@@ -42,8 +42,8 @@ SIGNATURES = {
     "num_csr_attach_f32": (C.c_int, [vp, vp]),
     "num_csr_destroy": (C.c_int, [vp]),
     "num_csr_shape": (C.c_int, [vp, pi32, pi32, pi64]),
-    "num_csr_set_schedule": (C.c_int, [vp, i32, i32]),
-    "num_csr_schedule_info": (C.c_int, [vp, pi32, pi32, pi32, pi32, pi32]),
+    "num_csr_set_schedule": (C.c_int, [vp, i32, i32, i32, i32, i32]),
+    "num_csr_schedule_info": (C.c_int, [vp, pi32, pi32, pi32, pi32, pi32, pi32, pi32, pi32]),
     "num_spmv": (C.c_int, [vp, vp, vp, vp]),
     "num_spmv_rows": (C.c_int, [vp, i32, i32, vp, vp, vp]),
     "num_spmv_split": (C.c_int, [vp, i32, i32, vp, vp, vp, vp]),
@@ -62,7 +62,7 @@ SIGNATURES = {
     "num_cg_workspace_bytes": (i64, [vp]),
     "num_cg_solve": (C.c_int, [vp, vp, vp, f64, i32, vp, i64, C.POINTER(CgResult), vp]),
     "num_cg_mixed_workspace_bytes": (i64, [vp]),
-    "num_cg_solve_mixed": (C.c_int, [vp, vp, vp, f64, i32, f64, i32, vp, i64,
+    "num_cg_solve_mixed": (C.c_int, [vp, vp, vp, f64, i32, f64, vp, i64,
                                      C.POINTER(MixedResult), vp]),
     "num_halo_pack": (C.c_int, [vp, i32, vp, vp, vp]),
     "num_halo_scatter": (C.c_int, [vp, i32, vp, i32, vp]),

@@ -1,17 +1,37 @@
-// cg.hip -- unpreconditioned CG (src/cg_solver.hpp:133-196) and the FP32-inner
-// / FP64-refined form, with fused kernels and device-resident scalars.
+// cg.hip -- unpreconditioned CG (src/cg_solver.hpp:133-196) and its
+// FP32-inner / FP64-refined form, with fused kernels and device-resident
+// scalars.
 //
-// FP64 CG, per iteration (same arithmetic as the reference, in its order):
-//   K1 spmv_pap   Ap = A p (row-block SpMV), per-block partials of p_i*Ap_i
-//   K2 alpha      pAp = sum partials (fixed order); breakdown guard; alpha = rr/pAp
-//   K3 update     x = alpha*p + x ; r = (-alpha)*Ap + r ; partials of r_i*r_i
-//   K4 beta       rrn = sum; rel = sqrt(rrn)/||b||; stop if rel < tol (strict),
-//                 else beta = rrn/rr, rr = rrn
-//   K5 pupdate    p = 1.0*r + beta*p
-// Every kernel first reads the device `done` flag and returns at once when it
-// is set, so the host can queue iterations ahead and poll the flag
+// Both solvers run as a uniform sequence of 4-launch slots; every scalar
+// (alpha, beta, r.r, iteration count, stop flags) lives in a device state
+// record, so the host only queues slots and polls the `done` flag
 // asynchronously (two chunks in flight) instead of reading back two scalars
-// per iteration like the reference.
+// per iteration like the reference.  Global dot products finish in a
+// one-workgroup kernel that sums the per-workgroup partials in index order
+// (deterministic; a grid-wide "last workgroup finishes" reduction needs a
+// device-scope release per workgroup, an L2 write-back on this GPU, and
+// measured 5x slower).
+//
+// FP64 CG slot (the reference's arithmetic in its order):
+//   KA  p_new = 1*r + beta*p_old is recomputed inside the SpMV gather (the
+//       reference's axpy2, cg_solver.hpp:190, fused into the next SpMV), the
+//       tile writes its own rows of p_new to the other p buffer, Ap = A p_new,
+//       per-tile partials of p_new_i * Ap_i
+//   KB  pAp = sum of partials (fixed order); |pAp| < 1e-300 guard; alpha = rr/pAp
+//   KC  x = alpha*p_new + x ; r = (-alpha)*Ap + r ; partials of r_i * r_i
+//   KD  rrn = sum; rel = sqrt(rrn)/||b||; stop if rel < tol (strict), else
+//       beta = rrn/rr, rr = rrn; flip the p buffers
+//
+// Mixed slot (reliable updates): the same four kernels run the CG step in
+// FP32 (FP32 matrix copy and vectors, dot products accumulated in FP64,
+// scalars in FP64).  The FP32 solution correction x32 accumulates since the
+// last refinement.  When the FP32 recursive residual has dropped by a factor
+// delta since the last refinement, or looks converged, KD raises `ru` and the
+// next slot refines in FP64 instead of stepping: KA computes
+// x64' = x64 + x32 and the true residual r = b - A x64' in FP64 (re-seeding
+// the FP32 residual from it and restarting x32 at 0), KB tests the FP64 true
+// residual against tol and recomputes beta from the new residual, keeping the
+// search direction (the Krylov space survives the refinement).
 #include <algorithm>
 #include <cmath>
 
@@ -21,8 +41,13 @@
 
 namespace numamd {
 
+constexpr int kFin = 1024;  // threads of the single-workgroup finalize kernels
+
+static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// ============================================================== FP64 CG ===
 struct CgState {
-    double rr;       // current r.r
+    double rr;       // r.r of the current residual
     double bnorm;    // ||b|| (0 -> 1)
     double alpha;
     double beta;
@@ -33,88 +58,108 @@ struct CgState {
     int32_t done;
     int32_t converged;
     int32_t breakdown;
-    int32_t pad_[5];
+    int32_t psel;    // p buffer holding the current search direction
+    int32_t pad_[4];
 };
 static_assert(sizeof(CgState) <= 256, "state fits its slot");
 
-static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+// p_new[c] = 1*r[c] + beta*p_old[c], the p update of the previous iteration
+template <typename T>
+struct GatherPNew {
+    const T* __restrict__ r;
+    const T* __restrict__ p0;   // p buffer 0
+    const T* __restrict__ p1;   // p buffer 1
+    const double* beta_src;     // device beta
+    const int32_t* sel_src;     // device: which buffer holds p_old
+    const T* __restrict__ p_old;
+    T beta;
+    __device__ __forceinline__ void prepare()
+    {
+        beta = (T)(*beta_src);
+        p_old = *sel_src ? p1 : p0;
+    }
+    __device__ __forceinline__ T operator()(int c) const { return r[c] + beta * p_old[c]; }
+};
 
-// ---------------------------------------------------------------- epilogues
-// Set-up: Ap = A x0 ; r = -1*Ap + b ; p = r ; partials r.r and b.b
+// Set-up: Ax = A x0 ; r = -1*Ax + b ; partials r.r and b.b
 struct EpiInit {
     static constexpr int kRed = 2;
     const double* __restrict__ b;
     double* __restrict__ r;
-    double* __restrict__ p;
     double* partial_a;
     double* partial_b;
     __device__ __forceinline__ bool skip() const { return false; }
-    __device__ __forceinline__ void operator()(int row, double v, double& ca, double& cb) const
+    template <class G>
+    __device__ __forceinline__ void operator()(int row, double v, double& ca, double& cb,
+                                               const G&) const
     {
         const double bi = b[row];
         const double ri = -1.0 * v + bi;
         r[row] = ri;
-        p[row] = ri;
         ca = ri * ri;
         cb = bi * bi;
     }
 };
 
+// KA epilogue: own row of p_new to the other buffer, Ap, p_new_i * Ap_i
 struct EpiPAp {
     static constexpr int kRed = 1;
-    const double* __restrict__ p;
+    double* __restrict__ p0;
+    double* __restrict__ p1;
     double* __restrict__ Ap;
     const CgState* st;
     double* partial_a;
     double* partial_b;
     __device__ __forceinline__ bool skip() const { return st->done != 0; }
-    __device__ __forceinline__ void operator()(int row, double v, double& ca, double&) const
+    __device__ __forceinline__ void operator()(int row, double v, double& ca, double&,
+                                               const GatherPNew<double>& g) const
     {
+        const double pn = g(row);
+        (st->psel ? p0 : p1)[row] = pn;
         Ap[row] = v;
-        ca = p[row] * v;
+        ca = pn * v;
     }
 };
 
-// ------------------------------------------------------------- scalar steps
-__global__ __launch_bounds__(kRedBlock) void cg_init_finalize(const double* __restrict__ pa,
-                                                              const double* __restrict__ pb,
-                                                              int np, CgState* st)
+__global__ __launch_bounds__(kFin) void cg_init_finalize(const double* __restrict__ pa,
+                                                         const double* __restrict__ pb, int np,
+                                                         CgState* st, double tol,
+                                                         int32_t max_iter)
 {
-    __shared__ double red[2 * kRedBlock / kWave];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < np; i += kRedBlock) {
-        a += pa[i];
-        b += pb[i];
-    }
-    block_sum2<kRedBlock>(a, b, red);
+    __shared__ double red[kFin / kWave];
+    const double a = sum_partials_block<kFin>(pa, np, red);
+    const double b = sum_partials_block<kFin>(pb, np, red);
     if (threadIdx.x == 0) {
         double bn = sqrt(b);
         if (bn == 0.0) bn = 1.0;  // src/cg_solver.hpp:153
+        st->tol = tol;
+        st->max_iter = max_iter;
         st->bnorm = bn;
         st->rr = a;
         st->rel = sqrt(a) / bn;
+        st->alpha = 0.0;
+        st->beta = 0.0;  // first p_new = r + 0*p_old, p_old zero-filled
+        st->psel = 0;
         st->it = 0;
         st->breakdown = 0;
         st->converged = 0;
         st->done = 0;
-        if (sqrt(a) / bn < st->tol) {  // src/cg_solver.hpp:158-159
+        if (sqrt(a) / bn < tol) {  // early exit, src/cg_solver.hpp:158-159
             st->converged = 1;
             st->done = 1;
         }
-        if (st->max_iter <= 0) st->done = 1;
+        if (max_iter <= 0) st->done = 1;
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void cg_alpha(const double* __restrict__ pa, int np,
-                                                      CgState* st)
+__global__ __launch_bounds__(kFin) void cg_alpha(const double* __restrict__ pa, int np,
+                                                 CgState* st)
 {
-    __shared__ double red[kRedBlock / kWave];
+    __shared__ double red[kFin / kWave];
     if (st->done) return;
-    double a = 0.0;
-    for (int i = threadIdx.x; i < np; i += kRedBlock) a += pa[i];
-    const double pAp = block_sum<kRedBlock>(a, red);
+    const double pAp = sum_partials_block<kFin>(pa, np, red);
     if (threadIdx.x == 0) {
-        if (fabs(pAp) < 1e-300) {  // src/cg_solver.hpp:84-85
+        if (!(fabs(pAp) >= 1e-300)) {  // src/cg_solver.hpp:84-85 (NaN included)
             st->breakdown = 1;
             st->done = 1;
         } else {
@@ -123,7 +168,8 @@ __global__ __launch_bounds__(kRedBlock) void cg_alpha(const double* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void cg_update(int64_t n, const double* __restrict__ p,
+__global__ __launch_bounds__(kRedBlock) void cg_update(int64_t n, const double* __restrict__ p0,
+                                                       const double* __restrict__ p1,
                                                        const double* __restrict__ Ap, double* x,
                                                        double* r, const CgState* st,
                                                        double* __restrict__ partial)
@@ -132,6 +178,7 @@ __global__ __launch_bounds__(kRedBlock) void cg_update(int64_t n, const double* 
     if (st->done) return;
     const double alpha = st->alpha;
     const double nalpha = -alpha;
+    const double* __restrict__ p = st->psel ? p0 : p1;  // p_new was written to !psel
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -144,17 +191,16 @@ __global__ __launch_bounds__(kRedBlock) void cg_update(int64_t n, const double* 
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kRedBlock) void cg_beta(const double* __restrict__ pb, int np,
-                                                     CgState* st)
+__global__ __launch_bounds__(kFin) void cg_beta(const double* __restrict__ pb, int np,
+                                                CgState* st)
 {
-    __shared__ double red[kRedBlock / kWave];
+    __shared__ double red[kFin / kWave];
     if (st->done) return;
-    double a = 0.0;
-    for (int i = threadIdx.x; i < np; i += kRedBlock) a += pb[i];
-    const double rrn = block_sum<kRedBlock>(a, red);
+    const double rrn = sum_partials_block<kFin>(pb, np, red);
     if (threadIdx.x == 0) {
         const int it = st->it + 1;
         st->it = it;
+        st->psel ^= 1;
         const double rel = sqrt(rrn) / st->bnorm;
         st->rel = rel;
         if (rel < st->tol) {  // strict <, src/cg_solver.hpp:180-183
@@ -168,27 +214,8 @@ __global__ __launch_bounds__(kRedBlock) void cg_beta(const double* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void cg_pupdate(int64_t n, const double* __restrict__ r,
-                                                        double* p, const CgState* st)
-{
-    if (st->done) return;
-    const double beta = st->beta;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        p[i] = 1.0 * r[i] + beta * p[i];
-}
-
-__global__ void cg_set_params(CgState* st, double tol, int32_t max_iter)
-{
-    st->tol = tol;
-    st->max_iter = max_iter;
-    st->done = 0;
-}
-
-// ------------------------------------------------------------ host helpers
 struct CgLayout {
-    size_t off_r, off_p, off_Ap, off_pa, off_pb, off_state, total;
-    int np;  // partial slots
+    size_t off_r, off_p0, off_p1, off_Ap, off_pa, off_pb, off_state, total;
 };
 
 static CgLayout cg_layout(const num_csr* A)
@@ -198,27 +225,20 @@ static CgLayout cg_layout(const num_csr* A)
     const int np = std::max(A->sched.nblocks, dot_grid(A->nrows));
     size_t o = 0;
     L.off_r = o;  o += align256(n * 8);
-    L.off_p = o;  o += align256(n * 8);
+    L.off_p0 = o; o += align256(n * 8);
+    L.off_p1 = o; o += align256(n * 8);
     L.off_Ap = o; o += align256(n * 8);
     L.off_pa = o; o += align256((size_t)np * 8);
     L.off_pb = o; o += align256((size_t)np * 8);
     L.off_state = o; o += 256;
     L.total = o;
-    L.np = np;
     return L;
 }
 
-template <typename T, class Epi>
-static void launch_fused_spmv(const num_csr* A, const T* vals, const T* x, const Epi& epi,
-                              hipStream_t stream)
-{
-    launch_tiles<T>(A, A->sched, vals, GatherX<T>{x}, epi, stream);
-}
-
-// Pinned status slots for the asynchronous poll (one pair per host thread).
+// =========================================================== polling ===
 constexpr size_t kSlot = 256;
 struct PollSlots {
-    unsigned char* h = nullptr;  // 2 slots of kSlot bytes
+    unsigned char* h = nullptr;  // 2 slots of kSlot bytes, pinned
     hipEvent_t ev[2] = {nullptr, nullptr};
     int device = -1;
 };
@@ -229,7 +249,6 @@ static int poll_slots(PollSlots** out)
     int dev = 0;
     NUM_HIP(hipGetDevice(&dev));
     if (g_poll.h && g_poll.device != dev) {
-        // slots belong to another device: rebuild for this one
         (void)hipHostFree(g_poll.h);
         (void)hipEventDestroy(g_poll.ev[0]);
         (void)hipEventDestroy(g_poll.ev[1]);
@@ -245,25 +264,26 @@ static int poll_slots(PollSlots** out)
     return NUM_OK;
 }
 
-// Queue iterations in chunks, keep at most two chunks ahead of the poll.
-// `launch_iter()` enqueues one iteration; `stop(const State&)` reads a
-// polled copy of the device state.
+// Queue slots in chunks, keeping at most two chunks ahead of the poll.
+// `launch_slot()` enqueues one slot; `stop(const State&)` reads a polled copy
+// of the device state.  Kernels skip themselves once the state says done, so
+// slots queued past the end cost only their launch.
 template <class State, class F, class Stop>
-static int drive(hipStream_t stream, const State* d_state, int32_t max_iter, F launch_iter,
+static int drive(hipStream_t stream, const State* d_state, int64_t max_slots, F launch_slot,
                  Stop stop)
 {
     static_assert(sizeof(State) <= kSlot, "state fits a poll slot");
     PollSlots* ps = nullptr;
     int rc = poll_slots(&ps);
     if (rc) return rc;
-    int32_t launched = 0;
+    int64_t launched = 0;
     int chunk = 4;
     int slot = 0;
     bool have_prev = false;
-    while (launched < max_iter) {
-        const int m = std::min<int32_t>(chunk, max_iter - launched);
+    while (launched < max_slots) {
+        const int m = (int)std::min<int64_t>(chunk, max_slots - launched);
         for (int k = 0; k < m; ++k) {
-            rc = launch_iter();
+            rc = launch_slot();
             if (rc) return rc;
         }
         launched += m;
@@ -281,125 +301,195 @@ static int drive(hipStream_t stream, const State* d_state, int32_t max_iter, F l
     return NUM_OK;
 }
 
-// ======================================================================
-// Mixed precision: FP64 outer residual / update / test, FP32 inner CG with
-// dot products accumulated in FP64.
-// ======================================================================
+// ============================================================ mixed CG ===
 struct MixState {
-    double rr;         // inner r.r (FP64 accumulation of FP32 values)
-    double rr0;        // inner r.r at the start of this inner solve
+    double rr;       // FP32 residual r32.r32 (FP64 accumulation) of the current step
+    double rr_ref;   // rr right after the last refinement
     double alpha;
     double beta;
     double bnorm;
-    double rel;        // FP64 true residual of the latest outer evaluation
     double tol;
-    double inner_tol;
-    int32_t inner_it, inner_max, inner_done, inner_total;
-    int32_t outer, max_outer, done, converged, breakdown;
-    int32_t pad_[3];
+    double delta;    // refinement trigger: residual drop since the last one
+    double rel;      // FP64 true residual of the latest refinement
+    int32_t it;      // FP32 CG steps taken
+    int32_t max_iter;
+    int32_t done, converged, breakdown;
+    int32_t ru;       // 1: this slot refines in FP64 instead of stepping
+    int32_t post_ru;  // 1: this slot refined (KC / KD skip)
+    int32_t nref;     // refinements done (FP64 residual evaluations)
+    int32_t psel;     // p buffer holding the current search direction
+    int32_t xsel;     // x64 / x32 buffer pair holding the current iterate
+    int32_t final_;   // 1: the pending refinement ends the solve
+    int32_t pad_;
+};
+static_assert(sizeof(MixState) <= 256, "state fits its slot");
+
+struct MixArgs {
+    const double* b;
+    double* x64a;   // user x
+    double* x64b;   // workspace
+    float* x32a;
+    float* x32b;
+    float* r32;
+    float* p0;
+    float* p1;
+    float* q;
+    double* pa;
+    double* pb;
+    MixState* st;
 };
 
-// Outer residual: Ax = A x (FP64); r = -1*Ax + b; r32 = (float) r; p32 = r32;
-// d32 = 0; partials r.r (FP64) and r32.r32 (FP64 accumulation).
-struct EpiOuter {
+// FP64 gather of the refined iterate x64 + x32
+struct GatherXsum {
+    const double* __restrict__ x64;
+    const float* __restrict__ x32;
+    __device__ __forceinline__ void prepare() {}
+    __device__ __forceinline__ double operator()(int c) const
+    {
+        return x64[c] + (double)x32[c];
+    }
+};
+
+// Refinement epilogue: x64' = x64 + x32 (to the other buffer), x32' = 0,
+// r = -1*Ax + b in FP64, r32 = (float) r; partials r.r and r32.r32
+struct EpiRefine {
     static constexpr int kRed = 2;
     const double* __restrict__ b;
+    double* __restrict__ x64n;
+    float* __restrict__ x32n;
     float* __restrict__ r32;
-    float* __restrict__ p32;
-    float* __restrict__ d32;
     double* partial_a;
     double* partial_b;
     __device__ __forceinline__ bool skip() const { return false; }
-    __device__ __forceinline__ void operator()(int row, double v, double& ca, double& cb) const
+    __device__ __forceinline__ void operator()(int row, double v, double& ca, double& cb,
+                                               const GatherXsum& g) const
     {
+        x64n[row] = g(row);
+        x32n[row] = 0.0f;
         const double ri = -1.0 * v + b[row];
         const float rf = (float)ri;
         r32[row] = rf;
-        p32[row] = rf;
-        d32[row] = 0.0f;
         ca = ri * ri;
         cb = (double)rf * (double)rf;
     }
 };
 
+// FP32 step epilogue: p_new row to the other buffer, q = A p_new, p_new.q
 struct EpiPq {
     static constexpr int kRed = 1;
-    const float* __restrict__ p32;
-    float* __restrict__ q32;
-    const MixState* st;
+    float* __restrict__ pn_out;
+    float* __restrict__ q;
     double* partial_a;
     double* partial_b;
-    __device__ __forceinline__ bool skip() const { return st->inner_done != 0; }
-    __device__ __forceinline__ void operator()(int row, float v, double& ca, double&) const
+    __device__ __forceinline__ bool skip() const { return false; }
+    __device__ __forceinline__ void operator()(int row, float v, double& ca, double&,
+                                               const GatherPNew<float>& g) const
     {
-        q32[row] = v;
-        ca = (double)p32[row] * (double)v;
+        const float pn = g(row);
+        pn_out[row] = pn;
+        q[row] = v;
+        ca = (double)pn * (double)v;
     }
 };
 
-__global__ void mix_set_params(MixState* st, double tol, int32_t max_outer, double inner_tol,
-                               int32_t inner_max)
+// KA of the mixed slot: FP32 step SpMV, or the FP64 refinement SpMV.
+template <int NPT, bool NT32, bool NT64>
+__global__ __launch_bounds__(kSpmvBS) void mix_spmv_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ v64, const float* __restrict__ v32,
+    const TileEntry* __restrict__ sched, int ntiles, int chunk, MixArgs a)
 {
-    st->tol = tol;
-    st->inner_tol = inner_tol;
-    st->max_outer = max_outer;
-    st->inner_max = inner_max;
-    st->outer = 0;
-    st->inner_total = 0;
-    st->done = 0;
-    st->converged = 0;
-    st->breakdown = 0;
-    st->inner_done = 1;
-}
-
-__global__ __launch_bounds__(kRedBlock) void mix_outer_check(const double* __restrict__ pa,
-                                                             const double* __restrict__ pb,
-                                                             int np, MixState* st, int first)
-{
-    __shared__ double red[2 * kRedBlock / kWave];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < np; i += kRedBlock) {
-        a += pa[i];
-        b += pb[i];
-    }
-    block_sum2<kRedBlock>(a, b, red);
-    if (threadIdx.x == 0) {
-        const double rel = sqrt(a) / st->bnorm;
-        st->rel = rel;
-        if (!first) st->outer += 1;
-        if (rel < st->tol) {
-            st->converged = 1;
-            st->done = 1;
-        } else if (st->outer >= st->max_outer || !(rel == rel)) {
-            st->done = 1;
-        } else {
-            st->rr = b;
-            st->rr0 = b;
-            st->inner_it = 0;
-            st->inner_done = (b > 0.0 && st->inner_max > 0) ? 0 : 1;
+    __shared__ double lv[kSpmvBS * NPT];
+    __shared__ int32_t lc[kSpmvBS * NPT];
+    __shared__ double red[2 * kSpmvBS / kWave];
+    const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
+    if (tile < 0) return;
+    const MixState* st = a.st;
+    if (st->done) return;
+    double ca, cb;
+    if (st->ru) {
+        const int xs = st->xsel;
+        GatherXsum g{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
+        EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, a.pa, a.pb};
+        spmv_tile_body<double, kSpmvBS, NPT, NT64>(tile, rowptr, colind, v64, g, sched, epi, lv,
+                                                   lc, ca, cb);
+        block_sum2<kSpmvBS>(ca, cb, red);
+        if (threadIdx.x == 0) {
+            a.pa[tile] = ca;
+            a.pb[tile] = cb;
         }
+    } else {
+        GatherPNew<float> g{a.r32, a.p0, a.p1, &st->beta, &st->psel, nullptr, 0.0f};
+        g.prepare();
+        EpiPq epi{st->psel ? a.p0 : a.p1, a.q, a.pa, a.pb};
+        spmv_tile_body<float, kSpmvBS, NPT, NT32>(tile, rowptr, colind, v32, g, sched, epi,
+                                                  reinterpret_cast<float*>(lv), lc, ca, cb);
+        const double s = block_sum<kSpmvBS>(ca, red);
+        if (threadIdx.x == 0) a.pa[tile] = s;
     }
 }
 
-__global__ void mix_set_bnorm(const double* __restrict__ bb, MixState* st)
+__global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol, double delta,
+                         int32_t max_iter)
 {
+    if (threadIdx.x != 0) return;
     double bn = sqrt(bb[0]);
     if (bn == 0.0) bn = 1.0;
     st->bnorm = bn;
+    st->tol = tol;
+    st->delta = delta;
+    st->max_iter = max_iter;
+    st->rr = INFINITY;  // first refinement: beta = rr'/inf = 0
+    st->rr_ref = 0.0;
+    st->alpha = 0.0;
+    st->beta = 0.0;
+    st->rel = 0.0;
+    st->it = 0;
+    st->done = 0;
+    st->converged = 0;
+    st->breakdown = 0;
+    st->ru = 1;  // slot 0 evaluates r = b - A x0 in FP64
+    st->post_ru = 0;
+    st->nref = 0;
+    st->psel = 0;
+    st->xsel = 0;
+    st->final_ = max_iter <= 0 ? 1 : 0;
 }
 
-__global__ __launch_bounds__(kRedBlock) void mix_alpha(const double* __restrict__ pa, int np,
-                                                       MixState* st)
+// KB: alpha, or the refinement's FP64 convergence test and new beta
+__global__ __launch_bounds__(kFin) void mix_kb(const double* __restrict__ pa,
+                                               const double* __restrict__ pb, int np,
+                                               MixState* st)
 {
-    __shared__ double red[kRedBlock / kWave];
-    if (st->inner_done) return;
-    double a = 0.0;
-    for (int i = threadIdx.x; i < np; i += kRedBlock) a += pa[i];
-    const double pq = block_sum<kRedBlock>(a, red);
+    __shared__ double red[kFin / kWave];
+    if (st->done) return;
+    if (st->ru) {
+        const double rr64 = sum_partials_block<kFin>(pa, np, red);
+        const double rr32 = sum_partials_block<kFin>(pb, np, red);
+        if (threadIdx.x == 0) {
+            const double rel = sqrt(rr64) / st->bnorm;
+            st->rel = rel;
+            st->xsel ^= 1;
+            st->nref += 1;
+            st->ru = 0;
+            st->post_ru = 1;
+            if (rel < st->tol) {
+                st->converged = 1;
+                st->done = 1;
+            } else if (st->final_ || !(rel == rel)) {
+                st->done = 1;
+            } else {
+                st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
+                st->rr = rr32;
+                st->rr_ref = rr32;
+            }
+        }
+        return;
+    }
+    const double pq = sum_partials_block<kFin>(pa, np, red);
     if (threadIdx.x == 0) {
         if (!(fabs(pq) >= 1e-300)) {
             st->breakdown = 1;
-            st->inner_done = 1;
             st->done = 1;
         } else {
             st->alpha = st->rr / pq;
@@ -407,40 +497,52 @@ __global__ __launch_bounds__(kRedBlock) void mix_alpha(const double* __restrict_
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void mix_update(int64_t n, const float* __restrict__ p,
-                                                        const float* __restrict__ q, float* d,
-                                                        float* r, const MixState* st,
-                                                        double* __restrict__ partial)
+// KC: x32 += alpha p_new ; r32 -= alpha q (FP32), partials of r32.r32 (FP64)
+__global__ __launch_bounds__(kRedBlock) void mix_update(int64_t n, MixArgs a)
 {
     __shared__ double red[kRedBlock / kWave];
-    if (st->inner_done) return;
-    const float a = (float)st->alpha;
-    const float na = -a;
+    const MixState* st = a.st;
+    if (st->done || st->post_ru) return;
+    const float al = (float)st->alpha;
+    const float nal = -al;
+    const float* __restrict__ p = st->psel ? a.p0 : a.p1;
+    float* __restrict__ x32 = st->xsel ? a.x32b : a.x32a;
+    const float* __restrict__ q = a.q;
+    float* __restrict__ r = a.r32;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        d[i] = a * p[i] + d[i];
-        const float ri = na * q[i] + r[i];
+        x32[i] = al * p[i] + x32[i];
+        const float ri = nal * q[i] + r[i];
         r[i] = ri;
         acc += (double)ri * (double)ri;
     }
     const double s = block_sum<kRedBlock>(acc, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) a.pb[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kRedBlock) void mix_beta(const double* __restrict__ pb, int np,
-                                                      MixState* st)
+// KD: beta, or schedule a refinement
+__global__ __launch_bounds__(kFin) void mix_kd(const double* __restrict__ pb, int np,
+                                               MixState* st)
 {
-    __shared__ double red[kRedBlock / kWave];
-    if (st->inner_done) return;
-    double a = 0.0;
-    for (int i = threadIdx.x; i < np; i += kRedBlock) a += pb[i];
-    const double rrn = block_sum<kRedBlock>(a, red);
+    __shared__ double red[kFin / kWave];
+    if (st->done) return;
+    if (st->post_ru) {
+        if (threadIdx.x == 0) st->post_ru = 0;
+        return;
+    }
+    const double rrn = sum_partials_block<kFin>(pb, np, red);
     if (threadIdx.x == 0) {
-        st->inner_it += 1;
-        st->inner_total += 1;
-        if (sqrt(rrn / st->rr0) < st->inner_tol || st->inner_it >= st->inner_max) {
-            st->inner_done = 1;
+        const int it = st->it + 1;
+        st->it = it;
+        st->psel ^= 1;
+        const double est = sqrt(rrn) / st->bnorm;
+        if (it >= st->max_iter) {
+            st->ru = 1;
+            st->final_ = 1;
+        } else if (est < st->tol || sqrt(rrn) < st->delta * sqrt(st->rr_ref) ||
+                   !(rrn == rrn)) {
+            st->ru = 1;  // beta comes from the refined residual
         } else {
             st->beta = rrn / st->rr;
             st->rr = rrn;
@@ -448,27 +550,25 @@ __global__ __launch_bounds__(kRedBlock) void mix_beta(const double* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void mix_pupdate(int64_t n, const float* __restrict__ r,
-                                                         float* p, const MixState* st)
+__global__ __launch_bounds__(kRedBlock) void mix_finish(int64_t n, const double* __restrict__ x64,
+                                                        const float* __restrict__ x32,
+                                                        double* __restrict__ out)
 {
-    if (st->inner_done) return;
-    const float beta = (float)st->beta;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        p[i] = r[i] + beta * p[i];
+        out[i] = x64[i] + (double)x32[i];
 }
 
-__global__ __launch_bounds__(kRedBlock) void mix_xupdate(int64_t n, const float* __restrict__ d,
-                                                         double* x)
+__global__ void zero_f32(int64_t n, float* __restrict__ a)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        x[i] = x[i] + (double)d[i];
+        a[i] = 0.0f;
 }
 
 struct MixLayout {
-    size_t off_d, off_r, off_p, off_q, off_pa, off_pb, off_bb, off_state, total;
-    int np;
+    size_t off_x64b, off_x32a, off_x32b, off_r, off_p0, off_p1, off_q, off_pa, off_pb, off_bb,
+        off_state, total;
 };
 
 static MixLayout mix_layout(const num_csr* A)
@@ -477,17 +577,52 @@ static MixLayout mix_layout(const num_csr* A)
     const size_t n = (size_t)A->nrows;
     const int np = std::max(A->sched.nblocks, dot_grid(A->nrows));
     size_t o = 0;
-    L.off_d = o;  o += align256(n * 4);
+    L.off_x64b = o; o += align256(n * 8);
+    L.off_x32a = o; o += align256(n * 4);
+    L.off_x32b = o; o += align256(n * 4);
     L.off_r = o;  o += align256(n * 4);
-    L.off_p = o;  o += align256(n * 4);
+    L.off_p0 = o; o += align256(n * 4);
+    L.off_p1 = o; o += align256(n * 4);
     L.off_q = o;  o += align256(n * 4);
     L.off_pa = o; o += align256((size_t)np * 8);
     L.off_pb = o; o += align256((size_t)np * 8);
     L.off_bb = o; o += 256;
     L.off_state = o; o += 256;
     L.total = o;
-    L.np = np;
     return L;
+}
+
+template <int NPT>
+static void launch_mix_spmv_npt(const num_csr* A, const MixArgs& a, hipStream_t st)
+{
+    const Schedule& s = A->sched;
+    const int grid = chunk_grid(s.nblocks, s.chunk);
+    const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
+    if (s.nt32) {
+        if (s.nt64)
+            mix_spmv_kernel<NPT, true, true><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, tiles, s.nblocks, s.chunk, a);
+        else
+            mix_spmv_kernel<NPT, true, false><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, tiles, s.nblocks, s.chunk, a);
+    } else {
+        if (s.nt64)
+            mix_spmv_kernel<NPT, false, true><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, tiles, s.nblocks, s.chunk, a);
+        else
+            mix_spmv_kernel<NPT, false, false><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, tiles, s.nblocks, s.chunk, a);
+    }
+}
+
+static void launch_mix_spmv(const num_csr* A, const MixArgs& a, hipStream_t st)
+{
+    switch (A->sched.npt) {
+    case 5: launch_mix_spmv_npt<5>(A, a, st); break;
+    case 7: launch_mix_spmv_npt<7>(A, a, st); break;
+    case 12: launch_mix_spmv_npt<12>(A, a, st); break;
+    default: launch_mix_spmv_npt<16>(A, a, st); break;
+    }
 }
 
 }  // namespace numamd
@@ -517,7 +652,8 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol, i
     hipStream_t st = (hipStream_t)stream;
     char* w = (char*)d_work;
     double* r = (double*)(w + L.off_r);
-    double* p = (double*)(w + L.off_p);
+    double* p0 = (double*)(w + L.off_p0);
+    double* p1 = (double*)(w + L.off_p1);
     double* Ap = (double*)(w + L.off_Ap);
     double* pa = (double*)(w + L.off_pa);
     double* pb = (double*)(w + L.off_pb);
@@ -525,23 +661,26 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol, i
     const int64_t n = A->nrows;
     const int np_spmv = A->sched.nblocks;
     const int gu = dot_grid(n);
-    const int ge = elementwise_grid(n);
 
-    cg_set_params<<<1, 1, 0, st>>>(state, tol, max_iter);
-    launch_fused_spmv<double>(A, A->vals, d_x, EpiInit{d_b, r, p, pa, pb}, st);
-    cg_init_finalize<<<1, kRedBlock, 0, st>>>(pa, pb, np_spmv, state);
+    // p_old of the first step must be finite (it is multiplied by beta = 0)
+    rc = num_fill(n, 0.0, p0, stream);
+    if (rc) return rc;
+    launch_tiles<double>(A, A->sched, A->vals, GatherX<double>{d_x},
+                         EpiInit{d_b, r, pa, pb}, st);
+    cg_init_finalize<<<1, kFin, 0, st>>>(pa, pb, np_spmv, state, tol, max_iter);
     NUM_LAUNCH_CHECK();
 
-    auto iter = [&]() -> int {
-        launch_fused_spmv<double>(A, A->vals, p, EpiPAp{p, Ap, state, pa, nullptr}, st);
-        cg_alpha<<<1, kRedBlock, 0, st>>>(pa, np_spmv, state);
-        cg_update<<<gu, kRedBlock, 0, st>>>(n, p, Ap, d_x, r, state, pb);
-        cg_beta<<<1, kRedBlock, 0, st>>>(pb, gu, state);
-        cg_pupdate<<<ge, kRedBlock, 0, st>>>(n, r, p, state);
+    auto slot = [&]() -> int {
+        GatherPNew<double> g{r, p0, p1, &state->beta, &state->psel, nullptr, 0.0};
+        launch_tiles<double>(A, A->sched, A->vals, g, EpiPAp{p0, p1, Ap, state, pa, nullptr},
+                             st);
+        cg_alpha<<<1, kFin, 0, st>>>(pa, np_spmv, state);
+        cg_update<<<gu, kRedBlock, 0, st>>>(n, p0, p1, Ap, d_x, r, state, pb);
+        cg_beta<<<1, kFin, 0, st>>>(pb, gu, state);
         NUM_LAUNCH_CHECK();
         return NUM_OK;
     };
-    rc = drive(st, state, max_iter, iter, [](const CgState& h) { return h.done != 0; });
+    rc = drive(st, state, max_iter, slot, [](const CgState& h) { return h.done != 0; });
     if (rc) return rc;
     CgState h;
     NUM_HIP(hipMemcpyAsync(&h, state, sizeof(CgState), hipMemcpyDeviceToHost, st));
@@ -562,15 +701,15 @@ int64_t num_cg_mixed_workspace_bytes(const num_csr* A)
 }
 
 int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double tol,
-                       int32_t max_outer, double inner_tol, int32_t inner_max, void* d_work,
-                       int64_t work_bytes, num_mixed_result* res, void* stream)
+                       int32_t max_iter, double delta, void* d_work, int64_t work_bytes,
+                       num_mixed_result* res, void* stream)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
     if (!A->vals32) return fail(NUM_EINVAL, "no FP32 values attached (num_csr_attach_f32)");
     if (!d_b || !d_x || !res) return fail(NUM_EINVAL, "b, x or result is NULL");
     if (A->nrows != A->ncols) return fail(NUM_EINVAL, "CG needs a square matrix");
-    if (max_outer < 0 || inner_max < 0) return fail(NUM_EINVAL, "iteration limits must be >= 0");
-    if (!(inner_tol > 0.0 && inner_tol < 1.0)) return fail(NUM_EINVAL, "inner_tol must be in (0,1)");
+    if (max_iter < 0) return fail(NUM_EINVAL, "max_iter must be >= 0");
+    if (!(delta > 0.0 && delta < 1.0)) return fail(NUM_EINVAL, "delta must be in (0, 1)");
     int rc = check_device(A);
     if (rc) return rc;
     const MixLayout L = mix_layout(A);
@@ -578,51 +717,57 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
         return fail(NUM_EINVAL, "workspace too small: need %lld bytes", (long long)L.total);
     hipStream_t st = (hipStream_t)stream;
     char* w = (char*)d_work;
-    float* d32 = (float*)(w + L.off_d);
-    float* r32 = (float*)(w + L.off_r);
-    float* p32 = (float*)(w + L.off_p);
-    float* q32 = (float*)(w + L.off_q);
-    double* pa = (double*)(w + L.off_pa);
-    double* pb = (double*)(w + L.off_pb);
+    MixArgs a;
+    a.b = d_b;
+    a.x64a = d_x;
+    a.x64b = (double*)(w + L.off_x64b);
+    a.x32a = (float*)(w + L.off_x32a);
+    a.x32b = (float*)(w + L.off_x32b);
+    a.r32 = (float*)(w + L.off_r);
+    a.p0 = (float*)(w + L.off_p0);
+    a.p1 = (float*)(w + L.off_p1);
+    a.q = (float*)(w + L.off_q);
+    a.pa = (double*)(w + L.off_pa);
+    a.pb = (double*)(w + L.off_pb);
     double* bb = (double*)(w + L.off_bb);
-    MixState* state = (MixState*)(w + L.off_state);
+    a.st = (MixState*)(w + L.off_state);
     const int64_t n = A->nrows;
     const int np_spmv = A->sched.nblocks;
     const int gu = dot_grid(n);
     const int ge = elementwise_grid(n);
 
-    mix_set_params<<<1, 1, 0, st>>>(state, tol, max_outer, inner_tol, inner_max);
-    rc = launch_dot(n, d_b, d_b, bb, pb, 0, st);
+    zero_f32<<<ge, kRedBlock, 0, st>>>(n, a.x32a);
+    zero_f32<<<ge, kRedBlock, 0, st>>>(n, a.p0);
+    rc = launch_dot(n, d_b, d_b, bb, a.pb, 0, st);
     if (rc) return rc;
-    mix_set_bnorm<<<1, 1, 0, st>>>(bb, state);
+    mix_init<<<1, 64, 0, st>>>(bb, a.st, tol, delta, max_iter);
     NUM_LAUNCH_CHECK();
-    PollSlots* ps = nullptr;
-    rc = poll_slots(&ps);
+    auto slot = [&]() -> int {
+        launch_mix_spmv(A, a, st);
+        mix_kb<<<1, kFin, 0, st>>>(a.pa, a.pb, np_spmv, a.st);
+        mix_update<<<gu, kRedBlock, 0, st>>>(n, a);
+        mix_kd<<<1, kFin, 0, st>>>(a.pb, gu, a.st);
+        NUM_LAUNCH_CHECK();
+        return NUM_OK;
+    };
+    // FP32 steps plus one slot per refinement (at most one per step, plus
+    // the first evaluation and a final one)
+    rc = drive(st, a.st, 2 * (int64_t)max_iter + 2, slot,
+               [](const MixState& h) { return h.done != 0; });
     if (rc) return rc;
     MixState h;
-    for (int outer = 0;; ++outer) {
-        if (outer > 0) mix_xupdate<<<ge, kRedBlock, 0, st>>>(n, d32, d_x);
-        launch_fused_spmv<double>(A, A->vals, d_x, EpiOuter{d_b, r32, p32, d32, pa, pb}, st);
-        mix_outer_check<<<1, kRedBlock, 0, st>>>(pa, pb, np_spmv, state, outer == 0 ? 1 : 0);
+    NUM_HIP(hipMemcpyAsync(&h, a.st, sizeof(MixState), hipMemcpyDeviceToHost, st));
+    NUM_HIP(hipStreamSynchronize(st));
+    // the iterate lives in x64[xsel] + x32[xsel]; bring it to the caller's x
+    const double* xcur = h.xsel ? a.x64b : a.x64a;
+    const float* x32cur = h.xsel ? a.x32b : a.x32a;
+    if (xcur != d_x || h.breakdown) {
+        mix_finish<<<ge, kRedBlock, 0, st>>>(n, xcur, x32cur, d_x);
         NUM_LAUNCH_CHECK();
-        NUM_HIP(hipMemcpyAsync(&h, state, sizeof(MixState), hipMemcpyDeviceToHost, st));
         NUM_HIP(hipStreamSynchronize(st));
-        if (h.done) break;
-        auto iter = [&]() -> int {
-            launch_fused_spmv<float>(A, A->vals32, p32, EpiPq{p32, q32, state, pa, nullptr}, st);
-            mix_alpha<<<1, kRedBlock, 0, st>>>(pa, np_spmv, state);
-            mix_update<<<gu, kRedBlock, 0, st>>>(n, p32, q32, d32, r32, state, pb);
-            mix_beta<<<1, kRedBlock, 0, st>>>(pb, gu, state);
-            mix_pupdate<<<ge, kRedBlock, 0, st>>>(n, r32, p32, state);
-            NUM_LAUNCH_CHECK();
-            return NUM_OK;
-        };
-        rc = drive(st, state, inner_max, iter,
-                   [](const MixState& s) { return s.inner_done != 0; });
-        if (rc) return rc;
     }
-    res->outer = h.outer;
-    res->inner_total = h.inner_total;
+    res->outer = h.nref;
+    res->inner_total = h.it;
     res->converged = h.converged;
     res->breakdown = h.breakdown;
     res->true_residual = h.rel;

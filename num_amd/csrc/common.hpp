@@ -99,6 +99,26 @@ __device__ __forceinline__ double block_sum(double v, double* red)
     return s;
 }
 
+// Sum of n per-tile partials by one workgroup of BS threads in a fixed order:
+// thread t adds p[t], p[t + BS], ... with 8 independent loads in flight, then
+// the block tree.  Result valid in thread 0.  `red` holds BS / 64 doubles.
+template <int BS>
+__device__ __forceinline__ double sum_partials_block(const double* __restrict__ p, int n,
+                                                     double* red)
+{
+    double acc = 0.0;
+    int i = threadIdx.x;
+    for (; i + 7 * BS < n; i += 8 * BS) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[i + u * BS];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; i < n; i += BS) acc += p[i];
+    return block_sum<BS>(acc, red);
+}
+
 // Two independent block sums in one pass (r.r and b.b of the CG set-up).
 template <int BS>
 __device__ __forceinline__ void block_sum2(double& a, double& b, double* red)

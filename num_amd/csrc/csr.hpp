@@ -25,6 +25,7 @@ struct Schedule {
     int32_t nblocks = 0;         // tiles
     int32_t long_rows = 0;       // single-row tiles longer than nnz_per_block
     int32_t chunk = 1;           // XCD chunk size for chunk_tile
+    int32_t nt64 = 1, nt32 = 1;  // non-temporal matrix loads (FP64 / FP32 values)
     void* d_tiles = nullptr;     // device TileEntry[nblocks + 1]
 };
 
@@ -50,7 +51,8 @@ namespace numamd {
 
 // Builds `s` for rows [r0, r1) of A and uploads it.  Returns NUM_OK or error.
 int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
-                   int32_t nnz_per_block, Schedule& s);
+                   int32_t nnz_per_block, int32_t chunk, int32_t nt64, int32_t nt32,
+                   Schedule& s);
 // Default tile capacity for A: the smallest kNptChoices entry covering the
 // mean row length, times the block size.
 int32_t default_nnz_per_block(const num_csr* A);

@@ -74,8 +74,19 @@ void free_schedule(Schedule& s)
     s.nblocks = 0;
 }
 
+// Infinity Cache (MALL) capacity below which a matrix copy is loaded with
+// normal (caching) loads so repeated SpMVs inside a solve find it resident.
+constexpr double kMallResident = 256.0 * 1024 * 1024;
+
+static int32_t auto_nt(const num_csr* A, int value_bytes)
+{
+    const double bytes = (double)A->nnz * (value_bytes + 4) + 4.0 * (A->nrows + 1);
+    return bytes > kMallResident ? 1 : 0;
+}
+
 int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
-                   int32_t nnz_per_block, Schedule& s)
+                   int32_t nnz_per_block, int32_t chunk, int32_t nt64, int32_t nt32,
+                   Schedule& s)
 {
     if (rows_per_block < 1 || rows_per_block > kSpmvBS)
         return fail(NUM_EINVAL, "rows_per_block must be in [1, %d], got %d", kSpmvBS,
@@ -119,7 +130,9 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
     ns.npt = pick_npt(nnz_per_block);
     ns.nblocks = (int32_t)tiles.size() - 1;
     ns.long_rows = longs;
-    ns.chunk = tile_chunk(ns.nblocks);
+    ns.chunk = chunk > 0 ? chunk : tile_chunk(ns.nblocks);
+    ns.nt64 = nt64 >= 0 ? (nt64 != 0) : auto_nt(A, 8);
+    ns.nt32 = nt32 >= 0 ? (nt32 != 0) : auto_nt(A, 4);
     const size_t bytes = tiles.size() * sizeof(TileEntry);
     NUM_HIP(hipMalloc(&ns.d_tiles, bytes));
     hipError_t e = hipMemcpy(ns.d_tiles, tiles.data(), bytes, hipMemcpyHostToDevice);
@@ -143,7 +156,8 @@ int range_schedule(const num_csr* A, int32_t r0, int32_t r1, const Schedule** ou
     auto it = A->range_sched.find(key);
     if (it == A->range_sched.end()) {
         Schedule s;
-        int rc = build_schedule(A, r0, r1, A->sched.rows_per_block, A->sched.nnz_per_block, s);
+        int rc = build_schedule(A, r0, r1, A->sched.rows_per_block, A->sched.nnz_per_block,
+                                -1, A->sched.nt64, A->sched.nt32, s);
         if (rc) return rc;
         it = A->range_sched.emplace(key, std::move(s)).first;
     }
@@ -169,7 +183,8 @@ static int finish_create(num_csr* A)
     for (int32_t i = 0; i < A->nrows; ++i)
         mx = std::max(mx, A->h_rowptr[i + 1] - A->h_rowptr[i]);
     A->max_row_nnz = mx;
-    return build_schedule(A, 0, A->nrows, kSpmvBS, default_nnz_per_block(A), A->sched);
+    return build_schedule(A, 0, A->nrows, kSpmvBS, default_nnz_per_block(A), -1, -1, -1,
+                          A->sched);
 }
 
 }  // namespace numamd
@@ -310,10 +325,14 @@ int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz
     return NUM_OK;
 }
 
-int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block)
+int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block,
+                         int32_t chunk, int32_t nt64, int32_t nt32)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
-    int rc = build_schedule(A, 0, A->nrows, rows_per_block, nnz_per_block, A->sched);
+    if (rows_per_block <= 0) rows_per_block = kSpmvBS;
+    if (nnz_per_block <= 0) nnz_per_block = default_nnz_per_block(A);
+    int rc = build_schedule(A, 0, A->nrows, rows_per_block, nnz_per_block, chunk, nt64, nt32,
+                            A->sched);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(A->mu);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
@@ -322,7 +341,8 @@ int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_blo
 }
 
 int num_csr_schedule_info(const num_csr* A, int32_t* nblocks, int32_t* rows_per_block,
-                          int32_t* nnz_per_block, int32_t* long_rows, int32_t* max_row_nnz)
+                          int32_t* nnz_per_block, int32_t* long_rows, int32_t* max_row_nnz,
+                          int32_t* chunk, int32_t* nt64, int32_t* nt32)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
     if (nblocks) *nblocks = A->sched.nblocks;
@@ -330,6 +350,9 @@ int num_csr_schedule_info(const num_csr* A, int32_t* nblocks, int32_t* rows_per_
     if (nnz_per_block) *nnz_per_block = A->sched.nnz_per_block;
     if (long_rows) *long_rows = A->sched.long_rows;
     if (max_row_nnz) *max_row_nnz = A->max_row_nnz;
+    if (chunk) *chunk = A->sched.chunk;
+    if (nt64) *nt64 = A->sched.nt64;
+    if (nt32) *nt32 = A->sched.nt32;
     return NUM_OK;
 }
 

@@ -29,17 +29,25 @@
 
 namespace numamd {
 
-template <typename T>
-__device__ __forceinline__ T ld_stream(const T* p)
+// Matrix streams: non-temporal (NT) loads keep a matrix that is read once
+// per call from evicting x out of L2 / the Infinity Cache; a matrix small
+// enough to stay resident in the 256 MB Infinity Cache across calls (the
+// FP32 copy inside a CG solve) is loaded normally so it does stay.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_mat(const T* p)
 {
-    return __builtin_nontemporal_load(p);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
-// x accessors: one vector, or the [own | ghost] pair of a rank's slab where
-// columns >= n_own address the ghost block (no x_ext copy is ever built).
+// x accessors.  prepare() runs once per workgroup before any gather (it may
+// load scalars the accessor needs).  GatherX: one vector.  GatherSplit: the
+// [own | ghost] pair of a rank's slab, columns >= n_own address the ghost
+// block (no x_ext copy is ever built).
 template <typename T>
 struct GatherX {
     const T* __restrict__ x;
+    __device__ __forceinline__ void prepare() {}
     __device__ __forceinline__ T operator()(int c) const { return x[c]; }
 };
 
@@ -48,6 +56,7 @@ struct GatherSplit {
     const T* __restrict__ x_own;
     const T* __restrict__ x_ghost;
     int n_own;
+    __device__ __forceinline__ void prepare() {}
     __device__ __forceinline__ T operator()(int c) const
     {
         return c < n_own ? x_own[c] : x_ghost[c - n_own];
@@ -56,7 +65,7 @@ struct GatherSplit {
 
 // Body for one tile.  Leaves this thread's two epilogue contributions in
 // ca/cb (zero for lanes that finish no row).
-template <typename T, int BS, int NPT, class Epi, class Gather>
+template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_tile_body(
     int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const T* __restrict__ vals, const Gather& x, const TileEntry* __restrict__ sched,
@@ -81,8 +90,8 @@ __device__ __forceinline__ void spmv_tile_body(
             int c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                v[u] = ld_stream(vals + j + u * BS);
-                c[u] = ld_stream(colind + j + u * BS);
+                v[u] = ld_mat<NT>(vals + j + u * BS);
+                c[u] = ld_mat<NT>(colind + j + u * BS);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc += v[u] * x(c[u]);
@@ -96,7 +105,7 @@ __device__ __forceinline__ void spmv_tile_body(
             T s = T(0);
 #pragma unroll
             for (int w = 0; w < BS / kWave; ++w) s += lv[w];
-            epi(r0, s, ca, cb);
+            epi(r0, s, ca, cb, x);
         }
         return;
     }
@@ -118,8 +127,8 @@ __device__ __forceinline__ void spmv_tile_body(
 #pragma unroll
         for (int s = 0; s < NPT; ++s) {
             const int j = min(j0 + t + s * BS, last);
-            v[s] = ld_stream(vals + j);
-            c[s] = ld_stream(colind + j);
+            v[s] = ld_mat<NT>(vals + j);
+            c[s] = ld_mat<NT>(colind + j);
         }
 #pragma unroll
         for (int s = 0; s < NPT; ++s) {
@@ -163,12 +172,12 @@ __device__ __forceinline__ void spmv_tile_body(
     if (G > 1) {
         for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
     }
-    if (active && lane == 0) epi(r0 + lr, acc, ca, cb);
+    if (active && lane == 0) epi(r0 + lr, acc, ca, cb, x);
 }
 
 // Generic kernel: per-tile partial sums (if Epi::kRed) go to
 // epi.partial_a / partial_b[tile].
-template <typename T, int BS, int NPT, class Epi, class Gather = GatherX<T>>
+template <typename T, int BS, int NPT, bool NT, class Epi, class Gather = GatherX<T>>
 __global__ __launch_bounds__(BS) void spmv_tile_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const T* __restrict__ vals, Gather x, const TileEntry* __restrict__ sched, int ntiles,
@@ -180,8 +189,9 @@ __global__ __launch_bounds__(BS) void spmv_tile_kernel(
     const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
     if (tile < 0) return;
     if (epi.skip()) return;
+    x.prepare();
     double ca, cb;
-    spmv_tile_body<T, BS, NPT, Epi, Gather>(tile, rowptr, colind, vals, x, sched, epi, lv, lc,
+    spmv_tile_body<T, BS, NPT, NT, Epi, Gather>(tile, rowptr, colind, vals, x, sched, epi, lv, lc,
                                             ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
@@ -203,7 +213,8 @@ struct EpiY {
     double* partial_a = nullptr;
     double* partial_b = nullptr;
     __device__ __forceinline__ bool skip() const { return false; }
-    __device__ __forceinline__ void operator()(int row, T v, double&, double&) const
+    template <class Gather>
+    __device__ __forceinline__ void operator()(int row, T v, double&, double&, const Gather&) const
     {
         y[row] = v;
     }

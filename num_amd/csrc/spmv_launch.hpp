@@ -6,14 +6,35 @@
 
 namespace numamd {
 
-template <typename T, int NPT, class Epi, class Gather>
+template <typename T, int NPT, bool NT, class Epi, class Gather>
 inline void launch_tiles_npt(const num_csr* A, const Schedule& s, const T* vals,
                              const Gather& x, const Epi& epi, hipStream_t stream)
 {
-    spmv_tile_kernel<T, kSpmvBS, NPT, Epi, Gather>
+    spmv_tile_kernel<T, kSpmvBS, NPT, NT, Epi, Gather>
         <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
             A->rowptr, A->colind, vals, x, static_cast<const TileEntry*>(s.d_tiles), s.nblocks,
             s.chunk, epi);
+}
+
+template <typename T, bool NT, class Epi, class Gather>
+inline void launch_tiles_nt(const num_csr* A, const Schedule& s, const T* vals,
+                            const Gather& x, const Epi& epi, hipStream_t stream)
+{
+    switch (s.npt) {
+    case 5: launch_tiles_npt<T, 5, NT>(A, s, vals, x, epi, stream); break;
+    case 7: launch_tiles_npt<T, 7, NT>(A, s, vals, x, epi, stream); break;
+    case 12: launch_tiles_npt<T, 12, NT>(A, s, vals, x, epi, stream); break;
+    default: launch_tiles_npt<T, 16, NT>(A, s, vals, x, epi, stream); break;
+    }
+}
+
+// Streaming policy: the schedule's choice for this value type (an FP64 and
+// an FP32 copy of one pattern can differ: only the smaller may fit the
+// Infinity Cache).
+template <typename T>
+inline bool stream_nt(const Schedule& s)
+{
+    return sizeof(T) == 8 ? s.nt64 != 0 : s.nt32 != 0;
 }
 
 template <typename T, class Epi, class Gather>
@@ -21,12 +42,8 @@ inline void launch_tiles(const num_csr* A, const Schedule& s, const T* vals, con
                          const Epi& epi, hipStream_t stream)
 {
     if (s.nblocks == 0) return;
-    switch (s.npt) {
-    case 5: launch_tiles_npt<T, 5>(A, s, vals, x, epi, stream); break;
-    case 7: launch_tiles_npt<T, 7>(A, s, vals, x, epi, stream); break;
-    case 12: launch_tiles_npt<T, 12>(A, s, vals, x, epi, stream); break;
-    default: launch_tiles_npt<T, 16>(A, s, vals, x, epi, stream); break;
-    }
+    if (stream_nt<T>(s)) launch_tiles_nt<T, true>(A, s, vals, x, epi, stream);
+    else launch_tiles_nt<T, false>(A, s, vals, x, epi, stream);
 }
 
 }  // namespace numamd

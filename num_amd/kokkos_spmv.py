@@ -179,13 +179,17 @@ class CrsMatrix:
             self._values32 = self._values.to(torch.float32)
             call("num_csr_attach_f32", self._h, _ptr(self._values32))
 
-    def set_schedule(self, rows_per_block: int, nnz_per_block: int) -> None:
-        call("num_csr_set_schedule", self._h, int(rows_per_block), int(nnz_per_block))
+    def set_schedule(self, rows_per_block: int = 0, nnz_per_block: int = 0, chunk: int = -1,
+                     nt64: int = -1, nt32: int = -1) -> None:
+        """Rebuild the row-tile schedule (0 / -1 = default for that knob)."""
+        call("num_csr_set_schedule", self._h, int(rows_per_block), int(nnz_per_block),
+             int(chunk), int(nt64), int(nt32))
 
     def schedule_info(self) -> dict:
-        v = [C.c_int32() for _ in range(5)]
+        v = [C.c_int32() for _ in range(8)]
         call("num_csr_schedule_info", self._h, *[C.byref(x) for x in v])
-        keys = ("nblocks", "rows_per_block", "nnz_per_block", "long_rows", "max_row_nnz")
+        keys = ("nblocks", "rows_per_block", "nnz_per_block", "long_rows", "max_row_nnz",
+                "chunk", "nt64", "nt32")
         return {k: x.value for k, x in zip(keys, v)}
 
     def __del__(self):
@@ -371,15 +375,17 @@ def cg_solve_local(A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000)
     return _cg_result(res)
 
 
-def cg_solve_mixed(A: CrsMatrix, b, x, tol: float = 1e-10, max_outer: int = 50,
-                   inner_tol: float = 1e-4, inner_max: int = 100000) -> dict:
+def cg_solve_mixed(A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 100000,
+                   delta: float = 0.1) -> dict:
     """FP32-inner / FP64-refined CG (no reference counterpart).
 
-    Outer loop in FP64 on the GPU: r = b - A x, stop when
-    ||r||/||b|| < tol (strict), else solve A d = r with FP32 CG (FP32 matrix
-    copy and vectors, FP64-accumulated dots) to relative tolerance
-    ``inner_tol`` and update x += d in FP64.  Returns {'outer', 'inner_total',
-    'residual' (FP64 true residual), 'converged', 'iters' (= inner_total)}.
+    CG steps run in FP32 (FP32 matrix copy and vectors, FP64-accumulated
+    dots).  Each time the FP32 residual has dropped by ``delta`` since the last
+    refinement, the GPU refines in FP64: x += x32, r = b - A x (FP64 true
+    residual, also the convergence test, strict < tol), the FP32 residual is
+    re-seeded from r and the search direction kept.  Returns {'outer' (FP64
+    refinements), 'inner_total' / 'iters' (FP32 steps), 'residual' (FP64 true
+    residual), 'converged'}.
     """
     b = _dev_tensor(b, "b", 1)
     x = _dev_tensor(x, "x", 1)
@@ -392,9 +398,8 @@ def cg_solve_mixed(A: CrsMatrix, b, x, tol: float = 1e-10, max_outer: int = 50,
     nbytes = int(lib.num_cg_mixed_workspace_bytes(A._h))
     ws = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
     res = _lib.MixedResult()
-    rc = lib.num_cg_solve_mixed(A._h, _ptr(b), _ptr(x), float(tol), int(max_outer),
-                                float(inner_tol), int(inner_max), _ptr(ws), nbytes,
-                                C.byref(res), _stream())
+    rc = lib.num_cg_solve_mixed(A._h, _ptr(b), _ptr(x), float(tol), int(max_iter),
+                                float(delta), _ptr(ws), nbytes, C.byref(res), _stream())
     _sync()
     _lib.check(rc, "cg_solve_mixed")
     return {"outer": int(res.outer), "inner_total": int(res.inner_total),

@@ -61,7 +61,7 @@ def lib():
                                          C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.orc_cg_solve_local.restype = C.c_int
         L.orc_cg_mixed.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
-                                   C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                                   C.c_int32, C.c_double, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                    C.POINTER(C.c_int32)]
         L.orc_cg_mixed.restype = C.c_int
@@ -171,16 +171,18 @@ def cg_solve_local(rowptr, colind, vals, b, x0=None, tol=1e-10, max_iter=1000):
     return x, {"iters": it.value, "residual": res.value, "converged": bool(conv.value)}
 
 
-def cg_mixed(rowptr, colind, vals, b, x0=None, tol=1e-10, max_outer=50, inner_tol=1e-4,
-             inner_max=10000):
+def cg_mixed(rowptr, colind, vals, b, x0=None, tol=1e-10, max_iter=100000, delta=0.1):
+    """Reliable-update mixed-precision CG; returns (x, {'outer' (FP64
+    refinements), 'inner_total' (FP32 steps), 'true_residual', 'converged',
+    'breakdown'})."""
     rp, ci, v = _csr(rowptr, colind, vals)
     n = len(rp) - 1
     b = np.ascontiguousarray(b, np.float64)
     x = np.zeros(n, np.float64) if x0 is None else np.array(x0, np.float64, copy=True)
     oi, tot, res, conv = C.c_int32(), C.c_int32(), C.c_double(), C.c_int32()
-    rc = lib().orc_cg_mixed(n, rp, ci, v, b, x, tol, max_outer, inner_tol, inner_max,
-                            C.byref(oi), C.byref(tot), C.byref(res), C.byref(conv))
-    if rc != 0:
+    rc = lib().orc_cg_mixed(n, rp, ci, v, b, x, tol, max_iter, delta, C.byref(oi),
+                            C.byref(tot), C.byref(res), C.byref(conv))
+    if rc < 0:
         raise MemoryError("oracle mixed cg: allocation failed")
     return x, {"outer": oi.value, "inner_total": tot.value, "true_residual": res.value,
-               "converged": bool(conv.value)}
+               "converged": bool(conv.value), "breakdown": rc == 1}

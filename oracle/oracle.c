@@ -236,12 +236,16 @@ int orc_cg_solve_local(int32_t n, const int32_t* rowptr, const int32_t* colind,
 }
 
 /* ---- Mixed-precision refinement (no reference counterpart) --------------- */
-/* The oracle for the FP32-inner / FP64-refined form: the outer loop is the
- * FP64 residual r = b - A x (orc_spmv, orc_axpy order), the inner solve is
- * the reference CG sequence run in FP32 storage with dot products accumulated
- * in FP64, and the correction is added in FP64.  Pinned by the FP64 true
- * residual ||b - A x|| / ||b|| < tol it must reach (a property independent of
- * rounding details), checked against orc_cg_solve_local's solution. */
+/* The oracle for the FP32-inner / FP64-refined form with reliable updates:
+ * CG steps in FP32 storage (FP32 matrix copy, FP32 vectors, dot products
+ * accumulated in FP64, scalars in FP64), in the reference's CG order.  When
+ * the FP32 residual has dropped by `delta` since the last refinement, or
+ * looks converged, the iterate is refined in FP64: x64 += x32, x32 = 0,
+ * r = b - A x64 (orc_spmv / orc_axpy order), the FP32 residual is re-seeded
+ * from r, beta = r32.r32 / rr, and the search direction is kept.  The FP64
+ * true residual is the convergence test (strict <).  Pinned by that true
+ * residual ||b - A x|| / ||b|| < tol, a property independent of rounding
+ * details, and by the solution agreeing with orc_cg_solve_local's. */
 static double dot_f32_acc64(int64_t n, const float* a, const float* b)
 {
     double s = 0.0;
@@ -249,60 +253,85 @@ static double dot_f32_acc64(int64_t n, const float* a, const float* b)
     return s;
 }
 
-int orc_cg_mixed(int32_t n, const int32_t* rowptr, const int32_t* colind,
-                 const double* vals, const double* b, double* x, double tol,
-                 int32_t max_outer, double inner_tol, int32_t inner_max,
-                 int32_t* outer_iters, int32_t* inner_total, double* true_res,
-                 int32_t* converged)
+/* x64 += x32, x32 = 0, r = b - A x64 (FP64), r32 = (float) r.
+ * Returns ||r||^2 in *rr64 and r32.r32 in *rr32. */
+static void refine(int32_t n, const int32_t* rowptr, const int32_t* colind, const double* vals,
+                   const double* b, double* x, float* x32, double* r, float* r32,
+                   double* rr64, double* rr32)
+{
+    for (int32_t i = 0; i < n; ++i) { x[i] = x[i] + (double)x32[i]; x32[i] = 0.0f; }
+    orc_spmv(n, rowptr, colind, vals, x, r);
+    orc_axpy(n, -1.0, r, b, r);
+    double s64 = 0.0, s32 = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+        r32[i] = (float)r[i];
+        s64 += r[i] * r[i];
+        s32 += (double)r32[i] * (double)r32[i];
+    }
+    *rr64 = s64;
+    *rr32 = s32;
+}
+
+int orc_cg_mixed(int32_t n, const int32_t* rowptr, const int32_t* colind, const double* vals,
+                 const double* b, double* x, double tol, int32_t max_iter, double delta,
+                 int32_t* nref, int32_t* steps, double* true_res, int32_t* converged)
 {
     const int32_t nnz = rowptr[n];
     float* v32 = (float*)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
     double* r = (double*)malloc(sizeof(double) * (size_t)n);
-    double* Ax = (double*)malloc(sizeof(double) * (size_t)n);
     float* r32 = (float*)malloc(sizeof(float) * (size_t)n);
-    float* d32 = (float*)malloc(sizeof(float) * (size_t)n);
-    float* p32 = (float*)malloc(sizeof(float) * (size_t)n);
-    float* q32 = (float*)malloc(sizeof(float) * (size_t)n);
-    if (!v32 || !r || !Ax || !r32 || !d32 || !p32 || !q32) return -1;
+    float* x32 = (float*)calloc((size_t)n, sizeof(float));
+    float* pold = (float*)calloc((size_t)n, sizeof(float));
+    float* pnew = (float*)malloc(sizeof(float) * (size_t)n);
+    float* q = (float*)malloc(sizeof(float) * (size_t)n);
+    if (!v32 || !r || !r32 || !x32 || !pold || !pnew || !q) return -1;
     for (int32_t j = 0; j < nnz; ++j) v32[j] = (float)vals[j];
-
     double bn = sqrt(orc_dot(n, b, b));
     if (bn == 0.0) bn = 1.0;
-    int32_t tot = 0, outer = 0;
-    double rel = 0.0;
+    double rr64, rr32;
+    refine(n, rowptr, colind, vals, b, x, x32, r, r32, &rr64, &rr32);
+    int32_t refs = 1, it = 0, rc = 0;
+    double rel = sqrt(rr64) / bn;
     *converged = 0;
-    for (outer = 0; outer <= max_outer; ++outer) {
-        orc_spmv(n, rowptr, colind, vals, x, Ax);
-        orc_axpy(n, -1.0, Ax, b, r);
-        const double rn = sqrt(orc_dot(n, r, r));
-        rel = rn / bn;
-        if (rel < tol) { *converged = 1; break; }
-        if (outer == max_outer) break;
-        /* inner FP32 CG on A d = r / ||r||, d0 = 0 */
-        const double sc = 1.0 / rn;
-        for (int32_t i = 0; i < n; ++i) { r32[i] = (float)(r[i] * sc); d32[i] = 0.0f; p32[i] = r32[i]; }
-        double rr = dot_f32_acc64(n, r32, r32);
-        const double rr0 = rr;
-        for (int32_t it = 0; it < inner_max; ++it) {
-            orc_spmv_f32(n, rowptr, colind, v32, p32, q32);
-            const double pq = dot_f32_acc64(n, p32, q32);
-            const float alpha = (float)(rr / pq);
-            for (int32_t i = 0; i < n; ++i) {
-                d32[i] = alpha * p32[i] + d32[i];
-                r32[i] = -alpha * q32[i] + r32[i];
-            }
-            const double rrn = dot_f32_acc64(n, r32, r32);
-            ++tot;
-            if (sqrt(rrn / rr0) < inner_tol) break;
-            const float beta = (float)(rrn / rr);
-            rr = rrn;
-            for (int32_t i = 0; i < n; ++i) p32[i] = r32[i] + beta * p32[i];
+    if (rel < tol) { *converged = 1; goto out; }
+    if (max_iter <= 0) goto out;
+    double rr = rr32, rr_ref = rr32, beta = 0.0;
+    for (;;) {
+        const float bf = (float)beta;
+        for (int32_t i = 0; i < n; ++i) pnew[i] = r32[i] + bf * pold[i];
+        orc_spmv_f32(n, rowptr, colind, v32, pnew, q);
+        const double pq = dot_f32_acc64(n, pnew, q);
+        if (!(fabs(pq) >= 1e-300)) { rc = 1; break; }
+        const float al = (float)(rr / pq), nal = -al;
+        double rrn = 0.0;
+        for (int32_t i = 0; i < n; ++i) {
+            x32[i] = al * pnew[i] + x32[i];
+            r32[i] = nal * q[i] + r32[i];
+            rrn += (double)r32[i] * (double)r32[i];
         }
-        for (int32_t i = 0; i < n; ++i) x[i] = x[i] + rn * (double)d32[i];
+        ++it;
+        float* t = pold; pold = pnew; pnew = t;
+        const double est = sqrt(rrn) / bn;
+        const int final_ = it >= max_iter;
+        if (final_ || est < tol || sqrt(rrn) < delta * sqrt(rr_ref) || !(rrn == rrn)) {
+            refine(n, rowptr, colind, vals, b, x, x32, r, r32, &rr64, &rr32);
+            ++refs;
+            rel = sqrt(rr64) / bn;
+            if (rel < tol) { *converged = 1; break; }
+            if (final_ || !(rel == rel)) break;
+            beta = rr32 / rr;
+            rr = rr32;
+            rr_ref = rr32;
+        } else {
+            beta = rrn / rr;
+            rr = rrn;
+        }
     }
-    *outer_iters = outer;
-    *inner_total = tot;
+    for (int32_t i = 0; i < n; ++i) x[i] = x[i] + (double)x32[i];
+out:
+    *nref = refs;
+    *steps = it;
     *true_res = rel;
-    free(v32); free(r); free(Ax); free(r32); free(d32); free(p32); free(q32);
-    return 0;
+    free(v32); free(r); free(r32); free(x32); free(pold); free(pnew); free(q);
+    return rc;
 }
