@@ -163,6 +163,37 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
                        int32_t max_iter, double delta, void* d_work, int64_t work_bytes,
                        num_mixed_result* res, void* stream);
 
+/* ---- many systems at once: batched and served CG (no reference
+ * counterpart; the reference's batch is k right-hand sides on one matrix).
+ * An engine holds up to max_systems independent SPD systems, each a num_csr
+ * handle (schedule with <= 7 entries per thread, i.e. nnz_per_block <= 1792)
+ * plus caller-owned b, x and workspace, all resident on the device.  Every
+ * num_engine_step iteration advances all live systems with four launches;
+ * each system keeps its own scalars, convergence flag and iteration count,
+ * and computes exactly what it would alone (bit-identical). -------------- */
+typedef struct num_engine num_engine;
+/* nt64 / nt32: stream the FP64 / FP32 matrix values non-temporally. */
+int num_engine_create(int32_t max_systems, int32_t nt64, int32_t nt32, num_engine** out);
+int num_engine_destroy(num_engine* E);
+/* Workspace bytes for one system of n rows; mode 0 = FP64 CG, 1 = mixed. */
+int64_t num_engine_work_bytes(int32_t n, int32_t mode);
+/* Admit a system (joins the running batch at the next step).  mode 0: the
+ * reference CG sequence in FP64; mode 1: FP32 steps with FP64 reliable-update
+ * refinement (needs num_csr_attach_f32), stop on the FP64 true residual.
+ * x holds x0; on completion it holds the solution.  Returns the slot. */
+int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d_x, double tol,
+                   int32_t max_iter, int32_t mode, double delta, void* d_work,
+                   int64_t work_bytes, void* stream, int32_t* slot_out);
+/* Run `iters` iterations of every live system; finished systems leave the
+ * launch lists.  *n_active = systems still running. */
+int num_engine_step(num_engine* E, int32_t iters, int32_t* n_active, void* stream);
+/* Result of a finished system (mixed: refinements = FP64 evaluations). */
+int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* refinements,
+                      void* stream);
+/* Free a slot (its buffers are the caller's). */
+int num_engine_remove(num_engine* E, int32_t slot);
+int num_engine_info(const num_engine* E, int32_t* n_live, int32_t* n_tiles, int32_t* n_chunks);
+
 /* ---- halo pack / scatter of one rank's share (src/halo_exchange.hpp) --- */
 /* buf[i] = x[idx[i]] for i < count (halo_pack, halo_exchange.hpp:222-235). */
 int num_halo_pack(const int32_t* d_idx, int32_t count, const double* d_x, double* d_buf,

@@ -1,0 +1,115 @@
+"""Batched CG: many independent SPD systems advancing together on one GPU.
+
+``CGEngine`` wraps the native engine (include/num_amd.h, num_engine_*): each
+admitted system keeps its CSR handle, right-hand side, iterate and workspace
+resident in HBM, and every ``step(k)`` runs k CG iterations of all live
+systems with four launches per iteration (row tiles of every system in one
+SpMV launch, one workgroup per system for its scalars, 2048-row chunks for
+the vector updates).  A system that converges leaves the launch lists at the
+next step; a system admitted between steps joins the running batch.  Each
+system's arithmetic depends only on its own data, so its result is
+bit-identical whether it runs alone or with any other systems.
+
+Modes: FP64 (the reference's CG sequence, src/cg_solver.hpp:133-196) or mixed
+(FP32 CG steps with FP64-accumulated dots inside FP64 reliable-update
+refinement; convergence on the FP64 true residual).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import call
+from .kokkos_spmv import CrsMatrix, _dev_tensor, _ptr, _stream
+
+ENGINE_MAX_NNZ_PER_TILE = 7 * 256  # the engine's tile kernel holds 7 entries per thread
+
+
+class CGEngine:
+    """A pool of up to ``max_systems`` concurrent CG solves."""
+
+    def __init__(self, max_systems: int = 4096, nt64: bool = True, nt32: bool = True):
+        h = C.c_void_p()
+        call("num_engine_create", int(max_systems), int(bool(nt64)), int(bool(nt32)),
+             C.byref(h))
+        self._h = h
+        self._sys = {}
+        self.max_systems = int(max_systems)
+
+    @staticmethod
+    def work_bytes(n: int, mixed: bool = False) -> int:
+        return int(_lib.load().num_engine_work_bytes(int(n), 1 if mixed else 0))
+
+    def add(self, A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000,
+            mixed: bool = False, delta: float = 0.1, work=None) -> int:
+        """Admit A x = b (x holds x0, receives the solution).  Returns the slot."""
+        b = _dev_tensor(b, "b", 1)
+        x = _dev_tensor(x, "x", 1)
+        if A.nrows != A.ncols or b.shape[0] != A.nrows or x.shape[0] != A.nrows:
+            raise ValueError("engine.add: A must be square and b, x of length A.nrows")
+        if A.schedule_info()["nnz_per_block"] > ENGINE_MAX_NNZ_PER_TILE:
+            A.set_schedule(nnz_per_block=ENGINE_MAX_NNZ_PER_TILE)
+        if mixed:
+            A.ensure_f32()
+        nbytes = self.work_bytes(A.nrows, mixed)
+        if work is None:
+            work = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+        slot = C.c_int32()
+        call("num_engine_add", self._h, A.handle, _ptr(b), _ptr(x), float(tol), int(max_iter),
+             1 if mixed else 0, float(delta), _ptr(work), nbytes, _stream(), C.byref(slot))
+        self._sys[slot.value] = (A, b, x, work, bool(mixed))
+        return slot.value
+
+    def step(self, iters: int = 8) -> int:
+        """Run ``iters`` iterations of every live system; returns how many
+        are still running (the call returns after the device work)."""
+        n = C.c_int32()
+        call("num_engine_step", self._h, int(iters), C.byref(n), _stream())
+        return n.value
+
+    def result(self, slot: int) -> dict:
+        res = _lib.CgResult()
+        nref = C.c_int32()
+        call("num_engine_result", self._h, int(slot), C.byref(res), C.byref(nref), _stream())
+        return {"iters": int(res.iters), "residual": float(res.residual),
+                "converged": bool(res.converged), "breakdown": bool(res.breakdown),
+                "refinements": int(nref.value)}
+
+    def remove(self, slot: int) -> None:
+        call("num_engine_remove", self._h, int(slot))
+        self._sys.pop(int(slot), None)
+
+    def info(self) -> dict:
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        call("num_engine_info", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return {"live": a.value, "tiles": b.value, "chunks": c.value}
+
+    def run(self, iters_per_step: int = 16) -> None:
+        """Step until every admitted system has finished."""
+        while self.step(iters_per_step) > 0:
+            pass
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().num_engine_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def solve_batch(systems, tol: float = 1e-10, max_iter: int = 1000, mixed: bool = False,
+                delta: float = 0.1, iters_per_step: int = 16, engine: CGEngine | None = None):
+    """Solve every (A, b, x) of ``systems`` together; x is overwritten.
+    Returns the per-system result dicts in input order."""
+    eng = engine or CGEngine(max_systems=max(1, len(systems)))
+    slots = [eng.add(A, b, x, tol=tol, max_iter=max_iter, mixed=mixed, delta=delta)
+             for A, b, x in systems]
+    eng.run(iters_per_step)
+    out = [eng.result(s) for s in slots]
+    for s in slots:
+        eng.remove(s)
+    return out

@@ -1,0 +1,138 @@
+"""GPU tests of the batched CG engine: every system of a batch must match the
+single-system solver / oracle, a result must be bit-identical alone and in
+any mix, systems admitted into a running batch must behave the same, and the
+edge cases of tests/test_cg.cpp (zero RHS, max_iter, breakdown) must hold per
+system without disturbing the others."""
+import numpy as np
+import pytest
+import torch
+
+from num_amd import generators as g
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _systems(count, seed=4, nmax=5000):
+    return g.batched_systems(count=count, nmin=16, nmax=nmax, seed=seed)
+
+
+def _upload(ks, dims, shift):
+    A = g.stencil(dims, shift)
+    return A, ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+
+
+def test_batch_matches_single_and_oracle(ks, orc):
+    from num_amd.batched import solve_batch
+    sysl = _systems(40)
+    mats, entries = [], []
+    for dims, shift, b in sysl:
+        A, dA = _upload(ks, dims, shift)
+        mats.append((A, b))
+        entries.append((dA, _dev(b), _dev(np.zeros(A.nrows))))
+    res = solve_batch(entries, tol=1e-10, max_iter=5000)
+    for (A, b), (dA, db, x), r in zip(mats, entries, res):
+        assert r["converged"]
+        xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=5000)
+        assert abs(r["iters"] - ro["iters"]) <= max(1, int(0.02 * ro["iters"]))
+        xs = x.cpu().numpy()
+        assert np.linalg.norm(xs - xo) / np.linalg.norm(xo) < 1e-8
+        true = np.linalg.norm(b - A.to_scipy() @ xs) / np.linalg.norm(b)
+        assert true < 2e-10
+
+
+def test_alone_equals_in_mix_bitwise(ks):
+    from num_amd.batched import CGEngine
+    sysl = _systems(12, seed=9)
+    ups = [_upload(ks, d, s) for d, s, _ in sysl]
+    # all together
+    eng = CGEngine(max_systems=16)
+    xs_mix = []
+    slots = []
+    for (A, dA), (_, _, b) in zip(ups, sysl):
+        x = _dev(np.zeros(A.nrows))
+        xs_mix.append(x)
+        slots.append(eng.add(dA, _dev(b), x, tol=1e-10, max_iter=5000))
+    eng.run(7)
+    mix = [eng.result(s) for s in slots]
+    # each alone, in a fresh engine
+    for k, ((A, dA), (_, _, b)) in enumerate(zip(ups, sysl)):
+        e1 = CGEngine(max_systems=1)
+        x = _dev(np.zeros(A.nrows))
+        s = e1.add(dA, _dev(b), x, tol=1e-10, max_iter=5000)
+        e1.run(5)
+        r = e1.result(s)
+        assert r["iters"] == mix[k]["iters"] and r["residual"] == mix[k]["residual"]
+        assert torch.equal(x, xs_mix[k])
+
+
+def test_admission_into_running_batch(ks):
+    from num_amd.batched import CGEngine
+    sysl = _systems(10, seed=5)
+    ups = [_upload(ks, d, s) for d, s, _ in sysl]
+    eng = CGEngine(max_systems=16)
+    xs, slots = [], []
+    for i, ((A, dA), (_, _, b)) in enumerate(zip(ups, sysl)):
+        x = _dev(np.zeros(A.nrows))
+        xs.append(x)
+        slots.append(eng.add(dA, _dev(b), x, tol=1e-10, max_iter=5000))
+        eng.step(3)  # the earlier systems are mid-solve when the next arrives
+    eng.run(8)
+    for i, ((A, dA), (_, _, b)) in enumerate(zip(ups, sysl)):
+        e1 = CGEngine(max_systems=1)
+        x = _dev(np.zeros(A.nrows))
+        s = e1.add(dA, _dev(b), x, tol=1e-10, max_iter=5000)
+        e1.run(16)
+        assert torch.equal(x, xs[i])
+        assert e1.result(s)["iters"] == eng.result(slots[i])["iters"]
+
+
+def test_mixed_batch_true_residual(ks):
+    from num_amd.batched import solve_batch
+    sysl = _systems(20, seed=6)
+    mats, entries = [], []
+    for dims, shift, b in sysl:
+        A, dA = _upload(ks, dims, shift)
+        mats.append((A, b))
+        entries.append((dA, _dev(b), _dev(np.zeros(A.nrows))))
+    res = solve_batch(entries, tol=1e-10, max_iter=20000, mixed=True)
+    for (A, b), (dA, db, x), r in zip(mats, entries, res):
+        assert r["converged"] and r["residual"] < 1e-10 and r["refinements"] >= 1
+        xs = x.cpu().numpy()
+        true = np.linalg.norm(b - A.to_scipy() @ xs) / np.linalg.norm(b)
+        assert true < 1e-10
+
+
+def test_edge_cases_per_system(ks):
+    from num_amd.batched import CGEngine
+    eng = CGEngine(max_systems=8)
+    A, dA = _upload(ks, (20,), 0.0)
+    # zero RHS -> 0 iterations, residual 0 (tests/test_cg.cpp [5])
+    x0 = _dev(np.zeros(20))
+    s0 = eng.add(dA, _dev(np.zeros(20)), x0, tol=1e-10, max_iter=100)
+    # max_iter = 5 (tests/test_cg.cpp [7])
+    A2, dA2 = _upload(ks, (50,), 0.0)
+    x1 = _dev(np.zeros(50))
+    s1 = eng.add(dA2, _dev(np.ones(50)), x1, tol=1e-14, max_iter=5)
+    # breakdown: singular 2x2 with p'Ap = 0
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    v = np.array([1.0, -1.0, -1.0, 1.0])
+    dS = ks.upload_csr(rp, ci, v, 2, 2)
+    s2 = eng.add(dS, _dev(np.ones(2)), _dev(np.zeros(2)), tol=1e-10, max_iter=10)
+    # a healthy system beside them
+    A3, dA3 = _upload(ks, (30, 20), 0.1)
+    b3 = np.random.default_rng(1).uniform(-1, 1, A3.nrows)
+    x3 = _dev(np.zeros(A3.nrows))
+    s3 = eng.add(dA3, _dev(b3), x3, tol=1e-10, max_iter=2000)
+    eng.run(4)
+    r0, r1, r2, r3 = (eng.result(s) for s in (s0, s1, s2, s3))
+    assert r0["iters"] == 0 and r0["converged"] and r0["residual"] == 0.0
+    assert r1["iters"] == 5 and not r1["converged"]
+    assert r2["breakdown"] and not r2["converged"]
+    assert r3["converged"]
+    true = np.linalg.norm(b3 - A3.to_scipy() @ x3.cpu().numpy()) / np.linalg.norm(b3)
+    assert true < 2e-10
