@@ -190,9 +190,20 @@ int num_engine_step(num_engine* E, int32_t iters, int32_t* n_active, void* strea
 /* Result of a finished system (mixed: refinements = FP64 evaluations). */
 int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* refinements,
                       void* stream);
+/* Slots whose system finished and has not been removed yet (up to
+ * max_slots written; *count = how many there are). */
+int num_engine_finished(const num_engine* E, int32_t* slots, int32_t max_slots, int32_t* count);
 /* Free a slot (its buffers are the caller's). */
 int num_engine_remove(num_engine* E, int32_t slot);
 int num_engine_info(const num_engine* E, int32_t* n_live, int32_t* n_tiles, int32_t* n_chunks);
+
+/* Build the values and column indices of the 1-/2-/3-D Dirichlet stencil
+ * (5-point in 2-D, 7-point in 3-D; diagonal 2*ndim + shift, off-diagonals
+ * -1, columns sorted, x fastest) on an nx x ny x nz grid, on the device,
+ * given its rowptr (the closed form of python/matrix_generators.py:16-80 with
+ * a shift; no reference counterpart as a device routine). */
+int num_stencil_fill(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, double shift,
+                     const int32_t* d_rowptr, int32_t* d_colind, double* d_vals, void* stream);
 
 /* ---- halo pack / scatter of one rank's share (src/halo_exchange.hpp) --- */
 /* buf[i] = x[idx[i]] for i < count (halo_pack, halo_exchange.hpp:222-235). */

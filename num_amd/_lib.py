@@ -70,8 +70,10 @@ SIGNATURES = {
     "num_engine_add": (C.c_int, [vp, vp, vp, vp, f64, i32, i32, f64, vp, i64, vp, pi32]),
     "num_engine_step": (C.c_int, [vp, i32, pi32, vp]),
     "num_engine_result": (C.c_int, [vp, i32, C.POINTER(CgResult), pi32, vp]),
+    "num_engine_finished": (C.c_int, [vp, pi32, i32, pi32]),
     "num_engine_remove": (C.c_int, [vp, i32]),
     "num_engine_info": (C.c_int, [vp, pi32, pi32, pi32]),
+    "num_stencil_fill": (C.c_int, [i32, i32, i32, i32, f64, vp, vp, vp, vp]),
     "num_halo_pack": (C.c_int, [vp, i32, vp, vp, vp]),
     "num_halo_scatter": (C.c_int, [vp, i32, vp, i32, vp]),
 }

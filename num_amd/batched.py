@@ -77,6 +77,16 @@ class CGEngine:
                 "converged": bool(res.converged), "breakdown": bool(res.breakdown),
                 "refinements": int(nref.value)}
 
+    def finished(self) -> list:
+        """Slots whose system has finished and not been removed."""
+        cnt = C.c_int32()
+        call("num_engine_finished", self._h, None, 0, C.byref(cnt))
+        if cnt.value == 0:
+            return []
+        buf = (C.c_int32 * cnt.value)()
+        call("num_engine_finished", self._h, buf, cnt.value, C.byref(cnt))
+        return list(buf)
+
     def remove(self, slot: int) -> None:
         call("num_engine_remove", self._h, int(slot))
         self._sys.pop(int(slot), None)

@@ -695,8 +695,9 @@ int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* 
     SysState h;
     NUM_HIP(hipMemcpyAsync(&h, E->d_state + slot, sizeof(SysState), hipMemcpyDeviceToHost, st));
     NUM_HIP(hipStreamSynchronize(st));
+    if (h.phase != PH_DONE) return fail(NUM_ERUNTIME, "system %d is still running", slot);
     const SysDesc& d = E->h_desc[slot];
-    if (d.mode == 1 && (h.xsel || h.breakdown || h.phase != PH_DONE)) {
+    if (d.mode == 1 && (h.xsel || h.breakdown)) {
         // the iterate lives in x64[xsel] + x32[xsel]; bring it to the caller's x
         const double* x64 = h.xsel ? d.x64b : d.x;
         const float* x32 = h.xsel ? d.x32b : d.x32a;
@@ -709,7 +710,21 @@ int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* 
     res->breakdown = h.breakdown;
     res->residual = h.rel;
     if (refinements) *refinements = h.nref;
-    return h.phase == PH_DONE ? NUM_OK : fail(NUM_ERUNTIME, "system %d is still running", slot);
+    return NUM_OK;
+}
+
+int num_engine_finished(const num_engine* E, int32_t* slots, int32_t max_slots, int32_t* count)
+{
+    if (!E || !count) return fail(NUM_EINVAL, "engine or count is NULL");
+    int32_t c = 0;
+    for (int32_t s = 0; s < E->cap; ++s) {
+        if (E->used[s] && !E->live[s]) {
+            if (slots && c < max_slots) slots[c] = s;
+            ++c;
+        }
+    }
+    *count = c;
+    return NUM_OK;
 }
 
 int num_engine_remove(num_engine* E, int32_t slot)

@@ -62,6 +62,32 @@ static void launch_batch(const num_csr* A, const double* X, double* Y, int k,
                                                     A->nrows, k);
 }
 
+// One row of the Dirichlet stencil on an nx x ny x nz grid (x fastest):
+// entries in column order -z, -y, -x, self, +x, +y, +z, off-diagonals -1,
+// diagonal 2*ndim + shift.
+__global__ void stencil_fill_kernel(int ndim, int nx, int ny, int nz, double shift,
+                                    const int32_t* __restrict__ rowptr,
+                                    int32_t* __restrict__ colind, double* __restrict__ vals)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = (int)(i % nx);
+    const int iy = (int)((i / nx) % ny);
+    const int iz = (int)(i / ((int64_t)nx * ny));
+    const int sy = nx, sz = nx * ny;
+    int p = rowptr[i];
+    const int row = (int)i;
+    if (ndim >= 3 && iz > 0) { colind[p] = row - sz; vals[p++] = -1.0; }
+    if (ndim >= 2 && iy > 0) { colind[p] = row - sy; vals[p++] = -1.0; }
+    if (ix > 0) { colind[p] = row - 1; vals[p++] = -1.0; }
+    colind[p] = row;
+    vals[p++] = 2.0 * ndim + shift;
+    if (ix < nx - 1) { colind[p] = row + 1; vals[p++] = -1.0; }
+    if (ndim >= 2 && iy < ny - 1) { colind[p] = row + sy; vals[p++] = -1.0; }
+    if (ndim >= 3 && iz < nz - 1) { colind[p] = row + sz; vals[p++] = -1.0; }
+}
+
 __global__ void halo_pack_kernel(const int32_t* __restrict__ idx, int count,
                                  const double* __restrict__ x, double* __restrict__ buf)
 {
@@ -148,6 +174,22 @@ int num_batch_spmv(const num_csr* A, const double* d_X, double* d_Y, int32_t k, 
     else if (k <= 16) launch_batch<16>(A, d_X, d_Y, k, st);
     else if (k <= 32) launch_batch<32>(A, d_X, d_Y, k, st);
     else launch_batch<64>(A, d_X, d_Y, k, st);
+    NUM_LAUNCH_CHECK();
+    return NUM_OK;
+}
+
+int num_stencil_fill(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, double shift,
+                     const int32_t* d_rowptr, int32_t* d_colind, double* d_vals, void* stream)
+{
+    if (ndim < 1 || ndim > 3) return fail(NUM_EINVAL, "ndim must be 1, 2 or 3");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(NUM_EINVAL, "grid sizes must be >= 1");
+    if ((ndim < 3 && nz != 1) || (ndim < 2 && ny != 1))
+        return fail(NUM_EINVAL, "unused grid dimensions must be 1");
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (n * (2 * ndim + 1) >= (int64_t)1 << 31) return fail(NUM_EINVAL, "grid too large");
+    if (!d_rowptr || !d_colind || !d_vals) return fail(NUM_EINVAL, "NULL pointer");
+    stencil_fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        ndim, nx, ny, nz, shift, d_rowptr, d_colind, d_vals);
     NUM_LAUNCH_CHECK();
     return NUM_OK;
 }

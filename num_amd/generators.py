@@ -100,6 +100,22 @@ def _grid_stencil(dims, diag_of_row, off_val_fn=None):
     return CSR(rowptr.astype(np.int32), colind, values, n, n)
 
 
+def stencil_rowptr(dims) -> np.ndarray:
+    """rowptr of the Dirichlet stencil on ``dims`` (x fastest): row degree is
+    1 + sum over axes of (has lower neighbour) + (has upper neighbour)."""
+    dims = tuple(int(d) for d in dims)
+    deg = np.ones((1,) * len(dims), dtype=np.int32)
+    for a, d in enumerate(dims):
+        c = np.arange(d)
+        da = (c > 0).astype(np.int32) + (c < d - 1).astype(np.int32)
+        shape = [1] * len(dims)
+        shape[len(dims) - 1 - a] = d
+        deg = deg + da.reshape(shape)
+    rowptr = np.zeros(int(np.prod(dims)) + 1, dtype=np.int64)
+    np.cumsum(deg.reshape(-1), out=rowptr[1:])
+    return rowptr.astype(np.int32)
+
+
 def laplacian_2d(L: int) -> CSR:
     """5-point Dirichlet Laplacian on an L x L grid: n = L^2, nnz = 5n - 4L."""
     A = _grid_stencil((L, L), np.full(L * L, 4.0))

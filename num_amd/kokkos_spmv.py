@@ -234,6 +234,29 @@ def upload_csr(rowptr, colind, values, nrows: int, ncols: int) -> CrsMatrix:
     return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, nrows, ncols)
 
 
+def upload_stencil(dims, shift: float = 0.0) -> CrsMatrix:
+    """The 1-/2-/3-D Dirichlet Laplacian (5-point in 2-D, 7-point in 3-D) on
+    ``dims`` (x fastest) plus ``shift`` on the diagonal, assembled directly on
+    the GPU (the host computes only rowptr).  Same matrix as
+    ``generators.stencil(dims, shift)``."""
+    from .generators import stencil_rowptr
+    _ensure_init()
+    dims = tuple(int(d) for d in dims)
+    if not 1 <= len(dims) <= 3 or min(dims) < 1:
+        raise ValueError("dims must hold 1 to 3 positive sizes")
+    rp = stencil_rowptr(dims)
+    n = int(np.prod(dims))
+    nnz = int(rp[-1])
+    dev = torch.device("cuda", torch.cuda.current_device())
+    d_rp = torch.from_numpy(rp).to(dev)
+    d_ci = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    d_v = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+    full = dims + (1,) * (3 - len(dims))
+    call("num_stencil_fill", len(dims), full[0], full[1], full[2], float(shift), _ptr(d_rp),
+         _ptr(d_ci), _ptr(d_v), _stream())
+    return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, n, n)
+
+
 # ------------------------------------------------------------------ SpMV --
 
 def spmv(A: CrsMatrix, x, y, kokkoskernels: bool = True, *, sync: bool = True) -> None:
