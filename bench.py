@@ -103,6 +103,61 @@ def cg_bench(ks, A, dA, solves, tol=1e-10):
             "iter_GBps": round(iter_bytes * it / float(np.median(lat)) / 1e9, 1)}
 
 
+def time_launches(fn, steps, warmup):
+    """Average device time of fn() (launches on torch's current stream)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e-3 / steps
+
+
+def zipf_bench(ks, steps):
+    """SpMV on the irregular Zipf(2.0) matrix, n = 4M, rows clipped to
+    [1, 4096] (long rows take the multi-lane and whole-workgroup paths)."""
+    import torch
+    from num_amd import generators as g
+    Z = g.zipf_matrix(4_000_000, 2.0, 4096, seed=2)
+    dZ = ks.upload_csr(Z.rowptr, Z.colind, Z.values, Z.nrows, Z.ncols)
+    x = np.random.default_rng(1).uniform(-1, 1, Z.ncols)
+    dx = torch.from_numpy(x).cuda()
+    dy = torch.zeros(Z.nrows, dtype=torch.float64, device="cuda")
+    avg = time_launches(lambda: ks.spmv(dZ, dx, dy, sync=False), steps, 5)
+    B = spmv_bytes(Z.nrows, Z.ncols, Z.nnz)
+    return {"n": Z.nrows, "nnz": Z.nnz, "max_row": int(np.diff(Z.rowptr).max()),
+            "kernel_us": round(avg * 1e6, 2), "GBps": round(B / avg / 1e9, 1),
+            "frac": round(B / avg / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def slab_bench(ks, A, x, y_ref, steps):
+    """One GPU's share of the 8-slab row partition of the 192^3 Laplacian
+    (rank 3: 24 owned z-planes, 2 ghost planes): interior rows on x_local,
+    boundary rows on [x_local | x_ghost]."""
+    import torch
+    D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=3, size=8)
+    s, e = D.local_row_start, D.local_row_end
+    xl = torch.from_numpy(np.ascontiguousarray(x[s:e])).cuda()
+    xg = D.ghost_values(torch.from_numpy(x).cuda())
+    yl = torch.zeros(e - s, dtype=torch.float64, device="cuda")
+    avg = time_launches(lambda: ks.dist_spmv(D, xl, yl, xg, sync=False), steps, 5)
+    loc = D.local
+    B = 12 * loc.nnz + 4 * (loc.nrows + 1) + 8 * loc.ncols + 8 * loc.nrows
+    exact = y_ref is not None and bool(np.array_equal(yl.cpu().numpy(), y_ref[s:e]))
+    return {"rows": [s, e], "n_local": loc.nrows, "n_ghost": D.n_ghost, "nnz": loc.nnz,
+            "launches": len(D.interior_runs) + len(D.boundary_runs),
+            "us": round(avg * 1e6, 2), "GBps": round(B / avg / 1e9, 1),
+            "bytes": B, "cache_resident": B < 256 * 2**20,
+            "bit_exact_vs_oracle": exact if y_ref is not None else None}
+
+
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "spmv_lap3d192_pmc.txt")
 
 
@@ -138,6 +193,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the slab-share and Zipf SpMV measurements")
     ap.add_argument("--cg-solves", type=int, default=5)
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB copy used to scale PMC byte counters")
@@ -184,6 +241,12 @@ def main():
         cb, y_ref = cpu_baseline(A, x)
         out["cpu_baseline"] = cb
         out["parity_max_abs"] = float(np.max(np.abs(y_gpu - y_ref)))
+    if not args.no_extra:
+        log("bench: slab share (rank 3 of 8)")
+        out["spmv_slab_r3of8"] = slab_bench(ks, A, x, y_ref if not args.no_cpu else None,
+                                            args.steps)
+        log("bench: Zipf n=4M")
+        out["spmv_zipf4m"] = zipf_bench(ks, max(10, args.steps // 4))
     if not args.no_cg:
         log("bench: CG 1e-10")
         out["cg"] = cg_bench(ks, A, dA, args.cg_solves)

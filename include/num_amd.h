@@ -167,10 +167,13 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
  * counterpart; the reference's batch is k right-hand sides on one matrix).
  * An engine holds up to max_systems independent SPD systems, each a num_csr
  * handle (schedule with <= 7 entries per thread, i.e. nnz_per_block <= 1792)
- * plus caller-owned b, x and workspace, all resident on the device.  Every
- * num_engine_step iteration advances all live systems with four launches;
+ * or an assembled stencil, plus b, x and workspace, all resident on the
+ * device.  Every iteration advances all live systems with four launches;
  * each system keeps its own scalars, convergence flag and iteration count,
- * and computes exactly what it would alone (bit-identical). -------------- */
+ * and computes exactly what it would alone (bit-identical).  All calls on
+ * one engine must use one stream; none of them synchronises the host except
+ * num_engine_step, num_engine_poll(wait=1), and num_engine_launch when two
+ * steps are already in flight. -------------------------------------------- */
 typedef struct num_engine num_engine;
 /* nt64 / nt32: stream the FP64 / FP32 matrix values non-temporally. */
 int num_engine_create(int32_t max_systems, int32_t nt64, int32_t nt32, num_engine** out);
@@ -178,14 +181,33 @@ int num_engine_destroy(num_engine* E);
 /* Workspace bytes for one system of n rows; mode 0 = FP64 CG, 1 = mixed. */
 int64_t num_engine_work_bytes(int32_t n, int32_t mode);
 /* Admit a system (joins the running batch at the next step).  mode 0: the
- * reference CG sequence in FP64; mode 1: FP32 steps with FP64 reliable-update
- * refinement (needs num_csr_attach_f32), stop on the FP64 true residual.
- * x holds x0; on completion it holds the solution.  Returns the slot. */
+ * reference CG sequence in FP64 (src/cg_solver.hpp:133-196); mode 1: FP32
+ * steps with FP64 reliable-update refinement (needs num_csr_attach_f32), stop
+ * on the FP64 true residual.  x holds x0; on completion it holds the
+ * solution.  Returns the slot. */
 int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d_x, double tol,
                    int32_t max_iter, int32_t mode, double delta, void* d_work,
                    int64_t work_bytes, void* stream, int32_t* slot_out);
-/* Run `iters` iterations of every live system; finished systems leave the
- * launch lists.  *n_active = systems still running. */
+/* Device bytes of an assembled stencil system (see num_engine_add_stencil),
+ * or -1 for an invalid shape. */
+int64_t num_engine_stencil_bytes(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, int32_t mode);
+/* Admit A x = b for the Dirichlet stencil of num_stencil_fill on an
+ * nx x ny x nz grid, assembled on the device inside d_buf (>= the bytes
+ * above, 256-byte aligned): b[i] = 2u - 1 with u the SplitMix64 draw of
+ * (seed, i) (generators.hash_uniform), x0 = 0.  The solution is the first n
+ * doubles of d_buf. */
+int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, int32_t nz,
+                           double shift, uint64_t seed, double tol, int32_t max_iter,
+                           int32_t mode, double delta, void* d_buf, int64_t buf_bytes,
+                           void* stream, int32_t* slot_out);
+/* Queue `iters` iterations of every live system and return; at most two
+ * steps are in flight (with two queued, the older is waited for first). */
+int num_engine_launch(num_engine* E, int32_t iters, void* stream);
+/* Retire finished steps (all of them, waiting, if wait != 0): systems they
+ * finished leave the launch lists.  *n_live = systems not known finished,
+ * *n_inflight = steps still queued. */
+int num_engine_poll(num_engine* E, int32_t wait, int32_t* n_live, int32_t* n_inflight);
+/* num_engine_launch + num_engine_poll(wait = 1). */
 int num_engine_step(num_engine* E, int32_t iters, int32_t* n_active, void* stream);
 /* Result of a finished system (mixed: refinements = FP64 evaluations). */
 int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* refinements,
@@ -193,7 +215,8 @@ int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* 
 /* Slots whose system finished and has not been removed yet (up to
  * max_slots written; *count = how many there are). */
 int num_engine_finished(const num_engine* E, int32_t* slots, int32_t max_slots, int32_t* count);
-/* Free a slot (its buffers are the caller's). */
+/* Free a slot (its buffers are the caller's; a step still in flight may
+ * touch them, in stream order). */
 int num_engine_remove(num_engine* E, int32_t slot);
 int num_engine_info(const num_engine* E, int32_t* n_live, int32_t* n_tiles, int32_t* n_chunks);
 

@@ -68,6 +68,11 @@ SIGNATURES = {
     "num_engine_destroy": (C.c_int, [vp]),
     "num_engine_work_bytes": (i64, [i32, i32]),
     "num_engine_add": (C.c_int, [vp, vp, vp, vp, f64, i32, i32, f64, vp, i64, vp, pi32]),
+    "num_engine_stencil_bytes": (i64, [i32, i32, i32, i32, i32]),
+    "num_engine_add_stencil": (C.c_int, [vp, i32, i32, i32, i32, f64, C.c_uint64, f64, i32, i32,
+                                         f64, vp, i64, vp, pi32]),
+    "num_engine_launch": (C.c_int, [vp, i32, vp]),
+    "num_engine_poll": (C.c_int, [vp, i32, pi32, pi32]),
     "num_engine_step": (C.c_int, [vp, i32, pi32, vp]),
     "num_engine_result": (C.c_int, [vp, i32, C.POINTER(CgResult), pi32, vp]),
     "num_engine_finished": (C.c_int, [vp, pi32, i32, pi32]),

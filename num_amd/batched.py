@@ -13,6 +13,12 @@ bit-identical whether it runs alone or with any other systems.
 Modes: FP64 (the reference's CG sequence, src/cg_solver.hpp:133-196) or mixed
 (FP32 CG steps with FP64-accumulated dots inside FP64 reliable-update
 refinement; convergence on the FP64 true residual).
+
+Systems come either as a CrsMatrix with caller-owned b / x (``add``) or as a
+Dirichlet stencil assembled on the device in one buffer (``add_stencil``,
+the serving path: one allocation and two kernel launches per request, no
+host synchronisation).  ``launch`` / ``poll`` pipeline the steps: the host
+admits and retires systems while the device runs the previous step.
 """
 from __future__ import annotations
 
@@ -25,6 +31,37 @@ from ._lib import call
 from .kokkos_spmv import CrsMatrix, _dev_tensor, _ptr, _stream
 
 ENGINE_MAX_NNZ_PER_TILE = 7 * 256  # the engine's tile kernel holds 7 entries per thread
+
+
+def _al256(b: int) -> int:
+    return (b + 255) & ~255
+
+
+def stencil_layout(dims, mixed: bool) -> dict:
+    """Byte offsets inside an assembled stencil system's buffer (the layout
+    num_engine_add_stencil carves): x, b, rowptr, colind, values[, values32],
+    workspace."""
+    from .generators import stencil_nnz
+    n = 1
+    for d in dims:
+        n *= int(d)
+    nnz = stencil_nnz(dims)
+    off, o = {}, 0
+    for name, size in (("x", 8 * n), ("b", 8 * n), ("rowptr", 4 * (n + 1)),
+                       ("colind", 4 * nnz), ("values", 8 * nnz)) + \
+            ((("values32", 4 * nnz),) if mixed else ()):
+        off[name] = o
+        o += _al256(size)
+    off["work"] = o
+    off["n"], off["nnz"] = n, nnz
+    return off
+
+
+def _grid(dims):
+    dims = tuple(int(d) for d in dims)
+    if not 1 <= len(dims) <= 3:
+        raise ValueError("dims must have 1, 2 or 3 entries")
+    return (len(dims),) + dims + (1,) * (3 - len(dims))
 
 
 class CGEngine:
@@ -61,6 +98,55 @@ class CGEngine:
              1 if mixed else 0, float(delta), _ptr(work), nbytes, _stream(), C.byref(slot))
         self._sys[slot.value] = (A, b, x, work, bool(mixed))
         return slot.value
+
+    @staticmethod
+    def stencil_bytes(dims, mixed: bool = False) -> int:
+        b = int(_lib.load().num_engine_stencil_bytes(*_grid(dims), 1 if mixed else 0))
+        if b < 0:
+            raise ValueError(f"invalid stencil grid {tuple(dims)}")
+        return b
+
+    def add_stencil(self, dims, shift: float, seed: int, tol: float = 1e-10,
+                    max_iter: int = 1000, mixed: bool = False, delta: float = 0.1,
+                    buf=None, nbytes: int | None = None) -> int:
+        """Admit the Dirichlet stencil system on grid ``dims`` (x fastest) with
+        diagonal shift ``shift`` and b = generators.hash_uniform(n, seed),
+        x0 = 0, assembled on the device inside one buffer (allocated here
+        unless given).  Nothing waits for the device.  Returns the slot."""
+        g = _grid(dims)
+        if nbytes is None:
+            nbytes = self.stencil_bytes(dims, mixed)
+        if buf is None:
+            buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        elif not (buf.is_cuda and buf.dtype == torch.uint8 and buf.is_contiguous()
+                  and buf.numel() >= nbytes):
+            raise ValueError(f"add_stencil: buf must be a contiguous CUDA uint8 tensor of "
+                             f">= {nbytes} bytes")
+        slot = C.c_int32()
+        call("num_engine_add_stencil", self._h, *g, float(shift), int(seed) % 2**64, float(tol),
+             int(max_iter), 1 if mixed else 0, float(delta), _ptr(buf), buf.numel(), _stream(),
+             C.byref(slot))
+        n = g[1] * g[2] * g[3]
+        self._sys[slot.value] = (buf, buf[: 8 * n].view(torch.float64))
+        return slot.value
+
+    def solution(self, slot: int) -> torch.Tensor:
+        """The solution vector of a system admitted with add_stencil (a view
+        of its buffer) or the x given to add."""
+        ent = self._sys[int(slot)]
+        return ent[1] if len(ent) == 2 else ent[2]
+
+    def launch(self, iters: int = 8) -> None:
+        """Queue ``iters`` iterations of every live system and return (at most
+        two steps stay queued; the older one is waited for first)."""
+        call("num_engine_launch", self._h, int(iters), _stream())
+
+    def poll(self, wait: bool = False) -> tuple:
+        """Retire completed steps (all, waiting, if ``wait``); returns
+        (systems not known finished, steps still queued)."""
+        a, b = C.c_int32(), C.c_int32()
+        call("num_engine_poll", self._h, 1 if wait else 0, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def step(self, iters: int = 8) -> int:
         """Run ``iters`` iterations of every live system; returns how many

@@ -66,10 +66,17 @@ inline int chunk_grid(int ntiles, int chunk)
 // {r_end, 0, rowptr[r_end], 0}.
 struct TileEntry {
     int32_t row;    // first row
-    int32_t lanes;  // lanes per row in the walk phase (power of two <= 64)
+    int32_t wide;   // walk mode: kWalkShort, kWalkMedium or kWalkLong
     int32_t nz0;    // rowptr[row]
     int32_t pad;
 };
+// Row classes.  A row of up to kShortRow entries is walked by one thread; a
+// longer row of a staged tile by a whole wave; a row of more than kLongRow
+// entries gets a tile (workgroup) of its own and is read straight from
+// global memory.
+constexpr int kShortRow = 32;
+constexpr int kLongRow = 256;
+enum : int32_t { kWalkShort = 0, kWalkMedium = 1, kWalkLong = 2 };
 
 // ------------------------------------------------------------ reductions ---
 // Deterministic butterfly sum over a whole wave64: every lane ends with the

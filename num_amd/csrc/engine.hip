@@ -2,26 +2,39 @@
 // (the batched and served forms; no reference counterpart: the reference's
 // "batch" is k right-hand sides on one matrix, src/batch_spmv_kernel.hpp:4).
 //
-// Each admitted system keeps its CSR handle (with its own row-tile
-// schedule), its vectors and a per-system state record resident in HBM.  One
-// engine iteration ("slot") is four launches over ALL active systems:
-//   KA  row tiles of every active system (one workgroup per tile): the SpMV
+// Each admitted system keeps its matrix (its own row tiles), its vectors and
+// a per-system state record resident in HBM.  One engine iteration is four
+// launches over ALL live systems:
+//   KA  row tiles of every live system (one workgroup per tile): the SpMV
 //       of the system's current phase (set-up, FP64 step, FP64 refinement or
 //       FP32 step) with its fused epilogue; per-tile partial sums
-//   KB  one workgroup per active system: sums that system's tile partials in
+//   KB  one workgroup per live system: sums that system's tile partials in
 //       index order -> alpha (or the set-up / refinement finish)
-//   KC  2048-row chunks of every active system: the vector updates, per-chunk
+//   KC  2048-row chunks of every live system: the vector updates, per-chunk
 //       partials
-//   KD  one workgroup per active system: beta, convergence, the refinement
+//   KD  one workgroup per live system: beta, convergence, the refinement
 //       trigger of the mixed form
+// A step (k iterations) ends with two more launches: the mixed form's
+// finished systems fold x64 + x32 into the caller's x, and one record per
+// live system (phase, iterations, residual) is packed and copied to pinned
+// host memory behind an event.  Steps are pipelined: num_engine_launch
+// returns as soon as the step is queued, and the host learns which systems
+// finished when the step's event completes, while the next step runs.
+//
 // The per-system phase is the convergence mask: a finished system's tiles
-// and chunks return at once, and the host drops it from the launch lists at
-// the next step.  A newly admitted system joins the running batch at the next
-// step; its first slot evaluates r = b - A x0.  Every reduction of system s
-// reads only s's partials, laid out in s's own tile / chunk order, so a
-// system's result is bit-identical whatever else shares the batch.
+// and chunks return at once, and the host drops it from the launch lists
+// when it retires the step that finished it.  A newly admitted system joins
+// at the next step; its state record and descriptor are written by a
+// one-thread kernel from launch arguments (no host synchronisation).  The
+// launch lists are expanded on the device from per-system prefix sums, so a
+// change of the live set costs O(live systems) host work.  Every reduction
+// of system s reads only s's partials, laid out in s's own tile / chunk
+// order, so a system's result is bit-identical whatever else shares the
+// batch and whenever it was admitted.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <deque>
 #include <new>
 #include <vector>
 
@@ -37,18 +50,20 @@ enum : int32_t {
     PH_RUN64 = 2,   // FP64 CG steps
     PH_REFINE = 3,  // mixed: next KA refines in FP64
     PH_RUN32 = 4,   // mixed: FP32 CG steps
-    PH_DONE = 5,
+    PH_DONE = 5,    // finished, x final
+    PH_FIN = 6,     // mixed, finished: x = x64 + x32 still to be folded
 };
 
 constexpr int kEngNpt = 7;        // LDS entries per thread of the engine's tile kernel
 constexpr int kChunkRows = 2048;  // rows per KC workgroup (256 threads x 8)
+constexpr int kStencilRpt = 256;  // rows per tile of an assembled stencil (<= 7 entries/row)
 
 struct SysDesc {
     const int32_t* rowptr;
     const int32_t* colind;
     const double* vals;
     const float* vals32;
-    const TileEntry* tiles;
+    const TileEntry* tiles;  // nullptr: uniform tiles of rpt rows
     const double* b;
     double* x;
     double* r;
@@ -62,19 +77,28 @@ struct SysDesc {
     float* q32;
     float* p32a;
     float* p32b;
-    int32_t n, ntiles, mode, pad;
-    int32_t bt_begin, bt_end, ch_begin, ch_end;
+    int32_t n, ntiles, mode, rpt;
 };
 
 struct SysState {
     double rr, rr_ref, alpha, beta, bnorm, tol, delta, rel;
-    int32_t it, max_iter, phase, post, converged, breakdown, psel, xsel, nref, final_;
-    int32_t pad[6];
+    int32_t it, max_iter, phase, post, converged, breakdown, psel, xsel, nref, final_, gen;
+    int32_t pad[5];
 };
 static_assert(sizeof(SysState) == 128, "state record is 128 bytes");
 
+// What the host learns about a live system at the end of a step.
+struct EngRec {
+    int32_t phase, it, converged, breakdown, nref, gen;
+    double rel;
+};
+static_assert(sizeof(EngRec) == 32, "record is 32 bytes");
+
+// One entry of the tile list: the system and its tile bounds (rows [r0, r1),
+// entries [nz0, nz1), walk mode), resolved once when the list is built so a
+// KA workgroup starts its stage loads after a single dependent load.
 struct BTile {
-    int32_t sys, tile;
+    int32_t sys, tile, r0, r1, nz0, nz1, wide, pad;
 };
 struct Chunk {
     int32_t sys, row0, row1, pad;
@@ -132,23 +156,35 @@ struct EEpiStep {  // p_new row to the other buffer, Ap, p_new . Ap
     }
 };
 
-struct EEpiRefine {  // x64' = x64 + x32, x32' = 0, r = b - Ax (FP64), r32
+// x64' = x64 + x32, x32' = 0, r = b - Ax (FP64), r32 = (float)r; partials
+// r.r and r32.r32 -- or b.b instead of r32.r32 on the first refinement,
+// which also supplies ||b||.
+struct EEpiRefine {
     const double* __restrict__ b;
     double* __restrict__ x64n;
     float* __restrict__ x32n;
     float* __restrict__ r32;
+    bool first;
     __device__ __forceinline__ void operator()(int row, double v, double& ca, double& cb,
                                                const EGatherXsum& g) const
     {
         x64n[row] = g(row);
         x32n[row] = 0.0f;
-        const double ri = -1.0 * v + b[row];
+        const double bi = b[row];
+        const double ri = -1.0 * v + bi;
         const float rf = (float)ri;
         r32[row] = rf;
         ca = ri * ri;
-        cb = (double)rf * (double)rf;
+        cb = first ? bi * bi : (double)rf * (double)rf;
     }
 };
+
+// A finished mixed system whose iterate lives in x64b / x32 goes through
+// PH_FIN so its x is folded before the host hears it is done.
+__device__ __forceinline__ int32_t done_phase(const SysDesc* d, const SysState* st)
+{
+    return (d->mode == 1 && (st->xsel || st->breakdown)) ? PH_FIN : PH_DONE;
+}
 
 // ------------------------------------------------------------------ kernels
 template <bool NT64, bool NT32>
@@ -167,33 +203,35 @@ __global__ __launch_bounds__(kSpmvBS) void eng_ka(const BTile* __restrict__ bts,
     const SysDesc* d = sd + e.sys;
     const SysState* st = ss + e.sys;
     const int ph = st->phase;
+    if (ph != PH_INIT && ph != PH_RUN64 && ph != PH_REFINE && ph != PH_RUN32) return;
+    const TileEntry e0 = {e.r0, e.wide, e.nz0, 0};
+    const TileEntry e1 = {e.r1, kWalkShort, e.nz1, 0};
     double ca = 0.0, cb = 0.0;
     if (ph == PH_INIT) {
         GatherX<double> g{d->x};
         EEpiInit epi{d->b, d->r, d->p0};
-        spmv_tile_body<double, kSpmvBS, kEngNpt, NT64>(e.tile, d->rowptr, d->colind, d->vals,
-                                                       g, d->tiles, epi, lv, lc, ca, cb);
+        spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind, d->vals, g,
+                                                      epi, lv, lc, ca, cb);
     } else if (ph == PH_RUN64) {
         const int ps = st->psel;
         EGatherPNew<double> g{d->r, ps ? d->p1 : d->p0, st->beta};
         EEpiStep<double> epi{ps ? d->p0 : d->p1, d->Ap};
-        spmv_tile_body<double, kSpmvBS, kEngNpt, NT64>(e.tile, d->rowptr, d->colind, d->vals,
-                                                       g, d->tiles, epi, lv, lc, ca, cb);
+        spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind, d->vals, g,
+                                                      epi, lv, lc, ca, cb);
     } else if (ph == PH_REFINE) {
         const int xs = st->xsel;
         EGatherXsum g{xs ? d->x64b : d->x, xs ? d->x32b : d->x32a};
-        EEpiRefine epi{d->b, xs ? d->x : d->x64b, xs ? d->x32a : d->x32b, d->r32};
-        spmv_tile_body<double, kSpmvBS, kEngNpt, NT64>(e.tile, d->rowptr, d->colind, d->vals,
-                                                       g, d->tiles, epi, lv, lc, ca, cb);
-    } else if (ph == PH_RUN32) {
+        EEpiRefine epi{d->b, xs ? d->x : d->x64b, xs ? d->x32a : d->x32b, d->r32,
+                       st->nref == 0};
+        spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind, d->vals, g,
+                                                      epi, lv, lc, ca, cb);
+    } else {
         const int ps = st->psel;
         EGatherPNew<float> g{d->r32, ps ? d->p32b : d->p32a, (float)st->beta};
         EEpiStep<float> epi{ps ? d->p32a : d->p32b, d->q32};
-        spmv_tile_body<float, kSpmvBS, kEngNpt, NT32>(e.tile, d->rowptr, d->colind, d->vals32,
-                                                      g, d->tiles, epi,
-                                                      reinterpret_cast<float*>(lv), lc, ca, cb);
-    } else {
-        return;
+        spmv_tile_run<float, kSpmvBS, kEngNpt, NT32>(e0, e1, d->rowptr, d->colind, d->vals32, g,
+                                                     epi, reinterpret_cast<float*>(lv), lc, ca,
+                                                     cb);
     }
     block_sum2<kSpmvBS>(ca, cb, red);
     if (threadIdx.x == 0) {
@@ -202,7 +240,8 @@ __global__ __launch_bounds__(kSpmvBS) void eng_ka(const BTile* __restrict__ bts,
     }
 }
 
-__global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ act, int nact,
+__global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ act,
+                                                  const int32_t* __restrict__ toff,
                                                   const SysDesc* __restrict__ sd,
                                                   SysState* __restrict__ ss,
                                                   const double* __restrict__ pa,
@@ -212,7 +251,7 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
     const int s = act[blockIdx.x];
     SysState* st = ss + s;
     const int ph = st->phase;
-    const int b0 = sd[s].bt_begin, nb = sd[s].bt_end - sd[s].bt_begin;
+    const int b0 = toff[blockIdx.x], nb = toff[blockIdx.x + 1] - b0;
     if (ph == PH_INIT) {
         const double rr = sum_partials_block<kSpmvBS>(pa + b0, nb, red);
         const double bb = sum_partials_block<kSpmvBS>(pb + b0, nb, red);
@@ -237,8 +276,15 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
         }
     } else if (ph == PH_REFINE) {
         const double rr64 = sum_partials_block<kSpmvBS>(pa + b0, nb, red);
-        const double rr32 = sum_partials_block<kSpmvBS>(pb + b0, nb, red);
+        double rr32 = sum_partials_block<kSpmvBS>(pb + b0, nb, red);
         if (threadIdx.x == 0) {
+            if (st->nref == 0) {
+                // first refinement: the second partial was b.b
+                double bn = sqrt(rr32);
+                if (bn == 0.0) bn = 1.0;  // ||b|| = 0 -> 1, src/cg_solver.hpp:153
+                st->bnorm = bn;
+                rr32 = rr64;
+            }
             const double rel = sqrt(rr64) / st->bnorm;
             st->rel = rel;
             st->xsel ^= 1;
@@ -246,9 +292,9 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
             st->post = 1;
             if (rel < st->tol) {
                 st->converged = 1;
-                st->phase = PH_DONE;
+                st->phase = done_phase(sd + s, st);
             } else if (st->final_ || !(rel == rel)) {
-                st->phase = PH_DONE;
+                st->phase = done_phase(sd + s, st);
             } else {
                 st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
                 st->rr = rr32;
@@ -261,7 +307,7 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
         if (threadIdx.x == 0) {
             if (!(fabs(pq) >= 1e-300)) {  // src/cg_solver.hpp:84-85
                 st->breakdown = 1;
-                st->phase = PH_DONE;
+                st->phase = done_phase(sd + s, st);
             } else {
                 st->alpha = st->rr / pq;
             }
@@ -269,7 +315,7 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void eng_kc(const Chunk* __restrict__ chs, int nch,
+__global__ __launch_bounds__(kRedBlock) void eng_kc(const Chunk* __restrict__ chs,
                                                     const SysDesc* __restrict__ sd,
                                                     const SysState* __restrict__ ss,
                                                     double* __restrict__ pc)
@@ -310,7 +356,8 @@ __global__ __launch_bounds__(kRedBlock) void eng_kc(const Chunk* __restrict__ ch
     if (threadIdx.x == 0) pc[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ act, int nact,
+__global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ act,
+                                                  const int32_t* __restrict__ coff,
                                                   const SysDesc* __restrict__ sd,
                                                   SysState* __restrict__ ss,
                                                   const double* __restrict__ pc)
@@ -324,7 +371,7 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ ac
         if (threadIdx.x == 0) st->post = 0;
         return;
     }
-    const int c0 = sd[s].ch_begin, nc = sd[s].ch_end - sd[s].ch_begin;
+    const int c0 = coff[blockIdx.x], nc = coff[blockIdx.x + 1] - c0;
     const double rrn = sum_partials_block<kSpmvBS>(pc + c0, nc, red);
     if (threadIdx.x != 0) return;
     const int it = st->it + 1;
@@ -356,14 +403,103 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ ac
     }
 }
 
-__global__ void eng_pack(const int32_t* __restrict__ act, int nact,
-                         const SysState* __restrict__ ss, int32_t* __restrict__ out)
+// x = x64 + x32 over the chunks of systems in PH_FIN.
+__global__ __launch_bounds__(kRedBlock) void eng_fold(const Chunk* __restrict__ chs,
+                                                      const SysDesc* __restrict__ sd,
+                                                      const SysState* __restrict__ ss)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nact) out[i] = ss[act[i]].phase;
+    const Chunk e = chs[blockIdx.x];
+    const SysState* st = ss + e.sys;
+    if (st->phase != PH_FIN) return;
+    const SysDesc* d = sd + e.sys;
+    const double* __restrict__ x64 = st->xsel ? d->x64b : d->x;
+    const float* __restrict__ x32 = st->xsel ? d->x32b : d->x32a;
+    double* x = d->x;  // may alias x64 (elementwise, same index)
+    for (int i = e.row0 + threadIdx.x; i < e.row1; i += kRedBlock) x[i] = x64[i] + (double)x32[i];
 }
 
-__global__ void eng_fill(int64_t n, float* __restrict__ a, float* __restrict__ b)
+// One record per live system; PH_FIN becomes PH_DONE (eng_fold ran before).
+__global__ void eng_pack(const int32_t* __restrict__ act, int nact, SysState* __restrict__ ss,
+                         EngRec* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nact) return;
+    SysState* st = ss + act[i];
+    int32_t ph = st->phase;
+    if (ph == PH_FIN) {
+        ph = PH_DONE;
+        st->phase = ph;
+    }
+    EngRec r;
+    r.phase = ph;
+    r.it = st->it;
+    r.converged = st->converged;
+    r.breakdown = st->breakdown;
+    r.nref = st->nref;
+    r.gen = st->gen;
+    r.rel = st->rel;
+    out[i] = r;
+}
+
+// Launch lists from the live systems' prefix sums of tiles and chunks.
+__device__ __forceinline__ int owner_of(const int32_t* __restrict__ off, int nact, int g)
+{
+    int lo = 0, hi = nact - 1;  // largest i with off[i] <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void eng_expand(const int32_t* __restrict__ act, const int32_t* __restrict__ toff,
+                           const int32_t* __restrict__ coff, int nact, int nbt, int nch,
+                           const SysDesc* __restrict__ sd, BTile* __restrict__ bt,
+                           Chunk* __restrict__ ch)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nbt) {
+        const int i = owner_of(toff, nact, g);
+        const int s = act[i];
+        const int t = g - toff[i];
+        const SysDesc* d = sd + s;
+        BTile e;
+        e.sys = s;
+        e.tile = t;
+        e.pad = 0;
+        if (d->tiles) {
+            const TileEntry a = d->tiles[t], z = d->tiles[t + 1];
+            e.r0 = a.row;
+            e.wide = a.wide;
+            e.nz0 = a.nz0;
+            e.r1 = z.row;
+            e.nz1 = z.nz0;
+        } else {
+            e.r0 = t * d->rpt;
+            e.r1 = min(d->n, e.r0 + d->rpt);
+            e.wide = kWalkShort;
+            e.nz0 = d->rowptr[e.r0];
+            e.nz1 = d->rowptr[e.r1];
+        }
+        bt[g] = e;
+    }
+    if (g < nch) {
+        const int i = owner_of(coff, nact, g);
+        const int s = act[i];
+        const int r0 = (g - coff[i]) * kChunkRows;
+        ch[g] = {s, r0, min(sd[s].n, r0 + kChunkRows), 0};
+    }
+}
+
+__global__ void eng_install(SysDesc* __restrict__ sd, SysState* __restrict__ ss, int slot,
+                            SysDesc d, SysState h)
+{
+    sd[slot] = d;
+    ss[slot] = h;
+}
+
+__global__ void eng_zero32(int64_t n, float* __restrict__ a, float* __restrict__ b)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -372,119 +508,378 @@ __global__ void eng_fill(int64_t n, float* __restrict__ a, float* __restrict__ b
     }
 }
 
-__global__ void eng_mix_finish(int64_t n, const double* __restrict__ x64,
-                               const float* __restrict__ x32, double* __restrict__ out)
+// ------------------------------------------------- assembled stencil systems
+// Counter-based uniform draw in [-1, 1): SplitMix64 (Steele, Lea & Flood,
+// "Fast splittable pseudorandom number generators", OOPSLA 2014) of
+// seed * 0xD1B54A32D192ED03 + (i + 1) * 0x9E3779B97F4A7C15; the top 53 bits
+// make u in [0, 1), b = 2u - 1 (exact).  generators.hash_uniform is the same
+// function in numpy.
+__device__ __forceinline__ double hash_uniform(uint64_t seed, int64_t i)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        out[i] = x64[i] + (double)x32[i];
+    uint64_t z = seed * 0xD1B54A32D192ED03ull + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+}
+
+// rowptr[i] of the Dirichlet stencil on an nx x ny x nz grid (x fastest):
+// i diagonals plus, per axis of stride s and extent L (block B = s*L), the
+// rows before i that have a lower neighbour, (i/B)(B-s) + max(0, i%B - s),
+// and those that have an upper one, (i/B)(B-s) + min(i%B, B-s).
+__device__ __forceinline__ int64_t stencil_rowptr_at(int ndim, int nx, int ny, int nz, int64_t i)
+{
+    int64_t p = i;
+    const int64_t ext[3] = {nx, ny, nz};
+    int64_t s = 1;
+    for (int a = 0; a < ndim; ++a) {
+        const int64_t B = s * ext[a];
+        const int64_t q = i / B, m = i % B;
+        p += 2 * q * (B - s) + (m > s ? m - s : 0) + (m < B - s ? m : B - s);
+        s = B;
+    }
+    return p;
+}
+
+// One thread per row: rowptr, columns (-z, -y, -x, self, +x, +y, +z), values
+// (-1 off the diagonal, 2*ndim + shift on it), the FP32 copy, b, x = 0 and,
+// for the mixed form, the zeroed x32a / p32a.
+__global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double shift,
+                                       uint64_t seed, int32_t* __restrict__ rowptr,
+                                       int32_t* __restrict__ colind, double* __restrict__ vals,
+                                       float* __restrict__ vals32, double* __restrict__ b,
+                                       double* __restrict__ x, float* __restrict__ z32a,
+                                       float* __restrict__ z32b)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = (int)(i % nx);
+    const int iy = (int)((i / nx) % ny);
+    const int iz = (int)(i / ((int64_t)nx * ny));
+    const int sy = nx, sz = nx * ny;
+    int p = (int)stencil_rowptr_at(ndim, nx, ny, nz, i);
+    rowptr[i] = p;
+    if (i == n - 1) rowptr[n] = (int)stencil_rowptr_at(ndim, nx, ny, nz, n);
+    const int row = (int)i;
+    auto put = [&](int col, double val) {
+        colind[p] = col;
+        vals[p] = val;
+        if (vals32) vals32[p] = (float)val;
+        ++p;
+    };
+    if (ndim >= 3 && iz > 0) put(row - sz, -1.0);
+    if (ndim >= 2 && iy > 0) put(row - sy, -1.0);
+    if (ix > 0) put(row - 1, -1.0);
+    put(row, 2.0 * ndim + shift);
+    if (ix < nx - 1) put(row + 1, -1.0);
+    if (ndim >= 2 && iy < ny - 1) put(row + sy, -1.0);
+    if (ndim >= 3 && iz < nz - 1) put(row + sz, -1.0);
+    b[i] = hash_uniform(seed, i);
+    x[i] = 0.0;
+    if (z32a) {
+        z32a[i] = 0.0f;
+        z32b[i] = 0.0f;
+    }
 }
 
 static inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int64_t eng_work_bytes(int64_t n, int32_t mode)
+{
+    if (mode == 0) return (int64_t)(4 * al256((size_t)n * 8));
+    return (int64_t)(al256((size_t)n * 8) + 6 * al256((size_t)n * 4));
+}
+
+// Carve a system's workspace.
+static void carve_work(SysDesc& d, char* w, size_t n, int32_t mode)
+{
+    if (mode == 0) {
+        d.r = (double*)w;
+        d.p0 = (double*)(w + al256(n * 8));
+        d.p1 = (double*)(w + 2 * al256(n * 8));
+        d.Ap = (double*)(w + 3 * al256(n * 8));
+    } else {
+        size_t o = 0;
+        d.x64b = (double*)(w + o); o += al256(n * 8);
+        d.x32a = (float*)(w + o); o += al256(n * 4);
+        d.x32b = (float*)(w + o); o += al256(n * 4);
+        d.r32 = (float*)(w + o);  o += al256(n * 4);
+        d.q32 = (float*)(w + o);  o += al256(n * 4);
+        d.p32a = (float*)(w + o); o += al256(n * 4);
+        d.p32b = (float*)(w + o);
+    }
+}
+
+struct StencilShape {
+    int32_t ndim, nx, ny, nz;
+    int64_t n, nnz;
+};
+
+static int stencil_shape(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, StencilShape& g)
+{
+    if (ndim < 1 || ndim > 3) return fail(NUM_EINVAL, "ndim must be 1, 2 or 3");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(NUM_EINVAL, "grid sizes must be >= 1");
+    if ((ndim < 3 && nz != 1) || (ndim < 2 && ny != 1))
+        return fail(NUM_EINVAL, "unused grid dimensions must be 1");
+    g.ndim = ndim;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.n = (int64_t)nx * ny * nz;
+    if (g.n * (2 * ndim + 1) >= (int64_t)1 << 31) return fail(NUM_EINVAL, "grid too large");
+    const int64_t ext[3] = {nx, ny, nz};
+    g.nnz = g.n;
+    for (int a = 0; a < ndim; ++a) g.nnz += 2 * (ext[a] - 1) * (g.n / ext[a]);
+    return NUM_OK;
+}
+
+// Device bytes of an assembled stencil system: x, b, rowptr, colind, values
+// (+ FP32 values), workspace -- x first, so the solution is the first n
+// doubles of the buffer.
+static int64_t stencil_bytes(const StencilShape& g, int32_t mode)
+{
+    const size_t n = (size_t)g.n, nnz = (size_t)g.nnz;
+    size_t b = 2 * al256(n * 8) + al256((n + 1) * 4) + al256(nnz * 4) + al256(nnz * 8);
+    if (mode == 1) b += al256(nnz * 4);
+    return (int64_t)b + eng_work_bytes((int64_t)n, mode);
+}
 
 }  // namespace numamd
 
 using namespace numamd;
 
 struct num_engine {
+    struct Flight {
+        int buf;
+        std::vector<int32_t> act;
+    };
     int device = 0;
     int32_t cap = 0;
     int32_t nt64 = 1, nt32 = 1;
     SysDesc* d_desc = nullptr;
     SysState* d_state = nullptr;
-    std::vector<SysDesc> h_desc;
-    std::vector<int32_t> used;    // 1 if the slot holds a system
-    std::vector<int32_t> live;    // 1 if the system is in the launch lists
-    std::vector<int32_t> order;   // admission order of live systems
+    // per slot, host side
+    std::vector<int32_t> n, ntiles, nchunks;
+    std::vector<int32_t> used;  // holds a system
+    std::vector<int32_t> live;  // in the launch lists (not yet known finished)
+    std::vector<int32_t> done;  // finished and retired: res is valid
+    std::vector<int32_t> gen;   // admission counter, tags device records
+    std::vector<EngRec> res;
+    std::vector<int32_t> order;  // admission order of live systems
     bool dirty = true;
+    // launch lists
     BTile* d_bt = nullptr;
     Chunk* d_ch = nullptr;
-    int32_t* d_act = nullptr;
+    int32_t* d_lists = nullptr;  // [act cap | toff cap+1 | coff cap+1]
     double *d_pa = nullptr, *d_pb = nullptr, *d_pc = nullptr;
-    int32_t* d_flags = nullptr;
-    int32_t* h_flags = nullptr;  // pinned
-    int64_t bt_cap = 0, ch_cap = 0, act_cap = 0;
+    EngRec* d_rec = nullptr;
+    int64_t bt_cap = 0, ch_cap = 0;
     int32_t nbt = 0, nch = 0, nact = 0;
     std::vector<int32_t> act_host;
+    // pinned staging for the lists (two, alternating) and step records (two)
+    int32_t* h_stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_used[2] = {false, false};
+    int stage_next = 0;
+    EngRec* h_rec[2] = {nullptr, nullptr};
+    hipEvent_t rec_ev[2] = {nullptr, nullptr};
+    std::deque<Flight> flights;
 };
 
 namespace numamd {
 
-static void eng_free_lists(num_engine* E)
+static int32_t* eng_act(num_engine* E) { return E->d_lists; }
+static int32_t* eng_toff(num_engine* E) { return E->d_lists + E->cap; }
+static int32_t* eng_coff(num_engine* E) { return E->d_lists + 2 * E->cap + 1; }
+
+static void eng_free(num_engine* E)
 {
     if (E->d_bt) (void)hipFree(E->d_bt);
     if (E->d_ch) (void)hipFree(E->d_ch);
-    if (E->d_act) (void)hipFree(E->d_act);
     if (E->d_pa) (void)hipFree(E->d_pa);
     if (E->d_pb) (void)hipFree(E->d_pb);
     if (E->d_pc) (void)hipFree(E->d_pc);
-    if (E->d_flags) (void)hipFree(E->d_flags);
-    if (E->h_flags) (void)hipHostFree(E->h_flags);
     E->d_bt = nullptr;
     E->d_ch = nullptr;
-    E->d_act = nullptr;
     E->d_pa = E->d_pb = E->d_pc = nullptr;
-    E->d_flags = nullptr;
-    E->h_flags = nullptr;
-    E->bt_cap = E->ch_cap = E->act_cap = 0;
+    E->bt_cap = E->ch_cap = 0;
 }
 
-// Rebuild the launch lists from the live systems (admission order) and
-// upload them with the descriptors.
+// Retire the oldest in-flight step: fold its records into the host view.
+// Returns 1 if a step was retired, 0 if it is still running (non-blocking).
+static int eng_retire_one(num_engine* E, bool block, int* rc)
+{
+    *rc = NUM_OK;
+    if (E->flights.empty()) return 0;
+    num_engine::Flight& f = E->flights.front();
+    if (!block) {
+        const hipError_t q = hipEventQuery(E->rec_ev[f.buf]);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();  // not an error: clear it
+            return 0;
+        }
+        if (q != hipSuccess) {
+            *rc = fail(NUM_EHIP, "engine step failed: %s", hipGetErrorString(q));
+            return 0;
+        }
+    } else {
+        const hipError_t q = hipEventSynchronize(E->rec_ev[f.buf]);
+        if (q != hipSuccess) {
+            *rc = fail(NUM_EHIP, "engine step failed: %s", hipGetErrorString(q));
+            return 0;
+        }
+    }
+    const EngRec* rec = E->h_rec[f.buf];
+    bool changed = false;
+    for (size_t i = 0; i < f.act.size(); ++i) {
+        const int32_t s = f.act[i];
+        if (!E->used[s] || rec[i].gen != E->gen[s] || E->done[s]) continue;
+        if (rec[i].phase == PH_DONE) {
+            E->done[s] = 1;
+            E->res[s] = rec[i];
+            if (E->live[s]) {
+                E->live[s] = 0;
+                changed = true;
+            }
+        }
+    }
+    E->flights.pop_front();
+    if (changed) {
+        E->dirty = true;
+        std::vector<int32_t> keep;
+        keep.reserve(E->order.size());
+        for (int32_t s : E->order)
+            if (E->live[s]) keep.push_back(s);
+        E->order.swap(keep);
+    }
+    return 1;
+}
+
+static int eng_retire(num_engine* E, bool all)
+{
+    int rc = NUM_OK;
+    while (!E->flights.empty()) {
+        if (!eng_retire_one(E, all, &rc)) break;
+    }
+    return rc;
+}
+
+// Rebuild the launch lists of the live systems (admission order).
 static int eng_rebuild(num_engine* E, hipStream_t st)
 {
-    std::vector<BTile> bt;
-    std::vector<Chunk> ch;
-    std::vector<int32_t> act;
+    const int b = E->stage_next;
+    E->stage_next ^= 1;
+    if (E->stage_used[b]) NUM_HIP(hipEventSynchronize(E->stage_ev[b]));
+    int32_t* act = E->h_stage[b];
+    int32_t* toff = act + E->cap;
+    int32_t* coff = act + 2 * E->cap + 1;
+    int32_t na = 0;
+    toff[0] = 0;
+    coff[0] = 0;
     for (int32_t s : E->order) {
         if (!E->live[s]) continue;
-        SysDesc& d = E->h_desc[s];
-        d.bt_begin = (int32_t)bt.size();
-        for (int32_t t = 0; t < d.ntiles; ++t) bt.push_back({s, t});
-        d.bt_end = (int32_t)bt.size();
-        d.ch_begin = (int32_t)ch.size();
-        for (int32_t r0 = 0; r0 < d.n; r0 += kChunkRows)
-            ch.push_back({s, r0, std::min(d.n, r0 + kChunkRows), 0});
-        d.ch_end = (int32_t)ch.size();
-        act.push_back(s);
+        act[na] = s;
+        toff[na + 1] = toff[na] + E->ntiles[s];
+        coff[na + 1] = coff[na] + E->nchunks[s];
+        ++na;
     }
-    NUM_HIP(hipStreamSynchronize(st));
-    if ((int64_t)bt.size() > E->bt_cap || (int64_t)ch.size() > E->ch_cap ||
-        (int64_t)act.size() > E->act_cap) {
-        eng_free_lists(E);
-        E->bt_cap = std::max<int64_t>(1024, (int64_t)bt.size() * 5 / 4);
-        E->ch_cap = std::max<int64_t>(1024, (int64_t)ch.size() * 5 / 4);
-        E->act_cap = std::max<int64_t>(256, (int64_t)act.size() * 5 / 4);
+    const int64_t nbt = toff[na], nch = coff[na];
+    if (nbt >= ((int64_t)1 << 31) || nch >= ((int64_t)1 << 31))
+        return fail(NUM_EINVAL, "too many tiles in the batch");
+    if (nbt > E->bt_cap || nch > E->ch_cap) {
+        // in-flight steps still read the old lists
+        NUM_HIP(hipStreamSynchronize(st));
+        eng_free(E);
+        E->bt_cap = std::max<int64_t>(4096, nbt + nbt / 2);
+        E->ch_cap = std::max<int64_t>(4096, nch + nch / 2);
         NUM_HIP(hipMalloc(&E->d_bt, E->bt_cap * sizeof(BTile)));
         NUM_HIP(hipMalloc(&E->d_ch, E->ch_cap * sizeof(Chunk)));
-        NUM_HIP(hipMalloc(&E->d_act, E->act_cap * sizeof(int32_t)));
         NUM_HIP(hipMalloc(&E->d_pa, E->bt_cap * sizeof(double)));
         NUM_HIP(hipMalloc(&E->d_pb, E->bt_cap * sizeof(double)));
         NUM_HIP(hipMalloc(&E->d_pc, E->ch_cap * sizeof(double)));
-        NUM_HIP(hipMalloc(&E->d_flags, E->act_cap * sizeof(int32_t)));
-        NUM_HIP(hipHostMalloc((void**)&E->h_flags, E->act_cap * sizeof(int32_t),
-                              hipHostMallocDefault));
     }
-    if (!bt.empty()) NUM_HIP(hipMemcpy(E->d_bt, bt.data(), bt.size() * sizeof(BTile),
-                                       hipMemcpyHostToDevice));
-    if (!ch.empty()) NUM_HIP(hipMemcpy(E->d_ch, ch.data(), ch.size() * sizeof(Chunk),
-                                       hipMemcpyHostToDevice));
-    if (!act.empty()) NUM_HIP(hipMemcpy(E->d_act, act.data(), act.size() * sizeof(int32_t),
-                                        hipMemcpyHostToDevice));
-    NUM_HIP(hipMemcpy(E->d_desc, E->h_desc.data(), (size_t)E->cap * sizeof(SysDesc),
-                      hipMemcpyHostToDevice));
-    E->nbt = (int32_t)bt.size();
-    E->nch = (int32_t)ch.size();
-    E->nact = (int32_t)act.size();
-    E->act_host = std::move(act);
+    if (na > 0) {
+        NUM_HIP(hipMemcpyAsync(E->d_lists, act, (size_t)(3 * E->cap + 2) * sizeof(int32_t),
+                               hipMemcpyHostToDevice, st));
+        NUM_HIP(hipEventRecord(E->stage_ev[b], st));
+        E->stage_used[b] = true;
+        const int64_t g = std::max(nbt, nch);
+        eng_expand<<<(unsigned)((g + 255) / 256), 256, 0, st>>>(
+            eng_act(E), eng_toff(E), eng_coff(E), na, (int)nbt, (int)nch, E->d_desc, E->d_bt,
+            E->d_ch);
+        NUM_LAUNCH_CHECK();
+    }
+    E->nbt = (int32_t)nbt;
+    E->nch = (int32_t)nch;
+    E->nact = na;
+    E->act_host.assign(act, act + na);
     E->dirty = false;
     return NUM_OK;
 }
 
-static int64_t eng_work_bytes(int64_t n, int32_t mode)
+static int eng_slot(num_engine* E, int32_t* s)
 {
-    if (mode == 0) return (int64_t)(4 * al256((size_t)n * 8));
-    return (int64_t)(al256((size_t)n * 8) + 6 * al256((size_t)n * 4));
+    for (int32_t i = 0; i < E->cap; ++i)
+        if (!E->used[i]) {
+            *s = i;
+            return NUM_OK;
+        }
+    return fail(NUM_ERUNTIME, "engine full (%d systems)", E->cap);
+}
+
+// Queue the descriptor / state upload and register the system.
+static int eng_admit(num_engine* E, int32_t s, SysDesc& d, SysState& h, int32_t nchunks,
+                     hipStream_t st)
+{
+    E->gen[s] += 1;
+    h.gen = E->gen[s];
+    eng_install<<<1, 1, 0, st>>>(E->d_desc, E->d_state, s, d, h);
+    NUM_LAUNCH_CHECK();
+    E->n[s] = d.n;
+    E->ntiles[s] = d.ntiles;
+    E->nchunks[s] = nchunks;
+    E->used[s] = 1;
+    E->live[s] = 1;
+    E->done[s] = 0;
+    E->order.push_back(s);
+    E->dirty = true;
+    return NUM_OK;
+}
+
+static int eng_check(const num_engine* E)
+{
+    if (!E) return fail(NUM_EINVAL, "engine is NULL");
+    int dev = 0;
+    NUM_HIP(hipGetDevice(&dev));
+    if (dev != E->device) return fail(NUM_EINVAL, "engine lives on device %d", E->device);
+    return NUM_OK;
+}
+
+static SysState initial_state(double tol, int32_t max_iter, int32_t mode, double delta)
+{
+    SysState h{};
+    h.tol = tol;
+    h.max_iter = max_iter;
+    h.delta = delta;
+    if (mode == 0) {
+        h.phase = PH_INIT;
+    } else {
+        h.phase = PH_REFINE;  // first slot: r = b - A x0 in FP64, and ||b||
+        h.rr = INFINITY;      // first refinement: beta = rr'/inf = 0
+        h.final_ = max_iter <= 0 ? 1 : 0;
+    }
+    return h;
+}
+
+static int check_solver_args(double tol, int32_t max_iter, int32_t mode, double delta)
+{
+    if (mode != 0 && mode != 1) return fail(NUM_EINVAL, "mode must be 0 (FP64) or 1 (mixed)");
+    if (mode == 1 && !(delta > 0.0 && delta < 1.0))
+        return fail(NUM_EINVAL, "delta must be in (0, 1)");
+    if (max_iter < 0) return fail(NUM_EINVAL, "max_iter must be >= 0");
+    if (!(tol >= 0.0)) return fail(NUM_EINVAL, "tol must be >= 0");
+    return NUM_OK;
 }
 
 }  // namespace numamd
@@ -495,22 +890,37 @@ int num_engine_create(int32_t max_systems, int32_t nt64, int32_t nt32, num_engin
 {
     if (!out) return fail(NUM_EINVAL, "out is NULL");
     *out = nullptr;
-    if (max_systems <= 0) return fail(NUM_EINVAL, "max_systems must be > 0");
+    if (max_systems <= 0 || max_systems > (1 << 24))
+        return fail(NUM_EINVAL, "max_systems must be in [1, 2^24]");
     num_engine* E = new (std::nothrow) num_engine();
     if (!E) return fail(NUM_ENOMEM, "out of host memory");
-    hipError_t e = hipGetDevice(&E->device);
     E->cap = max_systems;
     E->nt64 = nt64 != 0;
     E->nt32 = nt32 != 0;
-    E->h_desc.assign((size_t)max_systems, SysDesc{});
-    E->used.assign((size_t)max_systems, 0);
-    E->live.assign((size_t)max_systems, 0);
-    if (e == hipSuccess) e = hipMalloc(&E->d_desc, (size_t)max_systems * sizeof(SysDesc));
-    if (e == hipSuccess) e = hipMalloc(&E->d_state, (size_t)max_systems * sizeof(SysState));
+    const size_t c = (size_t)max_systems;
+    E->n.assign(c, 0);
+    E->ntiles.assign(c, 0);
+    E->nchunks.assign(c, 0);
+    E->used.assign(c, 0);
+    E->live.assign(c, 0);
+    E->done.assign(c, 0);
+    E->gen.assign(c, 0);
+    E->res.assign(c, EngRec{});
+    const size_t lists = (3 * c + 2) * sizeof(int32_t);
+    hipError_t e = hipGetDevice(&E->device);
+    if (e == hipSuccess) e = hipMalloc(&E->d_desc, c * sizeof(SysDesc));
+    if (e == hipSuccess) e = hipMalloc(&E->d_state, c * sizeof(SysState));
+    if (e == hipSuccess) e = hipMalloc(&E->d_lists, lists);
+    if (e == hipSuccess) e = hipMalloc(&E->d_rec, c * sizeof(EngRec));
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipHostMalloc((void**)&E->h_stage[i], lists, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&E->h_rec[i], c * sizeof(EngRec), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&E->stage_ev[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&E->rec_ev[i], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
-        if (E->d_desc) (void)hipFree(E->d_desc);
-        if (E->d_state) (void)hipFree(E->d_state);
-        delete E;
+        (void)num_engine_destroy(E);
         return fail(NUM_EHIP, "engine allocation failed: %s", hipGetErrorString(e));
     }
     *out = E;
@@ -520,9 +930,19 @@ int num_engine_create(int32_t max_systems, int32_t nt64, int32_t nt32, num_engin
 int num_engine_destroy(num_engine* E)
 {
     if (!E) return NUM_OK;
-    eng_free_lists(E);
+    // queued steps and installs still read the engine's arrays
+    (void)hipDeviceSynchronize();
+    eng_free(E);
     if (E->d_desc) (void)hipFree(E->d_desc);
     if (E->d_state) (void)hipFree(E->d_state);
+    if (E->d_lists) (void)hipFree(E->d_lists);
+    if (E->d_rec) (void)hipFree(E->d_rec);
+    for (int i = 0; i < 2; ++i) {
+        if (E->stage_ev[i]) (void)hipEventDestroy(E->stage_ev[i]);
+        if (E->rec_ev[i]) (void)hipEventDestroy(E->rec_ev[i]);
+        if (E->h_stage[i]) (void)hipHostFree(E->h_stage[i]);
+        if (E->h_rec[i]) (void)hipHostFree(E->h_rec[i]);
+    }
     delete E;
     return NUM_OK;
 }
@@ -537,15 +957,15 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
                    int32_t max_iter, int32_t mode, double delta, void* d_work,
                    int64_t work_bytes, void* stream, int32_t* slot_out)
 {
-    if (!E || !A || !slot_out) return fail(NUM_EINVAL, "engine, matrix or slot is NULL");
+    int rc = eng_check(E);
+    if (rc) return rc;
+    if (!A || !slot_out) return fail(NUM_EINVAL, "matrix or slot is NULL");
     if (!d_b || !d_x) return fail(NUM_EINVAL, "b or x is NULL");
     if (A->nrows != A->ncols) return fail(NUM_EINVAL, "CG needs a square matrix");
-    if (mode != 0 && mode != 1) return fail(NUM_EINVAL, "mode must be 0 (FP64) or 1 (mixed)");
+    rc = check_solver_args(tol, max_iter, mode, delta);
+    if (rc) return rc;
     if (mode == 1 && !A->vals32)
         return fail(NUM_EINVAL, "mixed mode needs FP32 values (num_csr_attach_f32)");
-    if (mode == 1 && !(delta > 0.0 && delta < 1.0))
-        return fail(NUM_EINVAL, "delta must be in (0, 1)");
-    if (max_iter < 0) return fail(NUM_EINVAL, "max_iter must be >= 0");
     if (A->sched.npt > kEngNpt || A->sched.row_begin != 0 || A->sched.row_end != A->nrows)
         return fail(NUM_EINVAL, "matrix schedule holds %d entries per thread; the engine "
                     "handles at most %d (set_schedule with nnz_per_block <= %d)",
@@ -553,14 +973,13 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     const int64_t need = eng_work_bytes(A->nrows, mode);
     if (!d_work || work_bytes < need)
         return fail(NUM_EINVAL, "workspace too small: need %lld bytes", (long long)need);
-    int rc = check_device(A);
+    rc = check_device(A);
     if (rc) return rc;
     int32_t s = -1;
-    for (int32_t i = 0; i < E->cap; ++i)
-        if (!E->used[i]) { s = i; break; }
-    if (s < 0) return fail(NUM_ERUNTIME, "engine full (%d systems)", E->cap);
+    rc = eng_slot(E, &s);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
     const size_t n = (size_t)A->nrows;
-    char* w = (char*)d_work;
     SysDesc d{};
     d.rowptr = A->rowptr;
     d.colind = A->colind;
@@ -572,71 +991,104 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     d.n = A->nrows;
     d.ntiles = A->sched.nblocks;
     d.mode = mode;
-    SysState h{};
-    h.tol = tol;
-    h.max_iter = max_iter;
-    h.delta = delta;
-    if (mode == 0) {
-        d.r = (double*)w;
-        d.p0 = (double*)(w + al256(n * 8));
-        d.p1 = (double*)(w + 2 * al256(n * 8));
-        d.Ap = (double*)(w + 3 * al256(n * 8));
-        h.phase = PH_INIT;
-    } else {
-        size_t o = 0;
-        d.x64b = (double*)(w + o); o += al256(n * 8);
-        d.x32a = (float*)(w + o); o += al256(n * 4);
-        d.x32b = (float*)(w + o); o += al256(n * 4);
-        d.r32 = (float*)(w + o);  o += al256(n * 4);
-        d.q32 = (float*)(w + o);  o += al256(n * 4);
-        d.p32a = (float*)(w + o); o += al256(n * 4);
-        d.p32b = (float*)(w + o); o += al256(n * 4);
-        h.phase = PH_REFINE;  // first slot: r = b - A x0 in FP64
-        h.rr = INFINITY;      // first refinement: beta = rr'/inf = 0
-        h.final_ = max_iter <= 0 ? 1 : 0;
-        // ||b|| is needed by the first refinement's test.  Scratch: the sum
-        // lands in x64b and the partials in q32 (>= 256 bytes, >= one partial
-        // per 2048 rows), both overwritten before they are read as vectors.
-        double* bb = (double*)(d.x64b);
-        rc = launch_dot((int64_t)n, d_b, d_b, bb, (double*)d.q32, 0, (hipStream_t)stream);
-        if (rc) return rc;
-        double hb = 0.0;
-        NUM_HIP(hipMemcpyAsync(&hb, bb, sizeof(double), hipMemcpyDeviceToHost,
-                               (hipStream_t)stream));
-        NUM_HIP(hipStreamSynchronize((hipStream_t)stream));
-        h.bnorm = sqrt(hb) == 0.0 ? 1.0 : sqrt(hb);  // ||b|| = 0 -> 1
-        eng_fill<<<elementwise_grid((int64_t)n), kRedBlock, 0, (hipStream_t)stream>>>(
-            (int64_t)n, d.x32a, d.p32a);
+    d.rpt = 0;
+    carve_work(d, (char*)d_work, n, mode);
+    SysState h = initial_state(tol, max_iter, mode, delta);
+    if (mode == 1 && n > 0) {
+        eng_zero32<<<elementwise_grid((int64_t)n), kRedBlock, 0, st>>>((int64_t)n, d.x32a,
+                                                                        d.p32a);
         NUM_LAUNCH_CHECK();
     }
-    NUM_HIP(hipMemcpyAsync(E->d_state + s, &h, sizeof(SysState), hipMemcpyHostToDevice,
-                           (hipStream_t)stream));
-    NUM_HIP(hipStreamSynchronize((hipStream_t)stream));
-    E->h_desc[s] = d;
-    E->used[s] = 1;
-    E->live[s] = 1;
-    E->order.push_back(s);
-    E->dirty = true;
+    rc = eng_admit(E, s, d, h, (int32_t)((n + kChunkRows - 1) / kChunkRows), st);
+    if (rc) return rc;
     *slot_out = s;
     return NUM_OK;
 }
 
-int num_engine_step(num_engine* E, int32_t iters, int32_t* n_active, void* stream)
+int64_t num_engine_stencil_bytes(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, int32_t mode)
 {
-    if (!E) return fail(NUM_EINVAL, "engine is NULL");
+    StencilShape g;
+    if (mode != 0 && mode != 1) return -1;
+    if (stencil_shape(ndim, nx, ny, nz, g) != NUM_OK) return -1;
+    return stencil_bytes(g, mode);
+}
+
+int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, int32_t nz,
+                           double shift, uint64_t seed, double tol, int32_t max_iter,
+                           int32_t mode, double delta, void* d_buf, int64_t buf_bytes,
+                           void* stream, int32_t* slot_out)
+{
+    int rc = eng_check(E);
+    if (rc) return rc;
+    if (!slot_out) return fail(NUM_EINVAL, "slot is NULL");
+    rc = check_solver_args(tol, max_iter, mode, delta);
+    if (rc) return rc;
+    StencilShape g;
+    rc = stencil_shape(ndim, nx, ny, nz, g);
+    if (rc) return rc;
+    const int64_t need = stencil_bytes(g, mode);
+    if (!d_buf || buf_bytes < need)
+        return fail(NUM_EINVAL, "buffer too small: need %lld bytes", (long long)need);
+    if (((uintptr_t)d_buf & 255) != 0) return fail(NUM_EINVAL, "buffer must be 256-byte aligned");
+    int32_t s = -1;
+    rc = eng_slot(E, &s);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)g.n, nnz = (size_t)g.nnz;
+    char* w = (char*)d_buf;
+    SysDesc d{};
+    size_t o = 0;
+    d.x = (double*)(w + o); o += al256(n * 8);
+    d.b = (const double*)(w + o);
+    double* b = (double*)(w + o); o += al256(n * 8);
+    int32_t* rowptr = (int32_t*)(w + o); o += al256((n + 1) * 4);
+    int32_t* colind = (int32_t*)(w + o); o += al256(nnz * 4);
+    double* vals = (double*)(w + o); o += al256(nnz * 8);
+    float* vals32 = nullptr;
+    if (mode == 1) {
+        vals32 = (float*)(w + o);
+        o += al256(nnz * 4);
+    }
+    carve_work(d, w + o, n, mode);
+    d.rowptr = rowptr;
+    d.colind = colind;
+    d.vals = vals;
+    d.vals32 = vals32;
+    d.tiles = nullptr;
+    d.rpt = kStencilRpt;
+    d.n = (int32_t)n;
+    d.ntiles = (int32_t)((n + kStencilRpt - 1) / kStencilRpt);
+    d.mode = mode;
+    stencil_request_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+        ndim, nx, ny, nz, shift, seed, rowptr, colind, vals, vals32, b, d.x,
+        mode == 1 ? d.x32a : nullptr, mode == 1 ? d.p32a : nullptr);
+    NUM_LAUNCH_CHECK();
+    SysState h = initial_state(tol, max_iter, mode, delta);
+    rc = eng_admit(E, s, d, h, (int32_t)((n + kChunkRows - 1) / kChunkRows), st);
+    if (rc) return rc;
+    *slot_out = s;
+    return NUM_OK;
+}
+
+int num_engine_launch(num_engine* E, int32_t iters, void* stream)
+{
+    int rc = eng_check(E);
+    if (rc) return rc;
     if (iters < 0) return fail(NUM_EINVAL, "iters must be >= 0");
     hipStream_t st = (hipStream_t)stream;
-    int dev = 0;
-    NUM_HIP(hipGetDevice(&dev));
-    if (dev != E->device) return fail(NUM_EINVAL, "engine lives on device %d", E->device);
+    rc = eng_retire(E, false);
+    if (rc) return rc;
+    while (E->flights.size() >= 2) {
+        if (!eng_retire_one(E, true, &rc)) return rc ? rc : fail(NUM_ERUNTIME, "retire failed");
+    }
     if (E->dirty) {
-        int rc = eng_rebuild(E, st);
+        rc = eng_rebuild(E, st);
         if (rc) return rc;
     }
-    if (E->nact == 0) {
-        if (n_active) *n_active = 0;
-        return NUM_OK;
-    }
+    if (E->nact == 0) return NUM_OK;
+    const int32_t* act = eng_act(E);
+    const int32_t* toff = eng_toff(E);
+    const int32_t* coff = eng_coff(E);
     const int chunk = tile_chunk(E->nbt);
     const int grid_a = chunk_grid(E->nbt, chunk);
     for (int32_t k = 0; k < iters; ++k) {
@@ -655,61 +1107,61 @@ int num_engine_step(num_engine* E, int32_t iters, int32_t* n_active, void* strea
                 eng_ka<false, false><<<grid_a, kSpmvBS, 0, st>>>(
                     E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
         }
-        eng_kb<<<E->nact, kSpmvBS, 0, st>>>(E->d_act, E->nact, E->d_desc, E->d_state, E->d_pa,
-                                            E->d_pb);
-        eng_kc<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->nch, E->d_desc, E->d_state, E->d_pc);
-        eng_kd<<<E->nact, kSpmvBS, 0, st>>>(E->d_act, E->nact, E->d_desc, E->d_state, E->d_pc);
+        eng_kb<<<E->nact, kSpmvBS, 0, st>>>(act, toff, E->d_desc, E->d_state, E->d_pa, E->d_pb);
+        if (E->nch > 0)
+            eng_kc<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state, E->d_pc);
+        eng_kd<<<E->nact, kSpmvBS, 0, st>>>(act, coff, E->d_desc, E->d_state, E->d_pc);
     }
-    eng_pack<<<(E->nact + 255) / 256, 256, 0, st>>>(E->d_act, E->nact, E->d_state, E->d_flags);
+    if (E->nch > 0) eng_fold<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state);
+    eng_pack<<<(E->nact + 255) / 256, 256, 0, st>>>(act, E->nact, E->d_state, E->d_rec);
     NUM_LAUNCH_CHECK();
-    NUM_HIP(hipMemcpyAsync(E->h_flags, E->d_flags, (size_t)E->nact * sizeof(int32_t),
+    // the record buffer not held by the (at most one) step still in flight
+    const int buf = (!E->flights.empty() && E->flights.front().buf == 0) ? 1 : 0;
+    NUM_HIP(hipMemcpyAsync(E->h_rec[buf], E->d_rec, (size_t)E->nact * sizeof(EngRec),
                            hipMemcpyDeviceToHost, st));
-    NUM_HIP(hipStreamSynchronize(st));
-    int32_t still = 0;
-    for (int32_t i = 0; i < E->nact; ++i) {
-        const int32_t s = E->act_host[i];
-        if (E->h_flags[i] == PH_DONE) {
-            E->live[s] = 0;
-            E->dirty = true;
-        } else {
-            ++still;
-        }
-    }
-    if (E->dirty) {
-        std::vector<int32_t> keep;
-        for (int32_t s : E->order)
-            if (E->live[s]) keep.push_back(s);
-        E->order.swap(keep);
-    }
-    if (n_active) *n_active = still;
+    NUM_HIP(hipEventRecord(E->rec_ev[buf], st));
+    E->flights.push_back({buf, E->act_host});
     return NUM_OK;
+}
+
+int num_engine_poll(num_engine* E, int32_t wait, int32_t* n_live, int32_t* n_inflight)
+{
+    int rc = eng_check(E);
+    if (rc) return rc;
+    rc = eng_retire(E, wait != 0);
+    if (rc) return rc;
+    if (n_live) {
+        int32_t c = 0;
+        for (int32_t s : E->order) c += E->live[s];
+        *n_live = c;
+    }
+    if (n_inflight) *n_inflight = (int32_t)E->flights.size();
+    return NUM_OK;
+}
+
+int num_engine_step(num_engine* E, int32_t iters, int32_t* n_active, void* stream)
+{
+    int rc = num_engine_launch(E, iters, stream);
+    if (rc) return rc;
+    return num_engine_poll(E, 1, n_active, nullptr);
 }
 
 int num_engine_result(num_engine* E, int32_t slot, num_cg_result* res, int32_t* refinements,
                       void* stream)
 {
+    (void)stream;
     if (!E || !res) return fail(NUM_EINVAL, "engine or result is NULL");
     if (slot < 0 || slot >= E->cap || !E->used[slot])
         return fail(NUM_EINVAL, "slot %d holds no system", slot);
-    hipStream_t st = (hipStream_t)stream;
-    SysState h;
-    NUM_HIP(hipMemcpyAsync(&h, E->d_state + slot, sizeof(SysState), hipMemcpyDeviceToHost, st));
-    NUM_HIP(hipStreamSynchronize(st));
-    if (h.phase != PH_DONE) return fail(NUM_ERUNTIME, "system %d is still running", slot);
-    const SysDesc& d = E->h_desc[slot];
-    if (d.mode == 1 && (h.xsel || h.breakdown)) {
-        // the iterate lives in x64[xsel] + x32[xsel]; bring it to the caller's x
-        const double* x64 = h.xsel ? d.x64b : d.x;
-        const float* x32 = h.xsel ? d.x32b : d.x32a;
-        eng_mix_finish<<<elementwise_grid(d.n), kRedBlock, 0, st>>>(d.n, x64, x32, d.x);
-        NUM_LAUNCH_CHECK();
-        NUM_HIP(hipStreamSynchronize(st));
-    }
-    res->iters = h.it;
-    res->converged = h.converged;
-    res->breakdown = h.breakdown;
-    res->residual = h.rel;
-    if (refinements) *refinements = h.nref;
+    if (!E->done[slot])
+        return fail(NUM_ERUNTIME, "system %d has not finished (or its step is not retired)",
+                    slot);
+    const EngRec& r = E->res[slot];
+    res->iters = r.it;
+    res->converged = r.converged;
+    res->breakdown = r.breakdown;
+    res->residual = r.rel;
+    if (refinements) *refinements = r.nref;
     return NUM_OK;
 }
 
@@ -718,7 +1170,7 @@ int num_engine_finished(const num_engine* E, int32_t* slots, int32_t max_slots, 
     if (!E || !count) return fail(NUM_EINVAL, "engine or count is NULL");
     int32_t c = 0;
     for (int32_t s = 0; s < E->cap; ++s) {
-        if (E->used[s] && !E->live[s]) {
+        if (E->used[s] && E->done[s]) {
             if (slots && c < max_slots) slots[c] = s;
             ++c;
         }
@@ -741,7 +1193,7 @@ int num_engine_remove(num_engine* E, int32_t slot)
         E->dirty = true;
     }
     E->used[slot] = 0;
-    E->h_desc[slot] = SysDesc{};
+    E->done[slot] = 0;
     return NUM_OK;
 }
 

@@ -54,18 +54,10 @@ int32_t default_nnz_per_block(const num_csr* A)
     return kNptMax * kSpmvBS;
 }
 
-// Lanes per row in the walk phase: 1 (sequential, bit-identical to the row
-// loop) while the longest row has <= 32 entries, else enough lanes for ~16
-// entries each, capped by the wave width and by BS / rows.
-static int32_t walk_lanes(int32_t maxlen, int32_t nrows_blk)
-{
-    if (maxlen <= 32) return 1;
-    int32_t want = 1;
-    while (want * 16 < maxlen && want < kWave) want *= 2;
-    int32_t cap = 1;
-    while (cap * 2 * std::max(nrows_blk, 1) <= kSpmvBS && cap * 2 <= kWave) cap *= 2;
-    return std::min(want, cap);
-}
+// Walk mode of a staged tile: kWalkShort when every row has <= kShortRow
+// entries (one thread per row, summed in index order: bit-identical to the
+// row loop), kWalkMedium when some row is longer (those rows get a wave each).
+static int32_t walk_mode(int32_t maxlen) { return maxlen > kShortRow ? kWalkMedium : kWalkShort; }
 
 void free_schedule(Schedule& s)
 {
@@ -104,22 +96,22 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
     while (r < r1) {
         const int32_t start = r;
         const int32_t len = rp[r + 1] - rp[r];
-        if (len > nnz_per_block) {
+        if (len > nnz_per_block || len > kLongRow) {
             ++r;
             ++longs;
-            tiles.push_back({start, walk_lanes(len, 1), rp[start], 0});
+            tiles.push_back({start, kWalkLong, rp[start], 0});
             continue;
         }
         int64_t nz = 0;
         int32_t maxlen = 0;
         while (r < r1 && r - start < rows_per_block) {
             const int32_t l = rp[r + 1] - rp[r];
-            if (nz + l > nnz_per_block) break;
+            if (nz + l > nnz_per_block || l > kLongRow) break;
             nz += l;
             maxlen = std::max(maxlen, l);
             ++r;
         }
-        tiles.push_back({start, walk_lanes(maxlen, r - start), rp[start], 0});
+        tiles.push_back({start, walk_mode(maxlen), rp[start], 0});
     }
     tiles.push_back({r1, 0, rp[r1], 0});
     Schedule ns;

@@ -10,18 +10,19 @@
 //      LDS.  All loads are issued before any is used (clamped indices, no
 //      per-element branches), so each thread keeps 2*NPT loads in flight.
 //      The row bounds rowptr[r], rowptr[r+1] are fetched in the same batch.
-//   2. walk: G lanes per row (G = 1 unless the tile holds a row with more
-//      than 32 entries) walk the row's entries in LDS and gather x[col].  For
-//      stencil rows consecutive lanes own consecutive rows, so the gathers for
-//      one neighbour offset are contiguous across the wave.  With G = 1 the
-//      row is summed in index order from 0.0 with separately rounded products,
-//      the arithmetic of the reference's row loop (src/spmv_kernel.hpp:90-97):
-//      bit-identical results.
+//   2. walk: a row of up to kShortRow (32) entries is walked by one thread
+//      (consecutive threads own consecutive rows, so for stencil rows the
+//      gathers of one neighbour offset are contiguous across the wave) and
+//      summed in index order from 0.0 with separately rounded products, the
+//      arithmetic of the reference's row loop (src/spmv_kernel.hpp:90-97):
+//      bit-identical results.  A longer row of the tile gets a whole wave
+//      (64 lanes over its entries, fixed butterfly), so a power-law tile with
+//      a few medium rows among hundreds of short ones stays balanced.
 //   3. epilogue: a functor consumes (row, value) -- store y, or the fused CG
 //      updates -- and can contribute to a deterministic per-tile partial sum.
-// A row longer than the LDS budget gets a workgroup of its own ("long" tile)
-// that accumulates straight from global memory at stride BS and reduces with
-// a fixed butterfly.
+// A row longer than kLongRow entries (or than the LDS budget) gets a
+// workgroup of its own ("long" tile) that accumulates straight from global
+// memory at stride BS and reduces with a fixed butterfly.
 // Tiles are dealt to the 8 XCDs in chunks of consecutive tiles (common.hpp:
 // chunk_tile), so each XCD's L2 sees neighbouring x windows.
 #pragma once
@@ -63,40 +64,41 @@ struct GatherSplit {
     }
 };
 
-// Body for one tile.  Leaves this thread's two epilogue contributions in
-// ca/cb (zero for lanes that finish no row).
+// Body for one tile given its schedule entry and the next one.  Leaves this
+// thread's epilogue contributions in ca/cb (zero for threads that finish no
+// row).
 template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
-__device__ __forceinline__ void spmv_tile_body(
-    int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-    const T* __restrict__ vals, const Gather& x, const TileEntry* __restrict__ sched,
+__device__ __forceinline__ void spmv_tile_run(
+    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
+    const int32_t* __restrict__ colind, const T* __restrict__ vals, const Gather& x,
     const Epi& epi, T* __restrict__ lv, int32_t* __restrict__ lc, double& ca, double& cb)
 {
     constexpr int NNZ_MAX = BS * NPT;
+    constexpr int NW = BS / kWave;
     const int t = threadIdx.x;
-    const TileEntry e0 = sched[tile];
-    const TileEntry e1 = sched[tile + 1];
-    const int r0 = e0.row, G = e0.lanes, j0 = e0.nz0;
+    const int r0 = e0.row, j0 = e0.nz0;
     const int r1 = e1.row, j1 = e1.nz0;
     const int nz = j1 - j0;
     ca = 0.0;
     cb = 0.0;
-    if (nz > NNZ_MAX) {
-        // long row: one row, whole workgroup; 4 independent loads in flight
-        // per thread, then a fixed-shape tree (wave butterfly, waves in order).
+    if (e0.wide == kWalkLong || nz > NNZ_MAX) {
+        // long row: one row, whole workgroup; 8 independent loads in flight
+        // per thread (clamped, no per-element branches), then a fixed-shape
+        // tree (wave butterfly, waves in order).
         T acc = T(0);
-        int j = j0 + t;
-        for (; j + 3 * BS < j1; j += 4 * BS) {
-            T v[4];
-            int c[4];
+        const int last = j1 - 1;
+        for (int j = j0 + t; j < j1; j += 8 * BS) {
+            T v[8], xv[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                v[u] = ld_mat<NT>(vals + j + u * BS);
-                c[u] = ld_mat<NT>(colind + j + u * BS);
+            for (int u = 0; u < 8; ++u) {
+                const int jj = min(j + u * BS, last);
+                v[u] = ld_mat<NT>(vals + jj);
+                xv[u] = x(ld_mat<NT>(colind + jj));
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc += v[u] * x(c[u]);
+            for (int u = 0; u < 8; ++u)
+                if (j + u * BS < j1) acc += v[u] * xv[u];
         }
-        for (; j < j1; j += BS) acc += vals[j] * x(colind[j]);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
         if ((t & (kWave - 1)) == 0) lv[t / kWave] = acc;
@@ -104,20 +106,18 @@ __device__ __forceinline__ void spmv_tile_body(
         if (t == 0) {
             T s = T(0);
 #pragma unroll
-            for (int w = 0; w < BS / kWave; ++w) s += lv[w];
+            for (int w = 0; w < NW; ++w) s += lv[w];
             epi(r0, s, ca, cb, x);
         }
         return;
     }
     // row bounds for the walk phase, fetched together with the stage loads
     const int nr = r1 - r0;
-    const int lr = t / G;
-    const int lane = t & (G - 1);
-    const bool active = lr < nr;
+    const bool active = t < nr;
     int rb = 0, re = 0;
     if (active) {
-        rb = rowptr[r0 + lr] - j0;
-        re = rowptr[r0 + lr + 1] - j0;
+        rb = rowptr[r0 + t] - j0;
+        re = rowptr[r0 + t + 1] - j0;
     }
     // 1. stage
     if (nz > 0) {
@@ -140,17 +140,18 @@ __device__ __forceinline__ void spmv_tile_body(
         }
     }
     __syncthreads();
-    // 2. walk: lane visits k = rb + lane, rb + lane + G, ...; 8 gathers in
-    //    flight before the (ordered) sums
-    T acc = T(0);
-    if (active) {
-        int k = rb + lane;
-        for (; k + 7 * G < re; k += 8 * G) {
+    const bool wide = e0.wide != 0;  // uniform over the workgroup
+    // 2a. short rows: one thread per row, entries in index order from 0, 8
+    //     gathers in flight before the (ordered) sums
+    if (active && (!wide || re - rb <= kShortRow)) {
+        T acc = T(0);
+        int k = rb;
+        for (; k + 7 < re; k += 8) {
             T vv[8], xv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                vv[u] = lv[k + u * G];
-                xv[u] = x(lc[k + u * G]);
+                vv[u] = lv[k + u];
+                xv[u] = x(lc[k + u]);
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) acc += vv[u] * xv[u];
@@ -158,7 +159,7 @@ __device__ __forceinline__ void spmv_tile_body(
         T vv[7], xv[7];
 #pragma unroll
         for (int u = 0; u < 7; ++u) {
-            const int kk = k + u * G;
+            const int kk = k + u;
             const bool ok = kk < re;
             const int ks = ok ? kk : 0;  // slot 0 is always inside the LDS array
             vv[u] = lv[ks];
@@ -167,12 +168,61 @@ __device__ __forceinline__ void spmv_tile_body(
         }
 #pragma unroll
         for (int u = 0; u < 7; ++u)
-            if (k + u * G < re) acc += vv[u] * xv[u];
+            if (k + u < re) acc += vv[u] * xv[u];
+        epi(r0 + t, acc, ca, cb, x);
     }
-    if (G > 1) {
-        for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (!wide) return;
+    // 2b. rows longer than kShortRow: the m-th such row of the tile (in row
+    //     order) goes to wave m % NW, whose 64 lanes stride over its entries;
+    //     a fixed butterfly sums the lanes.
+    __shared__ unsigned long long wmask[NW];
+    const unsigned long long bal = __ballot(active && re - rb > kShortRow);
+    if ((t & (kWave - 1)) == 0) wmask[t / kWave] = bal;
+    __syncthreads();
+    const int w = t / kWave, l = t & (kWave - 1);
+    int m = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        unsigned long long bits = wmask[i];
+        while (bits) {
+            const int bpos = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            if ((m++ % NW) != w) continue;
+            const int row = i * kWave + bpos;
+            const int mb = rowptr[r0 + row] - j0, me = rowptr[r0 + row + 1] - j0;
+            T acc = T(0);
+            int k = mb + l;
+            for (; k + 3 * kWave < me; k += 4 * kWave) {
+                T vv[4], xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    vv[u] = lv[k + u * kWave];
+                    xv[u] = x(lc[k + u * kWave]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc += vv[u] * xv[u];
+            }
+            for (; k < me; k += kWave) acc += lv[k] * x(lc[k]);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (l == 0) {
+                double ta = 0.0, tb = 0.0;
+                epi(r0 + row, acc, ta, tb, x);
+                ca += ta;
+                cb += tb;
+            }
+        }
     }
-    if (active && lane == 0) epi(r0 + lr, acc, ca, cb, x);
+}
+
+template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
+__device__ __forceinline__ void spmv_tile_body(
+    int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const T* __restrict__ vals, const Gather& x, const TileEntry* __restrict__ sched,
+    const Epi& epi, T* __restrict__ lv, int32_t* __restrict__ lc, double& ca, double& cb)
+{
+    spmv_tile_run<T, BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], rowptr, colind,
+                                               vals, x, epi, lv, lc, ca, cb);
 }
 
 // Generic kernel: per-tile partial sums (if Epi::kRed) go to

@@ -350,12 +350,37 @@ def batched_systems(count: int = 4096, nmin: int = 64, nmax: int = 65536, seed: 
     return out
 
 
+def hash_uniform(n: int, seed: int, start: int = 0) -> np.ndarray:
+    """Counter-based uniform draws in [-1, 1): element i is 2u - 1 with u the
+    top 53 bits of SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) applied to
+    seed * 0xD1B54A32D192ED03 + (i + 1) * 0x9E3779B97F4A7C15 (mod 2^64).  The
+    engine assembles a served request's right-hand side with the same
+    function on the device, so host and device draws agree bit for bit."""
+    i = np.arange(start, start + int(n), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(int(seed) % 2**64) * np.uint64(0xD1B54A32D192ED03) + \
+            (i + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * 2.0**-53
+    return 2.0 * u - 1.0
+
+
+def stencil_nnz(dims) -> int:
+    """Non-zeros of the Dirichlet stencil on ``dims``: n + 2 (L_a - 1) n / L_a
+    per axis."""
+    dims = tuple(int(d) for d in dims)
+    n = int(np.prod(dims))
+    return n + sum(2 * (L - 1) * (n // L) for L in dims)
+
+
 def served_requests(rate: float, duration: float, seed: int = 5, nmin: int = 1000,
                     nmax: int = 2_000_000, p2d: float = 0.7, max_requests: int | None = None):
     """Open-loop Poisson arrivals at ``rate`` req/s for ``duration`` s.  Each
     request is a 2-D (p=0.7) or 3-D Laplacian with shift uniform[0.001, 1] and
-    n log-uniform in [nmin, nmax]; b is drawn lazily from the request's own
-    seed.  Returns a list of dicts {t, dims, shift, seed}."""
+    n log-uniform in [nmin, nmax]; its b is ``hash_uniform(n, seed)`` of the
+    request's own seed.  Returns a list of dicts {t, dims, shift, seed}."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []

@@ -3,11 +3,15 @@
 ``CGServer`` admits solve requests into a running ``CGEngine`` batch:
 
 * requests wait in a FIFO queue from arrival until their device memory
-  (matrix, FP32 copy, b, x and workspace) fits the admission budget and a
-  batch slot is free;
-* admitted requests join the running batch at the next engine step, so their
-  iterations overlap with everything already on the device; matrices and
-  Krylov vectors stay resident until the solve finishes;
+  (matrix, FP32 copy, b, x and workspace, one buffer) fits the admission
+  budget and a batch slot is free;
+* an admitted request's system is assembled on the device (no host work
+  proportional to its size, no synchronisation) and joins the running batch
+  at the next engine step, so its iterations overlap with everything already
+  on the device; matrices and Krylov vectors stay resident until the solve
+  finishes;
+* steps are pipelined: the host admits and retires requests while the
+  device runs the previous step;
 * a finished request leaves the batch, returns its memory to the budget and
   records its latency (completion - arrival).
 
@@ -29,6 +33,7 @@ import torch
 
 from . import kokkos_spmv as ks
 from .batched import CGEngine
+from .generators import stencil_nnz
 
 
 @dataclass
@@ -50,21 +55,25 @@ class Request:
         return self.t_done - self.t_arrival
 
 
-def request_bytes(n: int, nnz: int, mixed: bool) -> int:
-    """Device bytes a request holds while admitted: CSR (FP64 values, int32
-    columns and rowptr, plus the FP32 copy in mixed mode), b, x, the engine
-    workspace and the row-tile schedule."""
-    csr = 12 * nnz + 4 * (n + 1) + (4 * nnz if mixed else 0)
-    vecs = 16 * n
-    work = CGEngine.work_bytes(n, mixed)
-    sched = 16 * (nnz // 1024 + 2)
-    return csr + vecs + work + sched
+def request_bytes(dims, mixed: bool) -> int:
+    """Device bytes a request holds while admitted: its assembled stencil
+    (rowptr, int32 columns, FP64 values, the FP32 copy in mixed mode), b, x
+    and the engine workspace -- one buffer."""
+    return CGEngine.stencil_bytes(dims, mixed)
 
 
 class CGServer:
+    """Continuous-batching CG server on one GPU (see the module docstring).
+
+    ``timeout_s``: a request still running that long after admission is
+    cancelled (its slot and memory are freed) and reported with
+    ``result["timeout"] = True``.  A request larger than the whole budget is
+    rejected at once (``result["rejected"] = True``)."""
+
     def __init__(self, budget_bytes: float = 200e9, max_systems: int = 4096,
                  mixed: bool = True, iters_per_step: int = 8, max_iter: int = 20000,
-                 delta: float = 0.1, nt64: bool = True, nt32: bool = False):
+                 delta: float = 0.1, nt64: bool = True, nt32: bool = False,
+                 timeout_s: float | None = None, admit_per_pump: int = 32):
         self.budget = float(budget_bytes)
         self.in_use = 0.0
         self.peak_in_use = 0.0
@@ -72,12 +81,17 @@ class CGServer:
         self.iters_per_step = int(iters_per_step)
         self.max_iter = int(max_iter)
         self.delta = float(delta)
+        self.timeout_s = timeout_s
+        # admissions between two launches: bounded so a burst of arrivals
+        # never leaves the device without queued work for long
+        self.admit_per_pump = int(admit_per_pump)
         self.engine = CGEngine(max_systems=max_systems, nt64=nt64, nt32=nt32)
         self.max_systems = int(max_systems)
         self.queue: collections.deque = collections.deque()
-        self.running: dict = {}    # slot -> (request, bytes, keep-alive tensors)
+        self.running: dict = {}    # slot -> (request, bytes)
         self.done: list = []
-        self.clock0 = None
+        self.keep_solutions = False
+        self.clock0 = time.perf_counter()
 
     def now(self) -> float:
         return time.perf_counter() - self.clock0
@@ -85,51 +99,66 @@ class CGServer:
     def submit(self, req: Request) -> None:
         self.queue.append(req)
 
+    def _finish(self, req: Request, result: dict) -> None:
+        req.result = result
+        req.t_done = self.now()
+        self.done.append(req)
+
     def _admit(self) -> int:
         admitted = 0
-        while self.queue and len(self.running) < self.max_systems:
+        while (self.queue and len(self.running) < self.max_systems
+               and admitted < self.admit_per_pump):
             req = self.queue[0]
-            n = int(np.prod(req.dims))
-            nnz = n * (2 * len(req.dims) + 1)  # upper bound before assembly
-            need = request_bytes(n, nnz, self.mixed)
-            if self.in_use + need > self.budget and self.running:
-                break  # wait for memory to free up
+            need = request_bytes(req.dims, self.mixed)
+            if need > self.budget:
+                self.queue.popleft()
+                self._finish(req, {"rejected": True, "converged": False})
+                continue
+            if self.in_use + need > self.budget:
+                break  # wait for memory to free up (FIFO: no overtaking)
             self.queue.popleft()
-            A = ks.upload_stencil(req.dims, req.shift)
-            gen = torch.Generator(device="cuda")
-            gen.manual_seed(int(req.seed))
-            b = torch.rand(A.nrows, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
-            x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
-            slot = self.engine.add(A, b, x, tol=req.tol, max_iter=self.max_iter,
-                                   mixed=self.mixed, delta=self.delta)
-            req.n, req.nnz = A.nrows, A.nnz
+            slot = self.engine.add_stencil(req.dims, req.shift, req.seed, tol=req.tol,
+                                           max_iter=self.max_iter, mixed=self.mixed,
+                                           delta=self.delta, nbytes=need)
+            req.n = math.prod(req.dims)
+            req.nnz = stencil_nnz(req.dims)
             req.t_admit = self.now()
             self.in_use += need
             self.peak_in_use = max(self.peak_in_use, self.in_use)
-            self.running[slot] = (req, need, (A, b, x))
+            self.running[slot] = (req, need)
             admitted += 1
         return admitted
+
+    def _release(self, slot: int) -> Request:
+        req, need = self.running.pop(slot)
+        if self.keep_solutions:
+            req.solution = self.engine.solution(slot)
+        self.engine.remove(slot)
+        self.in_use -= need
+        return req
 
     def _collect(self) -> int:
         finished = 0
         for slot in self.engine.finished():
-            req, need, keep = self.running[slot]
             res = self.engine.result(slot)
-            req.result = res
-            req.t_done = self.now()
-            req.solution = keep[2]
-            self.engine.remove(slot)
-            del self.running[slot]
-            self.in_use -= need
-            self.done.append(req)
+            req = self._release(slot)
+            self._finish(req, res)
             finished += 1
+        if self.timeout_s is not None:
+            t = self.now()
+            for slot in [s for s, (r, _) in self.running.items()
+                         if t - r.t_admit > self.timeout_s]:
+                req = self._release(slot)
+                self._finish(req, {"timeout": True, "converged": False})
+                finished += 1
         return finished
 
     def pump(self) -> None:
-        """Admit what fits, run one engine step, retire finished solves."""
+        """Admit what fits, queue one engine step (the previous one keeps the
+        device busy meanwhile), retire finished solves."""
         self._admit()
         if self.running:
-            self.engine.step(self.iters_per_step)
+            self.engine.launch(self.iters_per_step)
             self._collect()
 
     def run_open_loop(self, requests, keep_solutions: bool = False,
@@ -137,6 +166,7 @@ class CGServer:
         """Replay ``requests`` (sorted by arrival time) open-loop; returns the
         latency / throughput summary.  Requests not finished when
         ``max_wall`` seconds have passed are reported as unfinished."""
+        self.keep_solutions = keep_solutions
         pending = collections.deque(sorted(requests, key=lambda r: r.t_arrival))
         self.clock0 = time.perf_counter()
         while pending or self.queue or self.running:
@@ -150,10 +180,8 @@ class CGServer:
                     time.sleep(max(0.0, min(pending[0].t_arrival - t, 0.01)))
                 continue
             self.pump()
+        self.engine.poll(wait=True)
         wall = self.now()
-        if not keep_solutions:
-            for r in self.done:
-                r.__dict__.pop("solution", None)
         return summarize(self.done, wall, len(requests))
 
 
@@ -164,6 +192,8 @@ def summarize(done, wall: float, submitted: int) -> dict:
     t_last = max((r.t_done for r in done), default=0.0)
     return {
         "submitted": submitted, "completed": len(done), "converged": ok,
+        "rejected": sum(1 for r in done if r.result.get("rejected")),
+        "timed_out": sum(1 for r in done if r.result.get("timeout")),
         "wall_s": wall,
         "solves_per_s": len(done) / max(t_last - t_first, 1e-9) if done else 0.0,
         "p50_ms": float(np.percentile(lat, 50) * 1e3) if done else None,
@@ -179,23 +209,30 @@ def requests_from_trace(trace, tol: float = 1e-10):
             for i, q in enumerate(trace)]
 
 
-def one_at_a_time(requests, mixed: bool = False) -> dict:
-    """Baseline: each request built and solved by itself with plain FP64
-    cg_solve_local (or cg_solve_mixed), one after another."""
+def one_at_a_time(requests, mixed: bool = False, max_iter: int = 20000) -> dict:
+    """Baseline: each request built and solved by itself, one after another,
+    with plain FP64 cg_solve_local (or cg_solve_mixed).  ``solves_per_s``
+    counts the solve calls only (matrix and b set-up excluded, which favours
+    the baseline); ``wall_solves_per_s`` includes the set-up."""
+    from .generators import hash_uniform
     ks.init(0)
     t0 = time.perf_counter()
+    t_solve = 0.0
     iters = []
     for req in requests:
         A = ks.upload_stencil(req.dims, req.shift)
-        gen = torch.Generator(device="cuda")
-        gen.manual_seed(int(req.seed))
-        b = torch.rand(A.nrows, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+        b = torch.from_numpy(hash_uniform(A.nrows, req.seed)).cuda()
         x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         if mixed:
-            r = ks.cg_solve_mixed(A, b, x, tol=req.tol)
+            r = ks.cg_solve_mixed(A, b, x, tol=req.tol, max_iter=max_iter)
         else:
-            r = ks.cg_solve_local(A, b, x, tol=req.tol, max_iter=20000)
+            r = ks.cg_solve_local(A, b, x, tol=req.tol, max_iter=max_iter)
+        t_solve += time.perf_counter() - t1
         iters.append(r["iters"])
     el = time.perf_counter() - t0
-    return {"requests": len(requests), "seconds": el, "solves_per_s": len(requests) / el,
+    return {"requests": len(requests), "seconds": el, "solve_seconds": t_solve,
+            "solves_per_s": len(requests) / t_solve if t_solve > 0 else None,
+            "wall_solves_per_s": len(requests) / el,
             "mean_iters": float(np.mean(iters)) if iters else None}

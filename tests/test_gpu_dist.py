@@ -1,0 +1,122 @@
+"""GPU parity of one rank's share of the 1-D row partition: dist_spmv (interior
+rows on x_local, boundary rows on [x_local | x_ghost]) against the oracle's
+SpMV of the whole matrix, and halo_pack against the ghost blocks the other
+ranks expect.
+
+The local matrix keeps every row's entries in the global order (only the
+column ids are remapped, dist_spmv.hpp:62-82), and the row kernel sums a row
+in entry order, so a share's rows must equal the global rows bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from num_amd import generators as g
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+CASES = [
+    ("lap3d_12", lambda: g.laplacian_3d(12), [(0, 1), (0, 3), (1, 3), (2, 3), (3, 8), (7, 8)]),
+    ("lap2d_40", lambda: g.laplacian_2d(40), [(0, 2), (1, 2), (2, 5)]),
+    ("lcg_5000", lambda: g.lcg_matrix(5000, 7, seed=11)[0], [(0, 4), (2, 4), (3, 4)]),
+    ("zipf_6000", lambda: g.zipf_matrix(6000, 2.0, 600, seed=2), [(1, 3), (0, 6)]),
+]
+
+
+@pytest.mark.parametrize("name,make,shares", CASES, ids=[c[0] for c in CASES])
+def test_share_rows_equal_global_rows(ks, orc, name, make, shares):
+    A = make()
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, A.ncols)
+    y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+    for rank, size in shares:
+        D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols,
+                              rank=rank, size=size)
+        s, e = D.local_row_start, D.local_row_end
+        xl = _dev(x[s:e])
+        xg = D.ghost_values(_dev(x)) if D.n_ghost else None
+        yl = torch.full((e - s,), float("nan"), dtype=torch.float64, device="cuda")
+        ks.dist_spmv(D, xl, yl, xg)
+        got = yl.cpu().numpy()
+        if int(np.diff(A.rowptr).max()) <= 32:
+            # one lane per row: the row is summed in entry order
+            assert np.array_equal(got, y_ref[s:e]), (name, rank, size)
+        else:
+            # rows split over lanes: the reference's 1e-12 relative bar
+            # (tests/test_spmv_kernel.cpp:205)
+            ref = y_ref[s:e]
+            assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), (name, rank, size)
+
+
+def test_slab_share_lap3d_48_rank3_of_8(ks, orc):
+    """The slab shape of the 192^3 share at a size the oracle finishes
+    quickly: 48^3, 8 slabs of 6 planes, rank 3 (two ghost planes)."""
+    L = 48
+    A = g.laplacian_3d(L)
+    D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=3, size=8)
+    plane = L * L
+    assert D.local_row_start == 3 * 6 * plane and D.local_row_end == 4 * 6 * plane
+    assert D.n_ghost == 2 * plane
+    assert len(D.boundary_rows) == 2 * plane
+    assert len(D.interior_rows) == 4 * plane
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, A.ncols)
+    y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+    s, e = D.local_row_start, D.local_row_end
+    yl = torch.empty(e - s, dtype=torch.float64, device="cuda")
+    ks.dist_spmv(D, _dev(x[s:e]), yl, D.ghost_values(x))  # host x path of ghost_values
+    assert np.array_equal(yl.cpu().numpy(), y_ref[s:e])
+
+
+def test_halo_pack_matches_neighbour_ghosts(ks):
+    """What rank r packs for rank q is exactly the slice of q's ghost block
+    owned by r (halo_exchange.hpp:154-191 send lists, 222-235 pack)."""
+    A = g.laplacian_3d(10)
+    size = 4
+    D = [ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=r, size=size)
+         for r in range(size)]
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, A.ncols)
+    for r in range(size):
+        s, e = D[r].local_row_start, D[r].local_row_end
+        buf = ks.halo_pack(D[r], _dev(x[s:e])).cpu().numpy()
+        off = 0
+        for q, ids in D[r].send:
+            got = buf[off:off + len(ids)]
+            off += len(ids)
+            # q's ghost ids owned by r, in q's slot order
+            gq = D[q].ghost_ids
+            mine = gq[(gq >= s) & (gq < e)]
+            assert np.array_equal(ids, mine)
+            assert np.array_equal(got, x[mine])
+        assert off == len(buf)
+
+
+def test_dist_spmv_needs_ghosts(ks):
+    A = g.laplacian_2d(8)
+    D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=1, size=2)
+    n = D.local_row_end - D.local_row_start
+    xl = torch.zeros(n, dtype=torch.float64, device="cuda")
+    yl = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError):
+        ks.dist_spmv(D, xl, yl)
+    with pytest.raises(ValueError):
+        ks.dist_spmv(D, xl, yl, torch.zeros(3, dtype=torch.float64, device="cuda"))
+    with pytest.raises(RuntimeError):
+        ks.cg_solve(D, xl, yl)
+
+
+def test_single_rank_share_is_whole_matrix(ks, orc):
+    A = g.laplacian_2d(20)
+    D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=0, size=1)
+    assert D.n_ghost == 0 and D.boundary_runs == []
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+    res = ks.cg_solve(D, _dev(b), x, tol=1e-8, max_iter=2000)
+    xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, np.zeros(A.nrows), 1e-8, 2000)
+    assert res["converged"] and abs(res["iters"] - ro["iters"]) <= max(1, 0.02 * ro["iters"])

@@ -21,6 +21,37 @@ def test_upload_stencil_equals_generator(ks, dims, shift):
     assert np.array_equal(d._values.cpu().numpy(), A.values)
 
 
+@pytest.mark.parametrize("dims", [(9,), (13, 7), (6, 5, 4), (1, 1), (1,), (200, 150),
+                                  (33, 17, 9)])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_engine_stencil_assembly(ks, dims, mixed):
+    """The engine's one-kernel request assembly (closed-form rowptr, columns,
+    values, FP32 copy, b = hash_uniform, x = 0) equals the host generators."""
+    from num_amd.batched import CGEngine, stencil_layout
+    eng = CGEngine(max_systems=2)
+    shift, seed = 0.25, 1234 + len(dims)
+    s = eng.add_stencil(dims, shift, seed, mixed=mixed)
+    torch.cuda.synchronize()
+    buf = eng._sys[s][0].cpu().numpy()
+    lay = stencil_layout(dims, mixed)
+    n, nnz = lay["n"], lay["nnz"]
+    A = g.stencil(dims, shift)
+    assert (A.nrows, A.nnz) == (n, nnz)
+
+    def arr(name, dt, count):
+        o = lay[name]
+        return buf[o:o + count * np.dtype(dt).itemsize].view(dt)
+    assert np.array_equal(arr("rowptr", np.int32, n + 1), A.rowptr)
+    assert np.array_equal(arr("colind", np.int32, nnz), A.colind)
+    assert np.array_equal(arr("values", np.float64, nnz), A.values)
+    if mixed:
+        assert np.array_equal(arr("values32", np.float32, nnz), A.values.astype(np.float32))
+    assert np.array_equal(arr("b", np.float64, n), g.hash_uniform(n, seed))
+    assert np.array_equal(arr("x", np.float64, n), np.zeros(n))
+    assert torch.equal(eng.solution(s).cpu(), torch.zeros(n, dtype=torch.float64))
+    assert CGEngine.stencil_bytes(dims, mixed) >= lay["work"] + CGEngine.work_bytes(n, mixed)
+
+
 def _trace(count, seed, nmax=60_000):
     tr = g.served_requests(rate=2000.0, duration=10.0, seed=seed, nmin=1000, nmax=nmax,
                            max_requests=count)
@@ -52,22 +83,69 @@ def test_served_results_equal_alone(ks, mixed):
         assert torch.equal(a.solution, r.solution)
         # true residual in FP64
         A = g.stencil(r.dims, r.shift)
-        gen = torch.Generator(device="cuda")
-        gen.manual_seed(int(r.seed))
-        b = (torch.rand(A.nrows, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1)
-        b = b.cpu().numpy()
+        b = g.hash_uniform(A.nrows, r.seed)
         true = np.linalg.norm(b - A.to_scipy() @ r.solution.cpu().numpy()) / np.linalg.norm(b)
         assert true < (1e-10 if mixed else 2e-10)
+        if not mixed:
+            # the same system through the one-at-a-time FP64 solver: the same
+            # CG sequence, iterations within the +-2% bar
+            dA = ks.upload_stencil(r.dims, r.shift)
+            x1 = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+            one = ks.cg_solve_local(dA, torch.from_numpy(b).cuda(), x1, tol=r.tol,
+                                    max_iter=20000)
+            assert abs(one["iters"] - r.result["iters"]) <= max(1, 0.02 * one["iters"])
 
 
 def test_budget_limits_admission(ks):
     from num_amd.serve import CGServer, request_bytes
     reqs = _trace(6, seed=7, nmax=20_000)
-    biggest = max(request_bytes(int(np.prod(r.dims)), int(np.prod(r.dims)) * 7, True)
-                  for r in reqs)
+    biggest = max(request_bytes(r.dims, True) for r in reqs)
     budget = biggest * 1.01
     srv = CGServer(budget_bytes=budget, max_systems=16, mixed=True, iters_per_step=4)
     summ = srv.run_open_loop(reqs)
     assert summ["completed"] == len(reqs) and summ["converged"] == len(reqs)
     assert 0 < srv.peak_in_use <= budget
     assert srv.in_use == 0.0
+
+
+def test_rejection_and_timeout(ks):
+    from num_amd.serve import CGServer, Request, request_bytes
+    small = Request(0, 0.0, (20, 20), 0.5, 1)
+    huge = Request(1, 0.0, (400, 400), 0.5, 2)
+    srv = CGServer(budget_bytes=request_bytes(small.dims, True) * 2, max_systems=4)
+    summ = srv.run_open_loop([small, huge])
+    assert summ["completed"] == 2 and summ["rejected"] == 1 and summ["converged"] == 1
+    assert srv.in_use == 0.0
+    # timeout 0: every request still running at its first collection is
+    # cancelled, its slot and memory returned
+    reqs = _trace(8, seed=3, nmax=200_000)
+    srv = CGServer(budget_bytes=50e9, max_systems=8, timeout_s=0.0)
+    summ = srv.run_open_loop(reqs)
+    assert summ["completed"] == len(reqs)
+    assert summ["timed_out"] + summ["converged"] == len(reqs) and summ["timed_out"] > 0
+    assert srv.in_use == 0.0 and srv.engine.info()["live"] == 0
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_pipelined_launch_equals_step(ks, mixed):
+    """launch/poll (steps queued while the host works) and the synchronous
+    step give bit-identical solutions and iteration counts."""
+    from num_amd.batched import CGEngine
+    grids = [((30, 20), 0.01), ((12, 11, 10), 0.3), ((500,), 0.001), ((7, 64), 1.0)]
+    out = []
+    for pipelined in (False, True):
+        eng = CGEngine(max_systems=8)
+        slots = [eng.add_stencil(d, sh, 10 + i, tol=1e-10, max_iter=5000, mixed=mixed)
+                 for i, (d, sh) in enumerate(grids)]
+        if pipelined:
+            while True:
+                eng.launch(5)
+                live, inflight = eng.poll()
+                if live == 0:
+                    break
+            eng.poll(wait=True)
+        else:
+            eng.run(5)
+        out.append([(eng.result(s)["iters"], eng.solution(s).clone()) for s in slots])
+    for (i0, x0), (i1, x1) in zip(*out):
+        assert i0 == i1 and torch.equal(x0, x1)

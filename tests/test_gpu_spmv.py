@@ -102,6 +102,31 @@ def test_zipf_long_rows(ks, orc):
     y = _spmv(ks, _up(ks, A), x)
     y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
     assert rel(y, y_ref) < 1e-12
+    # rows of <= 32 entries are walked by one thread in index order even in
+    # tiles that also hold longer rows: bit-identical to the row loop
+    short = np.diff(A.rowptr) <= 32
+    assert np.array_equal(y[short], y_ref[short])
+
+
+@pytest.mark.parametrize("lo,hi,frac_short", [(33, 300, 0.0), (1, 1800, 0.5), (30, 40, 0.3)])
+def test_medium_rows(ks, orc, lo, hi, frac_short):
+    """Tiles with many rows longer than 32 entries (each goes to a wave, the
+    m-th to wave m % 4) mixed with short rows."""
+    rng = np.random.default_rng(lo + hi)
+    n = 20000
+    k = rng.integers(lo, hi + 1, n)
+    short_rows = rng.random(n) < frac_short
+    k[short_rows] = rng.integers(0, 6, int(short_rows.sum()))
+    rp = np.concatenate([[0], np.cumsum(k)]).astype(np.int32)
+    ci = np.sort(rng.integers(0, n, int(rp[-1])).astype(np.int32))
+    v = rng.uniform(-1, 1, int(rp[-1]))
+    dA = ks.upload_csr(rp, ci, v, n, n)
+    x = rng.uniform(-1, 1, n)
+    y = _spmv(ks, dA, x)
+    y_ref = orc.spmv(rp, ci, v, x)
+    assert rel(y, y_ref) < 1e-12
+    s = k <= 32
+    assert np.array_equal(y[s], y_ref[s])
 
 
 @pytest.mark.parametrize("sched", [(256, 2048), (64, 512), (16, 4096), (256, 1024), (1, 1)])

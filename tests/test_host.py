@@ -93,6 +93,34 @@ def test_served_requests():
 
 # ---------------------------------------------- partition and ghost layout ---
 
+def test_hash_uniform_is_splitmix64():
+    """b of a served request: SplitMix64 outputs for state 0 are the published
+    e220a8397b1dcdaf, 6e789e6aa1b965f4, 06c45d188009454f (seed 0 draws
+    i = 0, 1, 2), mapped to 2 * (z >> 11) * 2^-53 - 1."""
+    known = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    want = [2.0 * ((z >> 11) * 2.0**-53) - 1.0 for z in known]
+    assert g.hash_uniform(3, 0).tolist() == want
+    # counter-based: any window of indices is a slice of the full draw
+    full = g.hash_uniform(1000, 77)
+    assert np.array_equal(g.hash_uniform(10, 77, start=500), full[500:510])
+    assert full.min() >= -1.0 and full.max() < 1.0
+    assert not np.array_equal(g.hash_uniform(1000, 78), full)
+    big = g.hash_uniform(200_000, 5)
+    assert abs(big.mean()) < 0.01 and abs(big.var() - 1.0 / 3.0) < 0.01
+
+
+@pytest.mark.parametrize("dims", [(1,), (9,), (1, 1), (13, 7), (2, 2, 2), (6, 5, 4), (33, 1, 2)])
+def test_stencil_nnz_and_layout(dims):
+    from num_amd.batched import stencil_layout
+    A = g.stencil(dims, 0.0)
+    assert g.stencil_nnz(dims) == A.nnz
+    lay = stencil_layout(dims, True)
+    assert lay["n"] == A.nrows and lay["nnz"] == A.nnz
+    offs = [lay[k] for k in ("x", "b", "rowptr", "colind", "values", "values32", "work")]
+    assert offs == sorted(offs) and all(o % 256 == 0 for o in offs)
+    assert lay["b"] >= 8 * A.nrows and lay["work"] - lay["values32"] >= 4 * A.nnz
+
+
 def test_row_partition_coverage_and_balance():
     # tests/test_halo_exchange.py:89-141 semantics at N = 127
     for size in range(1, 9):
