@@ -8,7 +8,10 @@ nnz = 49,324,032), inputs resident in HBM.  Prints ONE JSON line:
                  launches on the stream they run on) vs the 8 TB/s HBM peak
   cpu_baseline = the C oracle's SpMV (OpenMP, host cores) on a bounded sample
   cg           = plain FP64 CG on the same matrix to 1e-10: solves/s, p50/p99
+  spmv_slab_r3of8 / spmv_zipf4m = the slab-share and Zipf SpMV
+  cg_mixed_diff128 / cg_batched_4096 / cg_served = the three solver forms
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu]
+                       [--no-extra] [--no-solvers] [--served-seconds S]
 """
 from __future__ import annotations
 
@@ -158,6 +161,120 @@ def slab_bench(ks, A, x, y_ref, steps):
             "bit_exact_vs_oracle": exact if y_ref is not None else None}
 
 
+def mixed_bench(ks, solves=3, delta=0.01):
+    """Mixed-precision refined CG vs plain FP64 CG on the 128^3 variable-
+    coefficient diffusion (lognormal sigma 1, seed 3), b U(-1,1) seed 0, to
+    1e-10 FP64 relative residual; both matrix copies resident."""
+    import torch
+    from num_amd import generators as g
+    A = g.diffusion_3d(128, sigma=1.0, seed=3)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    dA.ensure_f32()
+    db = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, A.nrows)).cuda()
+    x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+
+    def true_res():
+        ks.spmv(dA, x, y)
+        return float(torch.linalg.norm(db - y) / torch.linalg.norm(db))
+
+    def timed(fn):
+        fn()
+        ts, r = [], None
+        for _ in range(solves):
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), r, true_res()
+    t64, r64, tr64 = timed(lambda: ks.cg_solve_local(dA, db, x, tol=1e-10, max_iter=20000))
+    t32, r32, tr32 = timed(lambda: ks.cg_solve_mixed(dA, db, x, tol=1e-10, delta=delta))
+    return {"n": A.nrows, "nnz": A.nnz, "fp64_ms": round(t64 * 1e3, 2), "fp64_iters": r64["iters"],
+            "fp64_true_residual": tr64, "mixed_ms": round(t32 * 1e3, 2),
+            "mixed_fp32_steps": r32["inner_total"], "mixed_refinements": r32["outer"],
+            "mixed_true_residual": tr32, "delta": delta,
+            "speedup": round(t64 / t32, 3), "both_converged": bool(r64["converged"] and
+                                                                   r32["converged"])}
+
+
+def batched_bench(ks, count=4096, sample=256):
+    """4096 independent 2-D shifted Laplacians (n log-uniform in [64, 65536],
+    aspect U[1,4], shift U[0.01,1], b U(-1,1); seed 4) advanced together by
+    the engine, FP64 and mixed, to 1e-10; against the same systems solved
+    one at a time with plain FP64 cg_solve_local (an evenly spaced sample,
+    solve calls only)."""
+    import torch
+    from num_amd import generators as g
+    from num_amd.batched import CGEngine
+    specs = g.batched_specs(count=count, seed=4)
+    out = {"systems": count, "sum_n": int(sum(d[0] * d[1] for d, _, _ in specs))}
+    for mixed in (False, True):
+        best = None
+        for _ in range(2):  # the second run reuses warm allocations
+            eng = CGEngine(max_systems=count)
+            slots = [eng.add_stencil(d, sh, sd, tol=1e-10, max_iter=20000, mixed=mixed)
+                     for d, sh, sd in specs]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            eng.run(16)
+            el = time.perf_counter() - t0
+            res = [eng.result(s) for s in slots]
+            best = el if best is None else min(best, el)
+        key = "mixed" if mixed else "fp64"
+        out[key] = {"seconds": round(best, 4), "solves_per_s": round(count / best, 1),
+                    "all_converged": all(r["converged"] for r in res),
+                    "max_residual": max(r["residual"] for r in res),
+                    "max_iters": max(r["iters"] for r in res)}
+    idx = list(range(0, count, max(1, count // sample)))
+    t_solve = 0.0
+    for j, k in enumerate(idx[:2] + idx):  # the first two are warm-up solves
+        d, sh, sd = specs[k]
+        A = ks.upload_stencil(d, sh)
+        b = torch.from_numpy(g.hash_uniform(A.nrows, sd)).cuda()
+        x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ks.cg_solve_local(A, b, x, tol=1e-10, max_iter=20000)
+        if j >= 2:
+            t_solve += time.perf_counter() - t0
+    base = len(idx) / t_solve
+    out["one_at_a_time_fp64"] = {"sample": len(idx), "solves_per_s": round(base, 1)}
+    out["speedup_fp64"] = round(out["fp64"]["solves_per_s"] / base, 1)
+    out["speedup_mixed"] = round(out["mixed"]["solves_per_s"] / base, 1)
+    return out
+
+
+def served_bench(seconds=(3.0, 3.0, 1.0), base_requests=200):
+    """Open-loop Poisson streams (lambda 100, 1k, 10k req/s; seed 5) of 2-D /
+    3-D shifted Laplacians, n log-uniform in [1k, 2M], tol 1e-10, mixed
+    refined solver, 200 GB admission budget; p50 / p99 latency and solves/s,
+    against the same requests solved one at a time with plain FP64 CG."""
+    from num_amd import generators as g
+    from num_amd.serve import CGServer, one_at_a_time, requests_from_trace
+    CGServer().run_open_loop(requests_from_trace(
+        g.served_requests(rate=1000, duration=1.0, seed=99, nmax=20000, max_requests=8)))
+    out = {}
+    for lam, dur in zip((100.0, 1000.0, 10000.0), seconds):
+        reqs = requests_from_trace(g.served_requests(rate=lam, duration=dur, seed=5))
+        srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True)
+        s = srv.run_open_loop(reqs, max_wall=dur + 120)
+        out[f"lambda_{int(lam)}"] = {
+            "duration_s": dur, "submitted": s["submitted"], "completed": s["completed"],
+            "converged": s["converged"], "solves_per_s": round(s["solves_per_s"], 1),
+            "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+            "max_residual": s["max_residual"], "peak_device_GB": round(srv.peak_in_use / 1e9, 2)}
+    base = requests_from_trace(g.served_requests(rate=1000, duration=5.0, seed=5))[:base_requests]
+    b = one_at_a_time(base, mixed=False)
+    out["one_at_a_time_fp64"] = {"requests": b["requests"],
+                                 "solves_per_s": round(b["solves_per_s"], 1),
+                                 "with_setup_solves_per_s": round(b["wall_solves_per_s"], 1)}
+    top = out["lambda_10000"]["solves_per_s"]
+    out["speedup_vs_solve_only"] = round(top / b["solves_per_s"], 2)
+    out["speedup_vs_with_setup"] = round(top / b["wall_solves_per_s"], 2)
+    return out
+
+
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "spmv_lap3d192_pmc.txt")
 
 
@@ -193,6 +310,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solvers", action="store_true",
+                    help="skip the mixed-precision, batched and served CG measurements")
+    ap.add_argument("--served-seconds", type=float, default=3.0,
+                    help="duration of the lambda = 100 and 1k served streams (60 = full)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the slab-share and Zipf SpMV measurements")
     ap.add_argument("--cg-solves", type=int, default=5)
@@ -250,6 +371,14 @@ def main():
     if not args.no_cg:
         log("bench: CG 1e-10")
         out["cg"] = cg_bench(ks, A, dA, args.cg_solves)
+    del dA, dx, dy
+    if not args.no_solvers:
+        log("bench: mixed-precision CG, 128^3 diffusion")
+        out["cg_mixed_diff128"] = mixed_bench(ks)
+        log("bench: batched CG, 4096 systems")
+        out["cg_batched_4096"] = batched_bench(ks)
+        log("bench: served CG streams")
+        out["cg_served"] = served_bench((args.served_seconds,) * 2 + (1.0,))
     print(json.dumps(out), flush=True)
 
 

@@ -350,6 +350,20 @@ def batched_systems(count: int = 4096, nmin: int = 64, nmax: int = 65536, seed: 
     return out
 
 
+def batched_specs(count: int = 4096, nmin: int = 64, nmax: int = 65536, seed: int = 4):
+    """The batched workload as request descriptions: the same draws as
+    ``batched_systems`` for the sizes, aspects and shifts, then one seed per
+    system (b = ``hash_uniform(n, seed_i)``, assembled on the device).
+    Returns a list of (dims, shift, seed_i)."""
+    rng = np.random.default_rng(seed)
+    logn = rng.uniform(math.log(nmin), math.log(nmax), size=count)
+    aspect = rng.uniform(1.0, 4.0, size=count)
+    shift = rng.uniform(0.01, 1.0, size=count)
+    seeds = rng.integers(0, 2**31 - 1, size=count)
+    return [(rect_dims(math.exp(logn[i]), aspect[i]), float(shift[i]), int(seeds[i]))
+            for i in range(count)]
+
+
 def hash_uniform(n: int, seed: int, start: int = 0) -> np.ndarray:
     """Counter-based uniform draws in [-1, 1): element i is 2u - 1 with u the
     top 53 bits of SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) applied to
