@@ -1,11 +1,11 @@
 """GPU parity: SpMV, FP32 SpMV, multi-RHS SpMV and the vector ops through the
 C ABI, against the CPU oracle on the same seeded inputs.
 
-Bar: the row-block kernel sums a row in index order from 0.0 whenever its
-block holds >= 128 rows (all stencil and LCG matrices here), so those results
-must be bit-identical to the oracle's sequential row loop
-(src/spmv_kernel.hpp:90-97).  Matrices with long rows are checked to 1e-12
-relative (the target of the reference's LCG test, tests/test_spmv_kernel.cpp:205).
+Bar: the row-tile kernel sums every row of <= 32 entries in index order
+from 0.0 with one thread, so those rows must be bit-identical to the
+oracle's sequential row loop (src/spmv_kernel.hpp:90-97); longer rows (a
+wave or a workgroup per row) are checked to 1e-12 relative (the target of
+the reference's LCG test, tests/test_spmv_kernel.cpp:205).
 """
 import math
 
@@ -265,3 +265,22 @@ def test_axpy_aliasing(ks, orc):
     dx, dy = _dev(x), _dev(y)
     ks.axpy(2.0, dx, dy, dy)  # z aliases y (vec_ops.hpp:14)
     assert np.array_equal(dy.cpu().numpy(), orc.axpy(2.0, x, y))
+
+
+def test_autotuner_keeps_results(ks, orc, tmp_path):
+    """The tuner times candidate schedules on the matrix and applies the
+    fastest; the result is unchanged (bit-exact rows of <= 32 entries) and
+    the choice is cached by structure."""
+    from num_amd.autotune import tune_spmv
+    cache = str(tmp_path / "sched.json")
+    for A in (g.laplacian_3d(40), g.zipf_matrix(n=20000, kmax=3000, seed=5)):
+        dA = _up(ks, A)
+        x = np.random.default_rng(9).uniform(-1, 1, A.ncols)
+        y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+        t = tune_spmv(dA, reps=3, cache_file=cache)
+        assert not t["cached"] and t["timed"] > 4 and t["us"] <= t["default_us"]
+        y = _spmv(ks, dA, x)
+        short = np.diff(A.rowptr) <= 32
+        assert np.array_equal(y[short], y_ref[short]) and rel(y, y_ref) < 1e-12
+        again = tune_spmv(_up(ks, A), cache_file=cache)
+        assert again["cached"] and again["schedule"] == t["schedule"]
