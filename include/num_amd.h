@@ -77,6 +77,12 @@ int num_csr_create_host(const int32_t* h_rowptr, const int32_t* h_colind,
                         num_csr** out);
 /* Attach an FP32 copy of the values (same pattern) for the FP32 inner solve. */
 int num_csr_attach_f32(num_csr* A, const float* d_vals32);
+/* Build column - row offsets as int16 into d_rel16 [nnz] for the FP32 inner
+ * solve (no reference counterpart): attached (*attached = 1) only for a
+ * square matrix whose rows all hold <= 32 entries and whose offsets all fit
+ * in int16 (stencils, banded matrices); otherwise the FP32 path keeps the
+ * int32 columns.  Synchronises the stream. */
+int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* attached);
 int num_csr_destroy(num_csr* A);
 /* nrows / ncols / nnz read-only attributes (src/bindings.cpp:79-88). */
 int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz);

@@ -392,6 +392,84 @@ struct EpiPq {
     }
 };
 
+// The scalar step after KA (thread 0): the refinement's FP64 convergence test
+// and new beta (ru), else alpha = rr / p.q.
+__device__ __forceinline__ void mix_after_ka(bool ru, double sa, double sb, MixState* st)
+{
+    if (ru) {
+        const double rr64 = sa, rr32 = sb;
+        const double rel = sqrt(rr64) / st->bnorm;
+        st->rel = rel;
+        st->xsel ^= 1;
+        st->nref += 1;
+        st->ru = 0;
+        st->post_ru = 1;
+        if (rel < st->tol) {
+            st->converged = 1;
+            st->done = 1;
+        } else if (st->final_ || !(rel == rel)) {
+            st->done = 1;
+        } else {
+            st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
+            st->rr = rr32;
+            st->rr_ref = rr32;
+        }
+        return;
+    }
+    const double pq = sa;
+    if (!(fabs(pq) >= 1e-300)) {
+        st->breakdown = 1;
+        st->done = 1;
+    } else {
+        st->alpha = st->rr / pq;
+    }
+}
+
+// The scalar step after KC (thread 0): beta, or schedule a refinement.
+__device__ __forceinline__ void mix_after_kc(double rrn, MixState* st)
+{
+    const int it = st->it + 1;
+    st->it = it;
+    st->psel ^= 1;
+    const double est = sqrt(rrn) / st->bnorm;
+    if (it >= st->max_iter) {
+        st->ru = 1;
+        st->final_ = 1;
+    } else if (est < st->tol || sqrt(rrn) < st->delta * sqrt(st->rr_ref) || !(rrn == rrn)) {
+        st->ru = 1;  // beta comes from the refined residual
+    } else {
+        st->beta = rrn / st->rr;
+        st->rr = rrn;
+    }
+}
+
+// KB of the mixed slot: alpha, or the refinement's convergence test and beta.
+__global__ __launch_bounds__(kFin) void mix_kb(const double* __restrict__ pa,
+                                               const double* __restrict__ pb, int np,
+                                               MixState* st)
+{
+    __shared__ double red[kFin / kWave];
+    if (st->done) return;
+    const bool ru = st->ru != 0;
+    const double sa = sum_partials_block<kFin>(pa, np, red);
+    const double sb = ru ? sum_partials_block<kFin>(pb, np, red) : 0.0;
+    if (threadIdx.x == 0) mix_after_ka(ru, sa, sb, st);
+}
+
+// KD of the mixed slot: beta, or schedule a refinement.
+__global__ __launch_bounds__(kFin) void mix_kd(const double* __restrict__ pb, int np,
+                                               MixState* st)
+{
+    __shared__ double red[kFin / kWave];
+    if (st->done) return;
+    if (st->post_ru) {
+        if (threadIdx.x == 0) st->post_ru = 0;
+        return;
+    }
+    const double rrn = sum_partials_block<kFin>(pb, np, red);
+    if (threadIdx.x == 0) mix_after_kc(rrn, st);
+}
+
 // KA of the mixed slot: FP32 step SpMV, or the FP64 refinement SpMV.
 template <int NPT, bool NT32, bool NT64>
 __global__ __launch_bounds__(kSpmvBS) void mix_spmv_kernel(
@@ -429,6 +507,55 @@ __global__ __launch_bounds__(kSpmvBS) void mix_spmv_kernel(
     }
 }
 
+// KA of the mixed slot for a matrix with int16 column offsets and short rows
+// only: one workgroup per PAIR of schedule tiles.  FP32 step: both tiles are
+// staged at once (FP32 values + int16 offsets fill the LDS of one FP64
+// tile); FP64 refinement: the two tiles in turn.  Partials: one per pair.
+template <int NPT, bool NT32, bool NT64>
+__global__ __launch_bounds__(kSpmvBS) void mix_pair_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ v64, const float* __restrict__ v32,
+    const int16_t* __restrict__ rel, const TileEntry* __restrict__ sched, int ntiles,
+    int npairs, int chunk, MixArgs a)
+{
+    __shared__ double lv[kSpmvBS * NPT];
+    __shared__ int32_t lc[kSpmvBS * NPT];
+    __shared__ double red[2 * kSpmvBS / kWave];
+    const int p = chunk_tile(blockIdx.x, npairs, chunk);
+    if (p < 0) return;
+    const MixState* st = a.st;
+    if (st->done) return;
+    const int t0 = 2 * p, t1 = min(2 * p + 2, ntiles);
+    double ca = 0.0, cb = 0.0;
+    if (st->ru) {
+        const int xs = st->xsel;
+        GatherXsum g{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
+        EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, a.pa, a.pb};
+        for (int t = t0; t < t1; ++t) {
+            double xa, xb;
+            spmv_tile_body<double, kSpmvBS, NPT, NT64>(t, rowptr, colind, v64, g, sched, epi, lv,
+                                                       lc, xa, xb);
+            ca += xa;
+            cb += xb;
+            __syncthreads();  // the LDS is staged again for the next tile
+        }
+        block_sum2<kSpmvBS>(ca, cb, red);
+        if (threadIdx.x == 0) {
+            a.pa[p] = ca;
+            a.pb[p] = cb;
+        }
+    } else {
+        GatherPNew<float> g{a.r32, a.p0, a.p1, &st->beta, &st->psel, nullptr, 0.0f};
+        g.prepare();
+        EpiPq epi{st->psel ? a.p0 : a.p1, a.q, a.pa, a.pb};
+        spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
+                                              reinterpret_cast<float*>(lv),
+                                              reinterpret_cast<int16_t*>(lc), ca, cb);
+        const double s = block_sum<kSpmvBS>(ca, red);
+        if (threadIdx.x == 0) a.pa[p] = s;
+    }
+}
+
 __global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol, double delta,
                          int32_t max_iter)
 {
@@ -456,49 +583,10 @@ __global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol
     st->final_ = max_iter <= 0 ? 1 : 0;
 }
 
-// KB: alpha, or the refinement's FP64 convergence test and new beta
-__global__ __launch_bounds__(kFin) void mix_kb(const double* __restrict__ pa,
-                                               const double* __restrict__ pb, int np,
-                                               MixState* st)
-{
-    __shared__ double red[kFin / kWave];
-    if (st->done) return;
-    if (st->ru) {
-        const double rr64 = sum_partials_block<kFin>(pa, np, red);
-        const double rr32 = sum_partials_block<kFin>(pb, np, red);
-        if (threadIdx.x == 0) {
-            const double rel = sqrt(rr64) / st->bnorm;
-            st->rel = rel;
-            st->xsel ^= 1;
-            st->nref += 1;
-            st->ru = 0;
-            st->post_ru = 1;
-            if (rel < st->tol) {
-                st->converged = 1;
-                st->done = 1;
-            } else if (st->final_ || !(rel == rel)) {
-                st->done = 1;
-            } else {
-                st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
-                st->rr = rr32;
-                st->rr_ref = rr32;
-            }
-        }
-        return;
-    }
-    const double pq = sum_partials_block<kFin>(pa, np, red);
-    if (threadIdx.x == 0) {
-        if (!(fabs(pq) >= 1e-300)) {
-            st->breakdown = 1;
-            st->done = 1;
-        } else {
-            st->alpha = st->rr / pq;
-        }
-    }
-}
-
-// KC: x32 += alpha p_new ; r32 -= alpha q (FP32), partials of r32.r32 (FP64)
-__global__ __launch_bounds__(kRedBlock) void mix_update(int64_t n, MixArgs a)
+// KC of the mixed slot: x32 += alpha p_new, r32 -= alpha q (FP32) over float4
+// groups (n4 = n / 4 of them; the n % 4 tail by thread 0 of workgroup 0),
+// per-workgroup partials of r32.r32 (FP64).
+__global__ __launch_bounds__(kRedBlock) void mix_update4(int64_t n, MixArgs a)
 {
     __shared__ double red[kRedBlock / kWave];
     const MixState* st = a.st;
@@ -509,45 +597,39 @@ __global__ __launch_bounds__(kRedBlock) void mix_update(int64_t n, MixArgs a)
     float* __restrict__ x32 = st->xsel ? a.x32b : a.x32a;
     const float* __restrict__ q = a.q;
     float* __restrict__ r = a.r32;
+    const int64_t n4 = n >> 2;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        x32[i] = al * p[i] + x32[i];
-        const float ri = nal * q[i] + r[i];
-        r[i] = ri;
-        acc += (double)ri * (double)ri;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const float4 pv = reinterpret_cast<const float4*>(p)[i];
+        const float4 qv = reinterpret_cast<const float4*>(q)[i];
+        float4 xv = reinterpret_cast<float4*>(x32)[i];
+        float4 rv = reinterpret_cast<float4*>(r)[i];
+        xv.x = al * pv.x + xv.x;
+        xv.y = al * pv.y + xv.y;
+        xv.z = al * pv.z + xv.z;
+        xv.w = al * pv.w + xv.w;
+        rv.x = nal * qv.x + rv.x;
+        rv.y = nal * qv.y + rv.y;
+        rv.z = nal * qv.z + rv.z;
+        rv.w = nal * qv.w + rv.w;
+        reinterpret_cast<float4*>(x32)[i] = xv;
+        reinterpret_cast<float4*>(r)[i] = rv;
+        acc += (double)rv.x * (double)rv.x;
+        acc += (double)rv.y * (double)rv.y;
+        acc += (double)rv.z * (double)rv.z;
+        acc += (double)rv.w * (double)rv.w;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int64_t i = n4 * 4; i < n; ++i) {
+            x32[i] = al * p[i] + x32[i];
+            const float ri = nal * q[i] + r[i];
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
     }
     const double s = block_sum<kRedBlock>(acc, red);
     if (threadIdx.x == 0) a.pb[blockIdx.x] = s;
-}
-
-// KD: beta, or schedule a refinement
-__global__ __launch_bounds__(kFin) void mix_kd(const double* __restrict__ pb, int np,
-                                               MixState* st)
-{
-    __shared__ double red[kFin / kWave];
-    if (st->done) return;
-    if (st->post_ru) {
-        if (threadIdx.x == 0) st->post_ru = 0;
-        return;
-    }
-    const double rrn = sum_partials_block<kFin>(pb, np, red);
-    if (threadIdx.x == 0) {
-        const int it = st->it + 1;
-        st->it = it;
-        st->psel ^= 1;
-        const double est = sqrt(rrn) / st->bnorm;
-        if (it >= st->max_iter) {
-            st->ru = 1;
-            st->final_ = 1;
-        } else if (est < st->tol || sqrt(rrn) < st->delta * sqrt(st->rr_ref) ||
-                   !(rrn == rrn)) {
-            st->ru = 1;  // beta comes from the refined residual
-        } else {
-            st->beta = rrn / st->rr;
-            st->rr = rrn;
-        }
-    }
 }
 
 __global__ __launch_bounds__(kRedBlock) void mix_finish(int64_t n, const double* __restrict__ x64,
@@ -615,8 +697,50 @@ static void launch_mix_spmv_npt(const num_csr* A, const MixArgs& a, hipStream_t 
     }
 }
 
+template <int NPT>
+static void launch_mix_pair_npt(const num_csr* A, const MixArgs& a, hipStream_t st)
+{
+    const Schedule& s = A->sched;
+    const int npairs = (s.nblocks + 1) / 2;
+    const int chunk = tile_chunk(npairs);
+    const int grid = chunk_grid(npairs, chunk);
+    const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
+    if (s.nt32) {
+        if (s.nt64)
+            mix_pair_kernel<NPT, true, true><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
+                chunk, a);
+        else
+            mix_pair_kernel<NPT, true, false><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
+                chunk, a);
+    } else {
+        if (s.nt64)
+            mix_pair_kernel<NPT, false, true><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
+                chunk, a);
+        else
+            mix_pair_kernel<NPT, false, false><<<grid, kSpmvBS, 0, st>>>(
+                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
+                chunk, a);
+    }
+}
+
+// The paired int16 path applies when the offsets are attached, no tile is a
+// single long row and the staging depth is one the pair kernel is built for.
+static bool mix_paired(const num_csr* A)
+{
+    return A->rel16 && A->sched.long_rows == 0 && A->max_row_nnz <= kShortRow &&
+           (A->sched.npt == 5 || A->sched.npt == 7);
+}
+
 static void launch_mix_spmv(const num_csr* A, const MixArgs& a, hipStream_t st)
 {
+    if (mix_paired(A)) {
+        if (A->sched.npt == 5) launch_mix_pair_npt<5>(A, a, st);
+        else launch_mix_pair_npt<7>(A, a, st);
+        return;
+    }
     switch (A->sched.npt) {
     case 5: launch_mix_spmv_npt<5>(A, a, st); break;
     case 7: launch_mix_spmv_npt<7>(A, a, st); break;
@@ -732,7 +856,6 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     double* bb = (double*)(w + L.off_bb);
     a.st = (MixState*)(w + L.off_state);
     const int64_t n = A->nrows;
-    const int np_spmv = A->sched.nblocks;
     const int gu = dot_grid(n);
     const int ge = elementwise_grid(n);
 
@@ -742,10 +865,11 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     if (rc) return rc;
     mix_init<<<1, 64, 0, st>>>(bb, a.st, tol, delta, max_iter);
     NUM_LAUNCH_CHECK();
+    const int np_spmv = mix_paired(A) ? (A->sched.nblocks + 1) / 2 : A->sched.nblocks;
     auto slot = [&]() -> int {
         launch_mix_spmv(A, a, st);
         mix_kb<<<1, kFin, 0, st>>>(a.pa, a.pb, np_spmv, a.st);
-        mix_update<<<gu, kRedBlock, 0, st>>>(n, a);
+        mix_update4<<<gu, kRedBlock, 0, st>>>(n, a);
         mix_kd<<<1, kFin, 0, st>>>(a.pb, gu, a.st);
         NUM_LAUNCH_CHECK();
         return NUM_OK;

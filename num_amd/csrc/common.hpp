@@ -55,7 +55,7 @@ __device__ __forceinline__ int chunk_tile(int b, int ntiles, int chunk)
     return t < ntiles ? t : -1;
 }
 // Chunk size used for a launch of `ntiles` tiles and the grid covering it.
-inline int tile_chunk(int ntiles) { return ntiles >= 4096 ? 16 : 1; }
+inline int tile_chunk(int ntiles) { return ntiles >= 16384 ? 64 : ntiles >= 4096 ? 16 : 1; }
 inline int chunk_grid(int ntiles, int chunk)
 {
     const int nchunks = (ntiles + chunk - 1) / chunk;

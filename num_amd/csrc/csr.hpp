@@ -39,6 +39,7 @@ struct num_csr {
     const int32_t* colind = nullptr;
     const double* vals = nullptr;
     const float* vals32 = nullptr;
+    const int16_t* rel16 = nullptr;     // column - row as int16 (FP32 path), if attached
     bool owns = false;                  // rowptr/colind/vals allocated here
     std::vector<int32_t> h_rowptr;      // host copy, drives scheduling
     int32_t max_row_nnz = 0;

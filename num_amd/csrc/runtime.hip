@@ -179,6 +179,23 @@ static int finish_create(num_csr* A)
                           A->sched);
 }
 
+// rel[j] = colind[j] - row(j) for every entry; *bad = 1 if one does not fit
+// in int16.
+__global__ void rel16_kernel(int32_t nrows, const int32_t* __restrict__ rowptr,
+                             const int32_t* __restrict__ colind, int16_t* __restrict__ rel,
+                             int32_t* __restrict__ bad)
+{
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int flag = 0;
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+        const int32_t d = colind[j] - r;
+        if (d < -32768 || d > 32767) flag = 1;
+        rel[j] = (int16_t)d;
+    }
+    if (flag) atomicOr(bad, 1);
+}
+
 }  // namespace numamd
 
 using namespace numamd;
@@ -291,6 +308,37 @@ int num_csr_attach_f32(num_csr* A, const float* d_vals32)
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
     if (!d_vals32 && A->nnz > 0) return fail(NUM_EINVAL, "d_vals32 is NULL");
     A->vals32 = d_vals32;
+    return NUM_OK;
+}
+
+int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* attached)
+{
+    if (!A || !attached) return fail(NUM_EINVAL, "matrix or attached is NULL");
+    *attached = 0;
+    if (!d_rel16 && A->nnz > 0) return fail(NUM_EINVAL, "d_rel16 is NULL");
+    int rc = check_device(A);
+    if (rc) return rc;
+    // only for matrices every row of which the one-thread walk handles
+    if (A->nrows != A->ncols || A->max_row_nnz > kShortRow || A->nnz == 0) return NUM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    int32_t* d_bad = nullptr;
+    NUM_HIP(hipMalloc(&d_bad, sizeof(int32_t)));
+    int32_t h_bad = 0;
+    hipError_t e = hipMemcpyAsync(d_bad, &h_bad, sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        rel16_kernel<<<(unsigned)((A->nrows + 255) / 256), 256, 0, st>>>(
+            A->nrows, A->rowptr, A->colind, d_rel16, d_bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&h_bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_bad);
+    if (e != hipSuccess) return fail(NUM_EHIP, "rel16 build failed: %s", hipGetErrorString(e));
+    if (h_bad == 0) {
+        A->rel16 = d_rel16;
+        *attached = 1;
+    }
     return NUM_OK;
 }
 

@@ -111,14 +111,14 @@ __device__ __forceinline__ void spmv_tile_run(
         }
         return;
     }
-    // row bounds for the walk phase, fetched together with the stage loads
+    // row bounds for the walk phase, fetched together with the stage loads:
+    // branch-free (a thread past the tile's rows reads rowptr[r1] twice) so
+    // the loads stay in flight while the stage loads are issued
     const int nr = r1 - r0;
     const bool active = t < nr;
-    int rb = 0, re = 0;
-    if (active) {
-        rb = rowptr[r0 + t] - j0;
-        re = rowptr[r0 + t + 1] - j0;
-    }
+    const int tr = min(t, nr);
+    const int rb_raw = rowptr[r0 + tr];
+    const int re_raw = rowptr[r0 + tr + (active ? 1 : 0)];
     // 1. stage
     if (nz > 0) {
         const int last = j1 - 1;
@@ -140,6 +140,7 @@ __device__ __forceinline__ void spmv_tile_run(
         }
     }
     __syncthreads();
+    const int rb = rb_raw - j0, re = re_raw - j0;
     const bool wide = e0.wide != 0;  // uniform over the workgroup
     // 2a. short rows: one thread per row, entries in index order from 0, 8
     //     gathers in flight before the (ordered) sums
@@ -212,6 +213,90 @@ __device__ __forceinline__ void spmv_tile_run(
                 cb += tb;
             }
         }
+    }
+}
+
+// FP32 walk over a group of consecutive tiles whose rows all hold <= kShortRow
+// entries, with column ids stored as int16 offsets from their row (half the
+// index bytes of int32).  Thread t owns rows t, t + BS, ..., t + (RPT-1) BS of
+// the group; the group's entries (<= RPT * NPT * BS) are staged at once, so
+// one workgroup moves RPT tiles' worth of data behind one latency chain.
+// Each row is summed in index order from 0.0 (the row loop's arithmetic).
+template <int BS, int NPT, int RPT, bool NT, class Epi, class Gather>
+__device__ __forceinline__ void spmv_rel16_run(
+    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
+    const int16_t* __restrict__ rel, const float* __restrict__ vals, const Gather& x,
+    const Epi& epi, float* __restrict__ lv, int16_t* __restrict__ lc, double& ca, double& cb)
+{
+    constexpr int M = NPT * RPT;
+    const int t = threadIdx.x;
+    const int r0 = e0.row, j0 = e0.nz0;
+    const int r1 = e1.row, j1 = e1.nz0;
+    const int nz = j1 - j0, nr = r1 - r0;
+    int rb[RPT], re[RPT];  // raw rowptr values, branch-free loads (see spmv_tile_run)
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int lr = min(t + k * BS, nr);
+        rb[k] = rowptr[r0 + lr];
+        re[k] = rowptr[r0 + lr + (t + k * BS < nr ? 1 : 0)];
+    }
+    if (nz > 0) {
+        const int last = j1 - 1;
+        float v[M];
+        int16_t c[M];
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const int j = min(j0 + t + s * BS, last);
+            v[s] = ld_mat<NT>(vals + j);
+            c[s] = ld_mat<NT>(rel + j);
+        }
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const int k = t + s * BS;
+            if (k < nz) {
+                lv[k] = v[s];
+                lc[k] = c[s];
+            }
+        }
+    }
+    __syncthreads();
+    ca = 0.0;
+    cb = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int lr = t + k * BS;
+        if (lr >= nr) continue;
+        const int row = r0 + lr;
+        const int b = rb[k] - j0, e = re[k] - j0;
+        float acc = 0.0f;
+        int q = b;
+        for (; q + 7 < e; q += 8) {
+            float vv[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                vv[u] = lv[q + u];
+                xv[u] = x(row + lc[q + u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += vv[u] * xv[u];
+        }
+        float vv[7], xv[7];
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
+            const int qq = q + u;
+            const bool ok = qq < e;
+            const int qs = ok ? qq : 0;  // slot 0 is always inside the LDS array
+            vv[u] = lv[qs];
+            const int c = lc[qs];
+            xv[u] = x(row + (ok ? c : 0));  // the row's own column exists (square)
+        }
+#pragma unroll
+        for (int u = 0; u < 7; ++u)
+            if (q + u < e) acc += vv[u] * xv[u];
+        double ta = 0.0, tb = 0.0;
+        epi(row, acc, ta, tb, x);
+        ca += ta;
+        cb += tb;
     }
 }
 

@@ -131,7 +131,7 @@ class CrsMatrix:
     """Device CSR matrix (src/crs_matrix.hpp:37-73).  Owns its device arrays
     (torch tensors) and the native handle with its row-block schedule."""
 
-    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_nrows", "_ncols",
+    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_rel16", "_nrows", "_ncols",
                  "_nnz", "_h_rowptr", "__weakref__")
 
     def __init__(self):
@@ -144,6 +144,7 @@ class CrsMatrix:
         self._h = None
         self._rowptr, self._colind, self._values = rowptr, colind, values
         self._values32 = None
+        self._rel16 = None
         self._h_rowptr = h_rowptr
         h = C.c_void_p()
         call("num_csr_create", h_rowptr.ctypes.data_as(C.c_void_p), _ptr(rowptr), _ptr(colind),
@@ -178,6 +179,11 @@ class CrsMatrix:
         if self._values32 is None:
             self._values32 = self._values.to(torch.float32)
             call("num_csr_attach_f32", self._h, _ptr(self._values32))
+            # int16 column offsets for the FP32 walk where they fit
+            rel = torch.empty(max(1, self.nnz), dtype=torch.int16, device=self._values.device)
+            ok = C.c_int32()
+            call("num_csr_attach_rel16", self._h, _ptr(rel), _stream(), C.byref(ok))
+            self._rel16 = rel if ok.value else None
 
     def set_schedule(self, rows_per_block: int = 0, nnz_per_block: int = 0, chunk: int = -1,
                      nt64: int = -1, nt32: int = -1) -> None:

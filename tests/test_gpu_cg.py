@@ -165,3 +165,50 @@ def test_mixed_refined(ks, orc):
     assert true < 1e-10
     x64, r64 = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=20000)
     assert rel(xh, x64) < 1e-7
+
+
+def _hub_spd(L=40, hub_deg=120, seed=1):
+    """2-D Laplacian plus one 'hub' row/column joined to hub_deg nodes:
+    symmetric, strictly diagonally dominant (SPD), with one row far longer
+    than 32 entries, so the FP32 path keeps int32 columns."""
+    import scipy.sparse as sp
+    A = g.laplacian_2d(L).to_scipy().tolil()
+    n = A.shape[0]
+    rng = np.random.default_rng(seed)
+    nbrs = rng.choice(np.arange(1, n), hub_deg, replace=False)
+    for j in nbrs:
+        A[0, j] = -0.5
+        A[j, 0] = -0.5
+        A[j, j] = A[j, j] + 0.5
+    A[0, 0] = 0.5 * hub_deg + 1.0
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return g.CSR(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64),
+                 n, n)
+
+
+@pytest.mark.parametrize("case", ["diffusion", "lap2d_odd", "hub"])
+def test_mixed_paths(ks, orc, case):
+    """The paired int16-offset FP32 path (stencils) and the int32 path (a row
+    longer than 32 entries) both reach the FP64 true residual, and agree
+    with the FP64 oracle solution."""
+    if case == "diffusion":
+        A = g.diffusion_3d(30, sigma=1.0, seed=5)
+    elif case == "lap2d_odd":
+        A = g.stencil((123, 77), 0.01)   # odd tile count: the last pair is a single tile
+    else:
+        A = _hub_spd()
+    b = np.random.default_rng(2).uniform(-1, 1, A.nrows)
+    dA = _up(ks, A)
+    dA.ensure_f32()
+    assert (dA._rel16 is not None) == (case != "hub")
+    x = _dev(np.zeros(A.nrows))
+    r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+    assert r["converged"] and r["residual"] < 1e-10
+    xh = x.cpu().numpy()
+    true = np.linalg.norm(b - A.to_scipy() @ xh) / np.linalg.norm(b)
+    assert true < 1e-10
+    x64, r64 = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=20000)
+    assert rel(xh, x64) < 1e-7
+    # the FP32 steps stay close to the FP64 iteration count
+    assert r["inner_total"] <= 1.5 * r64["iters"] + 10
