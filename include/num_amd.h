@@ -201,7 +201,8 @@ int64_t num_engine_stencil_bytes(int32_t ndim, int32_t nx, int32_t ny, int32_t n
  * nx x ny x nz grid, assembled on the device inside d_buf (>= the bytes
  * above, 256-byte aligned): b[i] = 2u - 1 with u the SplitMix64 draw of
  * (seed, i) (generators.hash_uniform), x0 = 0.  The solution is the first n
- * doubles of d_buf. */
+ * doubles of d_buf.  Grid lines (2-D) and planes (3-D) must hold <= 32767
+ * points, so that column offsets fit the int16 copy of the mixed form. */
 int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, int32_t nz,
                            double shift, uint64_t seed, double tol, int32_t max_iter,
                            int32_t mode, double delta, void* d_buf, int64_t buf_bytes,

@@ -56,7 +56,12 @@ enum : int32_t {
 
 constexpr int kEngNpt = 7;        // LDS entries per thread of the engine's tile kernel
 constexpr int kChunkRows = 2048;  // rows per KC workgroup (256 threads x 8)
-constexpr int kStencilRpt = 256;  // rows per tile of an assembled stencil (<= 7 entries/row)
+// Rows per tile of an assembled stencil (<= 7 entries per row): 256, or 512
+// in the mixed form, whose FP32 steps stage a 512-row tile at once (FP32
+// values and int16 column offsets fill the LDS of one 256-row FP64 tile) and
+// whose FP64 phases walk it as two 256-row halves.
+constexpr int kStencilRpt = 256;
+constexpr int kStencilRptPair = 512;
 
 struct SysDesc {
     const int32_t* rowptr;
@@ -77,6 +82,7 @@ struct SysDesc {
     float* q32;
     float* p32a;
     float* p32b;
+    const int16_t* rel16;  // column - row (FP32 phases), with 512-row tiles; or nullptr
     int32_t n, ntiles, mode, rpt;
 };
 
@@ -207,7 +213,34 @@ __global__ __launch_bounds__(kSpmvBS) void eng_ka(const BTile* __restrict__ bts,
     const TileEntry e0 = {e.r0, e.wide, e.nz0, 0};
     const TileEntry e1 = {e.r1, kWalkShort, e.nz1, 0};
     double ca = 0.0, cb = 0.0;
-    if (ph == PH_INIT) {
+    if (d->rel16) {
+        // 512-row tile of a mixed stencil system
+        if (ph == PH_RUN32) {
+            const int ps = st->psel;
+            EGatherPNew<float> g{d->r32, ps ? d->p32b : d->p32a, (float)st->beta};
+            EEpiStep<float> epi{ps ? d->p32a : d->p32b, d->q32};
+            spmv_rel16_run<kSpmvBS, kEngNpt, 2, NT32>(e0, e1, d->rowptr, d->rel16, d->vals32, g,
+                                                      epi, reinterpret_cast<float*>(lv),
+                                                      reinterpret_cast<int16_t*>(lc), ca, cb);
+        } else {
+            // FP64 refinement (the only other phase of a mixed system): the two
+            // 256-row halves in turn
+            const int xs = st->xsel;
+            EGatherXsum g{xs ? d->x64b : d->x, xs ? d->x32b : d->x32a};
+            EEpiRefine epi{d->b, xs ? d->x : d->x64b, xs ? d->x32a : d->x32b, d->r32,
+                           st->nref == 0};
+            const int rm = min(e.r1, e.r0 + kStencilRpt);
+            const TileEntry em = {rm, kWalkShort, d->rowptr[rm], 0};
+            double xa, xb;
+            spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, em, d->rowptr, d->colind, d->vals,
+                                                          g, epi, lv, lc, xa, xb);
+            __syncthreads();  // the LDS is staged again for the second half
+            spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(em, e1, d->rowptr, d->colind, d->vals,
+                                                          g, epi, lv, lc, ca, cb);
+            ca += xa;
+            cb += xb;
+        }
+    } else if (ph == PH_INIT) {
         GatherX<double> g{d->x};
         EEpiInit epi{d->b, d->r, d->p0};
         spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind, d->vals, g,
@@ -542,14 +575,14 @@ __device__ __forceinline__ int64_t stencil_rowptr_at(int ndim, int nx, int ny, i
 }
 
 // One thread per row: rowptr, columns (-z, -y, -x, self, +x, +y, +z), values
-// (-1 off the diagonal, 2*ndim + shift on it), the FP32 copy, b, x = 0 and,
-// for the mixed form, the zeroed x32a / p32a.
+// (-1 off the diagonal, 2*ndim + shift on it), b, x = 0 and, for the mixed
+// form, the FP32 copy, the int16 column offsets and the zeroed x32a / p32a.
 __global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double shift,
                                        uint64_t seed, int32_t* __restrict__ rowptr,
                                        int32_t* __restrict__ colind, double* __restrict__ vals,
-                                       float* __restrict__ vals32, double* __restrict__ b,
-                                       double* __restrict__ x, float* __restrict__ z32a,
-                                       float* __restrict__ z32b)
+                                       float* __restrict__ vals32, int16_t* __restrict__ rel16,
+                                       double* __restrict__ b, double* __restrict__ x,
+                                       float* __restrict__ z32a, float* __restrict__ z32b)
 {
     const int64_t n = (int64_t)nx * ny * nz;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -566,6 +599,7 @@ __global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double 
         colind[p] = col;
         vals[p] = val;
         if (vals32) vals32[p] = (float)val;
+        if (rel16) rel16[p] = (int16_t)(col - row);
         ++p;
     };
     if (ndim >= 3 && iz > 0) put(row - sz, -1.0);
@@ -628,6 +662,10 @@ static int stencil_shape(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, Stenc
     g.nz = nz;
     g.n = (int64_t)nx * ny * nz;
     if (g.n * (2 * ndim + 1) >= (int64_t)1 << 31) return fail(NUM_EINVAL, "grid too large");
+    if ((int64_t)nx * ny > 32767 && ndim == 3)
+        return fail(NUM_EINVAL, "3-D grid planes above 32767 points are not supported");
+    if ((int64_t)nx > 32767 && ndim >= 2)
+        return fail(NUM_EINVAL, "grid lines above 32767 points are not supported");
     const int64_t ext[3] = {nx, ny, nz};
     g.nnz = g.n;
     for (int a = 0; a < ndim; ++a) g.nnz += 2 * (ext[a] - 1) * (g.n / ext[a]);
@@ -635,13 +673,13 @@ static int stencil_shape(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, Stenc
 }
 
 // Device bytes of an assembled stencil system: x, b, rowptr, colind, values
-// (+ FP32 values), workspace -- x first, so the solution is the first n
-// doubles of the buffer.
+// (+ FP32 values and int16 column offsets), workspace -- x first, so the
+// solution is the first n doubles of the buffer.
 static int64_t stencil_bytes(const StencilShape& g, int32_t mode)
 {
     const size_t n = (size_t)g.n, nnz = (size_t)g.nnz;
     size_t b = 2 * al256(n * 8) + al256((n + 1) * 4) + al256(nnz * 4) + al256(nnz * 8);
-    if (mode == 1) b += al256(nnz * 4);
+    if (mode == 1) b += al256(nnz * 4) + al256(nnz * 2);
     return (int64_t)b + eng_work_bytes((int64_t)n, mode);
 }
 
@@ -986,6 +1024,7 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     d.vals = A->vals;
     d.vals32 = A->vals32;
     d.tiles = static_cast<const TileEntry*>(A->sched.d_tiles);
+    d.rel16 = nullptr;
     d.b = d_b;
     d.x = d_x;
     d.n = A->nrows;
@@ -1045,9 +1084,12 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
     int32_t* colind = (int32_t*)(w + o); o += al256(nnz * 4);
     double* vals = (double*)(w + o); o += al256(nnz * 8);
     float* vals32 = nullptr;
+    int16_t* rel16 = nullptr;
     if (mode == 1) {
         vals32 = (float*)(w + o);
         o += al256(nnz * 4);
+        rel16 = (int16_t*)(w + o);
+        o += al256(nnz * 2);
     }
     carve_work(d, w + o, n, mode);
     d.rowptr = rowptr;
@@ -1055,12 +1097,13 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
     d.vals = vals;
     d.vals32 = vals32;
     d.tiles = nullptr;
-    d.rpt = kStencilRpt;
+    d.rel16 = rel16;
+    d.rpt = mode == 1 ? kStencilRptPair : kStencilRpt;
     d.n = (int32_t)n;
-    d.ntiles = (int32_t)((n + kStencilRpt - 1) / kStencilRpt);
+    d.ntiles = (int32_t)((n + d.rpt - 1) / d.rpt);
     d.mode = mode;
     stencil_request_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-        ndim, nx, ny, nz, shift, seed, rowptr, colind, vals, vals32, b, d.x,
+        ndim, nx, ny, nz, shift, seed, rowptr, colind, vals, vals32, rel16, b, d.x,
         mode == 1 ? d.x32a : nullptr, mode == 1 ? d.p32a : nullptr);
     NUM_LAUNCH_CHECK();
     SysState h = initial_state(tol, max_iter, mode, delta);

@@ -46,6 +46,8 @@ def test_engine_stencil_assembly(ks, dims, mixed):
     assert np.array_equal(arr("values", np.float64, nnz), A.values)
     if mixed:
         assert np.array_equal(arr("values32", np.float32, nnz), A.values.astype(np.float32))
+        rows = np.repeat(np.arange(n), np.diff(A.rowptr))
+        assert np.array_equal(arr("rel16", np.int16, nnz), (A.colind - rows).astype(np.int16))
     assert np.array_equal(arr("b", np.float64, n), g.hash_uniform(n, seed))
     assert np.array_equal(arr("x", np.float64, n), np.zeros(n))
     assert torch.equal(eng.solution(s).cpu(), torch.zeros(n, dtype=torch.float64))
