@@ -312,6 +312,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solvers", action="store_true",
                     help="skip the mixed-precision, batched and served CG measurements")
+    ap.add_argument("--solvers", default="mixed,batched,served",
+                    help="which solver forms to measure (comma list)")
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="duration of the lambda = 100 and 1k served streams (60 = full)")
     ap.add_argument("--no-extra", action="store_true",
@@ -372,11 +374,14 @@ def main():
         log("bench: CG 1e-10")
         out["cg"] = cg_bench(ks, A, dA, args.cg_solves)
     del dA, dx, dy
-    if not args.no_solvers:
+    solvers = set() if args.no_solvers else set(args.solvers.split(","))
+    if "mixed" in solvers:
         log("bench: mixed-precision CG, 128^3 diffusion")
         out["cg_mixed_diff128"] = mixed_bench(ks)
+    if "batched" in solvers:
         log("bench: batched CG, 4096 systems")
         out["cg_batched_4096"] = batched_bench(ks)
+    if "served" in solvers:
         log("bench: served CG streams")
         out["cg_served"] = served_bench((args.served_seconds,) * 2 + (1.0,))
     print(json.dumps(out), flush=True)
