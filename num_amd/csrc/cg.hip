@@ -518,8 +518,8 @@ __global__ __launch_bounds__(kSpmvBS) void mix_pair_kernel(
     const int16_t* __restrict__ rel, const TileEntry* __restrict__ sched, int ntiles,
     int npairs, int chunk, MixArgs a)
 {
-    __shared__ double lv[kSpmvBS * NPT];
-    __shared__ int32_t lc[kSpmvBS * NPT];
+    __shared__ double lv[kSpmvBS * NPT + 2];   // FP32 pairs: 2 * NPT * BS + 4 floats
+    __shared__ int32_t lc[kSpmvBS * NPT + 2];  // FP32 pairs: 2 * NPT * BS + 4 int16
     __shared__ double red[2 * kSpmvBS / kWave];
     const int p = chunk_tile(blockIdx.x, npairs, chunk);
     if (p < 0) return;

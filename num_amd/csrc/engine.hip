@@ -200,8 +200,8 @@ __global__ __launch_bounds__(kSpmvBS) void eng_ka(const BTile* __restrict__ bts,
                                                   double* __restrict__ pa,
                                                   double* __restrict__ pb)
 {
-    __shared__ double lv[kSpmvBS * kEngNpt];
-    __shared__ int32_t lc[kSpmvBS * kEngNpt];
+    __shared__ double lv[kSpmvBS * kEngNpt + 2];   // FP32 pairs: 2 * NPT * BS + 4 floats
+    __shared__ int32_t lc[kSpmvBS * kEngNpt + 2];  // FP32 pairs: 2 * NPT * BS + 4 int16
     __shared__ double red[2 * kSpmvBS / kWave];
     const int b = chunk_tile(blockIdx.x, nbt, chunk);
     if (b < 0) return;
