@@ -350,6 +350,125 @@ def batched_systems(count: int = 4096, nmin: int = 64, nmax: int = 65536, seed: 
     return out
 
 
+# ---- the reference's other matrix families (python/matrix_generators.py) --
+# Restated for the benchmark harness and breadth tests of the row binning.
+# lattice_gauge draws its phases in the reference's order, so with the same
+# seed it builds the same matrix; random_sparse and power_law repeat the
+# reference's draw sequence call for call.
+
+def _to_csr(A) -> CSR:
+    A = A.tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    return CSR(A.indptr.astype(np.int32), A.indices.astype(np.int32),
+               A.data.astype(np.float64), A.shape[0], A.shape[1])
+
+
+def lattice_gauge(L: int, mass: float = 0.1, seed: int = 42) -> CSR:
+    """Scalar 2-D lattice 'Dirac' operator on an L x L torus with random U(1)
+    link phases, symmetrised (python/matrix_generators.py:85-139): diagonal
+    4 + mass, hopping -0.5 Re(exp(+-i phase)) to the four periodic
+    neighbours, then A = (A + A^T) / 2.  Draws: phi_x (L, L), then phi_y."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    phi_x = rng.uniform(0, 2 * np.pi, (L, L))
+    phi_y = rng.uniform(0, 2 * np.pi, (L, L))
+    N = L * L
+    y, x = np.divmod(np.arange(N), L)
+
+    def idx(xx, yy):
+        return (yy % L) * L + (xx % L)
+    i = np.arange(N)
+    # per site, in the reference's append order: self, +x, -x, +y, -y
+    rows = np.stack([i, i, i, i, i], axis=1).reshape(-1)
+    cols = np.stack([i, idx(x + 1, y), idx(x - 1, y), idx(x, y + 1), idx(x, y - 1)],
+                    axis=1).reshape(-1)
+    vals = np.stack([
+        np.full(N, 4.0 + mass),
+        -0.5 * np.exp(1j * phi_x[y, x]).real,
+        -0.5 * np.exp(-1j * phi_x[y, (x - 1) % L]).real,
+        -0.5 * np.exp(1j * phi_y[y, x]).real,
+        -0.5 * np.exp(-1j * phi_y[(y - 1) % L, x]).real,
+    ], axis=1).reshape(-1)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    return _to_csr(0.5 * (A + A.T))
+
+
+def random_sparse(N: int, nnz_per_row: int = 10, seed: int = 0, symmetric: bool = True) -> CSR:
+    """Random sparse SPD matrix (python/matrix_generators.py:144-170): row i
+    draws min(nnz_per_row - 1, i) distinct columns j < i, each with value
+    -U(0.1, 1) (mirrored when symmetric); the diagonal is the absolute row
+    sum + 1 (strict diagonal dominance)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(N):
+        n_off = min(nnz_per_row - 1, i)
+        if n_off > 0:
+            js = rng.choice(i, size=n_off, replace=False)
+            for j in js:
+                v = rng.uniform(0.1, 1.0)
+                rows.append(i); cols.append(int(j)); vals.append(-v)
+                if symmetric:
+                    rows.append(int(j)); cols.append(i); vals.append(-v)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N), dtype=np.float64)
+    diag = np.asarray(abs(A).sum(axis=1)).ravel() + 1.0
+    return _to_csr(A + sp.diags(diag))
+
+
+def power_law(N: int, m: int = 3, seed: int = 1) -> CSR:
+    """The reference's 'graph Laplacian' of a Barabasi-Albert preferential-
+    attachment graph (python/matrix_generators.py:175-221): a clique of m + 1
+    nodes, then each new node links to m distinct earlier nodes drawn with
+    probability proportional to degree.  The reference builds the adjacency
+    with -1 entries and returns diag(degree) - that, i.e. D + Adj (the
+    signless Laplacian), which is reproduced here: symmetric, weakly
+    diagonally dominant, positive semi-definite."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    edges = set()
+    degrees = np.zeros(N, dtype=np.int64)
+    first = min(m + 1, N)
+    for i in range(first):
+        for j in range(i):
+            edges.add((i, j))
+            degrees[i] += 1
+            degrees[j] += 1
+    for node in range(first, N):
+        deg_sum = degrees[:node].sum()
+        if deg_sum == 0:
+            targets = rng.choice(node, size=min(m, node), replace=False)
+        else:
+            targets = rng.choice(node, size=min(m, node), replace=False,
+                                 p=degrees[:node] / deg_sum)
+        for t in targets:
+            edges.add((node, int(t)))
+            degrees[node] += 1
+            degrees[t] += 1
+    e = np.array(sorted(edges), dtype=np.int64).reshape(-1, 2)
+    rows = np.concatenate([e[:, 0], e[:, 1]])
+    cols = np.concatenate([e[:, 1], e[:, 0]])
+    A = sp.csr_matrix((-np.ones(rows.size), (rows, cols)), shape=(N, N), dtype=np.float64)
+    d = np.asarray(abs(A).sum(axis=1)).ravel()
+    return _to_csr(sp.diags(d) - A)
+
+
+MATRIX_FAMILIES = {
+    "laplacian_2d": lambda N: laplacian_2d(int(N ** 0.5)),
+    "laplacian_3d": lambda N: laplacian_3d(int(round(N ** (1 / 3)))),
+    "lattice_gauge": lambda N: lattice_gauge(int(N ** 0.5)),
+    "random_sparse": lambda N: random_sparse(N),
+    "power_law": lambda N: power_law(N),
+}
+
+
+def get_matrix(name: str, N: int) -> CSR:
+    """The reference's get_matrix(name, N) (python/matrix_generators.py:226-258)."""
+    if name not in MATRIX_FAMILIES:
+        raise ValueError(f"Unknown matrix '{name}'. Choose from: {list(MATRIX_FAMILIES)}")
+    return MATRIX_FAMILIES[name](N)
+
+
 def batched_specs(count: int = 4096, nmin: int = 64, nmax: int = 65536, seed: int = 4):
     """The batched workload as request descriptions: the same draws as
     ``batched_systems`` for the sizes, aspects and shifts, then one seed per

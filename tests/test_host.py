@@ -121,6 +121,46 @@ def test_stencil_nnz_and_layout(dims):
     assert lay["b"] >= 8 * A.nrows and lay["work"] - lay["values32"] >= 4 * A.nnz
 
 
+@pytest.mark.parametrize("name", ["laplacian_2d", "laplacian_3d", "lattice_gauge",
+                                  "random_sparse", "power_law"])
+def test_matrix_families(name):
+    """The reference's families (python/matrix_generators.py): square,
+    symmetric, sorted int32 CSR; SPD by strict diagonal dominance (the
+    Laplacians and power_law are weakly dominant: PSD / SPD by structure)."""
+    A = g.get_matrix(name, 3000)
+    S = A.to_scipy()
+    assert A.nrows == A.ncols and A.rowptr.dtype == np.int32 and A.colind.dtype == np.int32
+    assert abs(S - S.T).max() == 0.0
+    for i in range(0, A.nrows, 97):
+        c = A.colind[A.rowptr[i]:A.rowptr[i + 1]]
+        assert np.all(np.diff(c) > 0)
+    d = S.diagonal()
+    off = np.asarray(abs(S).sum(axis=1)).ravel() - np.abs(d)
+    if name in ("lattice_gauge", "random_sparse"):
+        assert np.all(d > off)
+    else:
+        assert np.all(d >= off - 1e-12)
+    if name == "power_law":
+        # the reference subtracts an adjacency built with -1 entries, so the
+        # matrix is D + Adj (signless Laplacian): row sums are 2 * degree
+        assert np.array_equal(S @ np.ones(A.nrows), 2.0 * d)
+        assert np.diff(A.rowptr).max() > 10 * np.median(np.diff(A.rowptr))  # heavy tail
+    with pytest.raises(ValueError):
+        g.get_matrix("nope", 10)
+
+
+def test_lattice_gauge_small_torus_closed_form():
+    """L = 3: every site has 4 distinct neighbours; A + A^T halves each
+    hopping pair, so A[i, j] = -0.25 (Re e^{i phi} + Re e^{-i phi}) for the
+    shared link."""
+    A = g.lattice_gauge(3, mass=0.1, seed=42).to_scipy().toarray()
+    rng = np.random.default_rng(42)
+    phi_x = rng.uniform(0, 2 * np.pi, (3, 3))
+    # site (x=0, y=0) -> (x=1, y=0) uses phi_x[0, 0] both ways
+    assert A[0, 0] == 4.1
+    assert np.isclose(A[0, 1], -0.5 * np.cos(phi_x[0, 0]))
+
+
 def test_row_partition_coverage_and_balance():
     # tests/test_halo_exchange.py:89-141 semantics at N = 127
     for size in range(1, 9):
@@ -285,3 +325,21 @@ def test_library_rejects_bad_arguments_without_gpu():
         _lib.check(_lib.NUM_EINVAL, "x")
     with pytest.raises(RuntimeError):
         _lib.check(_lib.NUM_ERUNTIME, "x")
+
+
+def test_harness_cpu_leg_and_csv_schema(tmp_path, monkeypatch):
+    """The harness's CPU leg and its CSV schema (python/benchmark_spmv.py:
+    370-377: backend, N, nnz, elapsed_us, bw_GB_s, gflops)."""
+    import csv as _csv
+    from num_amd import harness
+    monkeypatch.chdir(tmp_path)
+    assert harness.main(["spmv", "--N", "4096", "--matrix", "laplacian_2d", "--repeat", "3",
+                         "--backend", "scipy"]) == 0
+    files = list((tmp_path / "benchmark" / "results").glob("spmv_*.csv"))
+    assert len(files) == 1
+    rows = list(_csv.DictReader(open(files[0])))
+    assert list(rows[0].keys()) == ["backend", "N", "nnz", "elapsed_us", "bw_GB_s", "gflops"]
+    assert rows[0]["backend"] == "scipy_cpu" and int(rows[0]["nnz"]) == 5 * 4096 - 4 * 64
+    m = harness.spmv_metrics(1e-3, 1000, 5000, k=2)
+    assert m["bw_GB_s"] == (5000 * 12 + 2 * 1000 * 8 + 1000 * 8) / 1e-3 / 1e9
+    assert m["gflops"] == 2 * 5000 * 2 / 1e-3 / 1e9
