@@ -343,3 +343,33 @@ def test_harness_cpu_leg_and_csv_schema(tmp_path, monkeypatch):
     m = harness.spmv_metrics(1e-3, 1000, 5000, k=2)
     assert m["bw_GB_s"] == (5000 * 12 + 2 * 1000 * 8 + 1000 * 8) / 1e-3 / 1e9
     assert m["gflops"] == 2 * 5000 * 2 / 1e-3 / 1e9
+
+
+def test_serving_host_logic():
+    """Request sizing and the latency summary, without a GPU: the stencil
+    buffer size is pure host arithmetic in the library."""
+    from num_amd.batched import CGEngine, stencil_layout
+    from num_amd.serve import Request, summarize, requests_from_trace
+    for dims, mixed in (((30, 20), False), ((30, 20), True), ((9, 8, 7), True), ((5,), False)):
+        lay = stencil_layout(dims, mixed)
+        b = CGEngine.stencil_bytes(dims, mixed)
+        assert b == lay["work"] + CGEngine.work_bytes(lay["n"], mixed)
+    with pytest.raises(ValueError):
+        CGEngine.stencil_bytes((40000, 2), True)   # grid line above 32767 points
+    with pytest.raises(ValueError):
+        CGEngine.stencil_bytes((2, 2, 2, 2), True)
+    tr = g.served_requests(rate=50.0, duration=2.0, seed=5, nmax=5000)
+    reqs = requests_from_trace(tr)
+    assert [r.rid for r in reqs] == list(range(len(reqs)))
+    assert all(r.tol == 1e-10 for r in reqs)
+    done = []
+    for i, (lat, conv) in enumerate([(0.001, True), (0.004, True), (0.010, False), (0.002, True)]):
+        r = Request(i, t_arrival=0.1 * i, dims=(4, 4), shift=0.1, seed=i)
+        r.t_done = r.t_arrival + lat
+        r.result = {"converged": conv, "iters": 10 + i, "residual": 1e-11 * (i + 1)}
+        done.append(r)
+    s = summarize(done, wall=1.0, submitted=5)
+    assert s["completed"] == 4 and s["converged"] == 3 and s["submitted"] == 5
+    assert s["p50_ms"] == pytest.approx(3.0) and s["p99_ms"] <= 10.0
+    assert s["solves_per_s"] == pytest.approx(4 / (0.3 + 0.002))
+    assert s["mean_iters"] == 11.5 and s["max_residual"] == pytest.approx(4e-11)
