@@ -258,7 +258,7 @@ def served_bench(seconds=(3.0, 3.0, 1.0), base_requests=200):
     for lam, dur in zip((100.0, 1000.0, 10000.0), seconds):
         reqs = requests_from_trace(g.served_requests(rate=lam, duration=dur, seed=5))
         srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True)
-        s = srv.run_open_loop(reqs, max_wall=dur + 120)
+        s = srv.run_open_loop(reqs, max_wall=dur + 600)
         out[f"lambda_{int(lam)}"] = {
             "duration_s": dur, "submitted": s["submitted"], "completed": s["completed"],
             "converged": s["converged"], "solves_per_s": round(s["solves_per_s"], 1),
@@ -316,6 +316,9 @@ def main():
                     help="which solver forms to measure (comma list)")
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="duration of the lambda = 100 and 1k served streams (60 = full)")
+    ap.add_argument("--served-10k-seconds", type=float, default=1.0,
+                    help="duration of the lambda = 10k served stream (60 = full; it then "
+                         "drains for a few minutes)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the slab-share and Zipf SpMV measurements")
     ap.add_argument("--cg-solves", type=int, default=5)
@@ -383,7 +386,7 @@ def main():
         out["cg_batched_4096"] = batched_bench(ks)
     if "served" in solvers:
         log("bench: served CG streams")
-        out["cg_served"] = served_bench((args.served_seconds,) * 2 + (1.0,))
+        out["cg_served"] = served_bench((args.served_seconds,) * 2 + (args.served_10k_seconds,))
     print(json.dumps(out), flush=True)
 
 
