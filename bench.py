@@ -258,7 +258,7 @@ def served_bench(seconds=(3.0, 3.0, 1.0), base_requests=200):
     for lam, dur in zip((100.0, 1000.0, 10000.0), seconds):
         reqs = requests_from_trace(g.served_requests(rate=lam, duration=dur, seed=5))
         srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True)
-        s = srv.run_open_loop(reqs, max_wall=dur + 600)
+        s = srv.run_open_loop(reqs, max_wall=dur + 600, progress_every=20.0)
         out[f"lambda_{int(lam)}"] = {
             "duration_s": dur, "submitted": s["submitted"], "completed": s["completed"],
             "converged": s["converged"], "solves_per_s": round(s["solves_per_s"], 1),

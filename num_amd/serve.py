@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import collections
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -162,17 +163,25 @@ class CGServer:
             self._collect()
 
     def run_open_loop(self, requests, keep_solutions: bool = False,
-                      max_wall: float | None = None) -> dict:
+                      max_wall: float | None = None,
+                      progress_every: float | None = None) -> dict:
         """Replay ``requests`` (sorted by arrival time) open-loop; returns the
         latency / throughput summary.  Requests not finished when
-        ``max_wall`` seconds have passed are reported as unfinished."""
+        ``max_wall`` seconds have passed are reported as unfinished.  With
+        ``progress_every`` a one-line status goes to stderr that often."""
         self.keep_solutions = keep_solutions
         pending = collections.deque(sorted(requests, key=lambda r: r.t_arrival))
         self.clock0 = time.perf_counter()
+        next_report = progress_every
         while pending or self.queue or self.running:
             t = self.now()
             if max_wall is not None and t > max_wall:
                 break
+            if next_report is not None and t >= next_report:
+                next_report = t + progress_every
+                print(f"serve: t={t:.0f}s done={len(self.done)} queued={len(self.queue)} "
+                      f"running={len(self.running)} pending={len(pending)}",
+                      file=sys.stderr, flush=True)
             while pending and pending[0].t_arrival <= t:
                 self.submit(pending.popleft())
             if not self.queue and not self.running:
