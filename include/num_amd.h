@@ -147,6 +147,9 @@ int num_dot(int64_t n, const double* d_x, const double* d_y, double* d_out, void
 int num_norm2(int64_t n, const double* d_x, double* d_out, void* d_work, void* stream);
 
 /* ---- CG (src/cg_solver.hpp:133-196, cg_solve_local) ---------------------- */
+/* Workspace bytes for num_cg_solve; every workspace handed to the CG and
+ * engine entry points must be 256-byte aligned (hipMalloc and torch
+ * allocations are). */
 int64_t num_cg_workspace_bytes(const num_csr* A);
 /* Solves A x = b; x holds x0 on entry.  Same operation order, ||b||=0 -> 1
  * rule, strict < test and iteration count as the reference; scalars stay on

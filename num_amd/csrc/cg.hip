@@ -8,9 +8,11 @@
 // asynchronously (two chunks in flight) instead of reading back two scalars
 // per iteration like the reference.  Global dot products finish in a
 // one-workgroup kernel that sums the per-workgroup partials in index order
-// (deterministic; a grid-wide "last workgroup finishes" reduction needs a
-// device-scope release per workgroup, an L2 write-back on this GPU, and
-// measured 5x slower).
+// (deterministic).  A grid-wide "last workgroup finishes" reduction instead
+// measured slower on this GPU: with a device-scope release per workgroup (an
+// L2 write-back) 5x, and with agent-scope atomic stores and ticket counters
+// (no write-back) still 25% per slot, from the per-workgroup atomic round
+// trips.
 //
 // FP64 CG slot (the reference's arithmetic in its order):
 //   KA  p_new = 1*r + beta*p_old is recomputed inside the SpMV gather (the
@@ -32,6 +34,11 @@
 // the FP32 residual from it and restarting x32 at 0), KB tests the FP64 true
 // residual against tol and recomputes beta from the new residual, keeping the
 // search direction (the Krylov space survives the refinement).
+// When every row holds <= 32 entries and every column - row offset fits in
+// int16 (num_csr_attach_rel16), KA is mix_pair_kernel: one workgroup stages
+// two schedule tiles of FP32 values and int16 offsets at once (6 bytes per
+// entry instead of 8, twice the data behind one latency chain) and walks a
+// refinement's FP64 SpMV as the two tiles in turn.
 #include <algorithm>
 #include <cmath>
 
@@ -518,8 +525,8 @@ __global__ __launch_bounds__(kSpmvBS) void mix_pair_kernel(
     const int16_t* __restrict__ rel, const TileEntry* __restrict__ sched, int ntiles,
     int npairs, int chunk, MixArgs a)
 {
-    __shared__ double lv[kSpmvBS * NPT + 2];   // FP32 pairs: 2 * NPT * BS + 4 floats
-    __shared__ int32_t lc[kSpmvBS * NPT + 2];  // FP32 pairs: 2 * NPT * BS + 4 int16
+    __shared__ double lv[kSpmvBS * NPT];   // FP32 pairs: 2 * NPT * BS floats
+    __shared__ int32_t lc[kSpmvBS * NPT];  // FP32 pairs: 2 * NPT * BS int16
     __shared__ double red[2 * kSpmvBS / kWave];
     const int p = chunk_tile(blockIdx.x, npairs, chunk);
     if (p < 0) return;
@@ -773,6 +780,8 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol, i
     const CgLayout L = cg_layout(A);
     if (!d_work || work_bytes < (int64_t)L.total)
         return fail(NUM_EINVAL, "workspace too small: need %lld bytes", (long long)L.total);
+    if (((uintptr_t)d_work & 255) != 0)
+        return fail(NUM_EINVAL, "workspace must be 256-byte aligned");
     hipStream_t st = (hipStream_t)stream;
     char* w = (char*)d_work;
     double* r = (double*)(w + L.off_r);
@@ -839,6 +848,8 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     const MixLayout L = mix_layout(A);
     if (!d_work || work_bytes < (int64_t)L.total)
         return fail(NUM_EINVAL, "workspace too small: need %lld bytes", (long long)L.total);
+    if (((uintptr_t)d_work & 255) != 0)
+        return fail(NUM_EINVAL, "workspace must be 256-byte aligned");
     hipStream_t st = (hipStream_t)stream;
     char* w = (char*)d_work;
     MixArgs a;

@@ -200,8 +200,8 @@ __global__ __launch_bounds__(kSpmvBS) void eng_ka(const BTile* __restrict__ bts,
                                                   double* __restrict__ pa,
                                                   double* __restrict__ pb)
 {
-    __shared__ double lv[kSpmvBS * kEngNpt + 2];   // FP32 pairs: 2 * NPT * BS + 4 floats
-    __shared__ int32_t lc[kSpmvBS * kEngNpt + 2];  // FP32 pairs: 2 * NPT * BS + 4 int16
+    __shared__ double lv[kSpmvBS * kEngNpt];   // FP32 pairs: 2 * NPT * BS floats
+    __shared__ int32_t lc[kSpmvBS * kEngNpt];  // FP32 pairs: 2 * NPT * BS int16
     __shared__ double red[2 * kSpmvBS / kWave];
     const int b = chunk_tile(blockIdx.x, nbt, chunk);
     if (b < 0) return;
@@ -1011,6 +1011,8 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     const int64_t need = eng_work_bytes(A->nrows, mode);
     if (!d_work || work_bytes < need)
         return fail(NUM_EINVAL, "workspace too small: need %lld bytes", (long long)need);
+    if (((uintptr_t)d_work & 255) != 0)
+        return fail(NUM_EINVAL, "workspace must be 256-byte aligned");
     rc = check_device(A);
     if (rc) return rc;
     int32_t s = -1;

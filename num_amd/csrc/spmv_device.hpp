@@ -216,12 +216,9 @@ __device__ __forceinline__ void spmv_tile_run(
     }
 }
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef int16_t i16x4 __attribute__((ext_vector_type(4)));
-
 // FP32 walk over a group of consecutive tiles whose rows all hold <= kShortRow
 // entries, with column ids stored as int16 offsets from their row (half the
-// index bytes of int32).  LDS: lv / lc hold RPT * NPT * BS + 4 entries.  Thread t owns rows t, t + BS, ..., t + (RPT-1) BS of
+// index bytes of int32).  LDS: lv / lc hold RPT * NPT * BS entries.  Thread t owns rows t, t + BS, ..., t + (RPT-1) BS of
 // the group; the group's entries (<= RPT * NPT * BS) are staged at once, so
 // one workgroup moves RPT tiles' worth of data behind one latency chain.
 // Each row is summed in index order from 0.0 (the row loop's arithmetic).
@@ -243,32 +240,24 @@ __device__ __forceinline__ void spmv_rel16_run(
         rb[k] = rowptr[r0 + lr];
         re[k] = rowptr[r0 + lr + (t + k * BS < nr ? 1 : 0)];
     }
-    // stage in 16-byte (float4) / 8-byte (short4) groups from the 4-aligned
-    // entry ja <= j0; LDS slot k holds entry ja + k.  Groups are clamped to
-    // the last one holding tile entries, so a read goes at most 3 entries past
-    // j1 (the arrays carry that padding).
-    const int ja = j0 & ~3;
-    const int cnt = j1 - ja;
+    // stage: each thread issues all M value and offset loads (clamped, no
+    // per-element branches) before parking them in LDS
     if (nz > 0) {
-        constexpr int G = (M * BS + 3 + 4 * BS - 1) / (4 * BS);
-        const int gl = (cnt - 1) >> 2;
-        f32x4 v[G];
-        i16x4 c[G];
+        const int last = j1 - 1;
+        float v[M];
+        int16_t c[M];
 #pragma unroll
-        for (int s = 0; s < G; ++s) {
-            const int g = min(t + s * BS, gl);
-            v[s] = ld_mat<NT>(reinterpret_cast<const f32x4*>(vals + ja) + g);
-            c[s] = ld_mat<NT>(reinterpret_cast<const i16x4*>(rel + ja) + g);
+        for (int s = 0; s < M; ++s) {
+            const int j = min(j0 + t + s * BS, last);
+            v[s] = ld_mat<NT>(vals + j);
+            c[s] = ld_mat<NT>(rel + j);
         }
 #pragma unroll
-        for (int s = 0; s < G; ++s) {
-            const int k = 4 * (t + s * BS);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (k + i < cnt) {
-                    lv[k + i] = v[s][i];
-                    lc[k + i] = c[s][i];
-                }
+        for (int s = 0; s < M; ++s) {
+            const int k = t + s * BS;
+            if (k < nz) {
+                lv[k] = v[s];
+                lc[k] = c[s];
             }
         }
     }
@@ -280,7 +269,7 @@ __device__ __forceinline__ void spmv_rel16_run(
         const int lr = t + k * BS;
         if (lr >= nr) continue;
         const int row = r0 + lr;
-        const int b = rb[k] - ja, e = re[k] - ja;
+        const int b = rb[k] - j0, e = re[k] - j0;
         float acc = 0.0f;
         int q = b;
         for (; q + 7 < e; q += 8) {

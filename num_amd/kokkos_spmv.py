@@ -177,14 +177,10 @@ class CrsMatrix:
     def ensure_f32(self) -> None:
         """Attach an FP32 copy of the values (mixed-precision inner solve)."""
         if self._values32 is None:
-            # FP32 values and int16 offsets carry 16 entries of padding: the
-            # FP32 tile walk reads 4-entry groups, up to 3 entries past the end
-            v32 = torch.zeros(self.nnz + 16, dtype=torch.float32, device=self._values.device)
-            v32[: self.nnz] = self._values
-            self._values32 = v32
+            self._values32 = self._values.to(torch.float32)
             call("num_csr_attach_f32", self._h, _ptr(self._values32))
             # int16 column offsets for the FP32 walk where they fit
-            rel = torch.zeros(self.nnz + 16, dtype=torch.int16, device=self._values.device)
+            rel = torch.empty(max(1, self.nnz), dtype=torch.int16, device=self._values.device)
             ok = C.c_int32()
             call("num_csr_attach_rel16", self._h, _ptr(rel), _stream(), C.byref(ok))
             self._rel16 = rel if ok.value else None

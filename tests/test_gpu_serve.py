@@ -133,7 +133,8 @@ def test_pipelined_launch_equals_step(ks, mixed):
     """launch/poll (steps queued while the host works) and the synchronous
     step give bit-identical solutions and iteration counts."""
     from num_amd.batched import CGEngine
-    grids = [((30, 20), 0.01), ((12, 11, 10), 0.3), ((500,), 0.001), ((7, 64), 1.0)]
+    grids = [((30, 20), 0.01), ((12, 11, 10), 0.3), ((500,), 0.001), ((7, 64), 1.0),
+             ((1, 1), 0.5), ((2,), 0.1), ((3, 1), 0.2)]  # tiny: fewer than 4 entries
     out = []
     for pipelined in (False, True):
         eng = CGEngine(max_systems=8)
