@@ -210,6 +210,10 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
                            double shift, uint64_t seed, double tol, int32_t max_iter,
                            int32_t mode, double delta, void* d_buf, int64_t buf_bytes,
                            void* stream, int32_t* slot_out);
+/* on != 0: capture a step's kernels into a hipGraph (on a private stream)
+ * and replay it on the caller's stream; the graph is re-captured when the
+ * live set or the iteration count changes.  Results are identical. */
+int num_engine_set_graph(num_engine* E, int32_t on);
 /* Queue `iters` iterations of every live system and return; at most two
  * steps are in flight (with two queued, the older is waited for first). */
 int num_engine_launch(num_engine* E, int32_t iters, void* stream);

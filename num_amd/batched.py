@@ -65,13 +65,17 @@ def _grid(dims):
 
 
 class CGEngine:
-    """A pool of up to ``max_systems`` concurrent CG solves."""
+    """A pool of up to ``max_systems`` concurrent CG solves.  ``graph``:
+    replay each step's kernels from a captured hipGraph."""
 
-    def __init__(self, max_systems: int = 4096, nt64: bool = True, nt32: bool = True):
+    def __init__(self, max_systems: int = 4096, nt64: bool = True, nt32: bool = True,
+                 graph: bool = False):
         h = C.c_void_p()
         call("num_engine_create", int(max_systems), int(bool(nt64)), int(bool(nt32)),
              C.byref(h))
         self._h = h
+        if graph:
+            call("num_engine_set_graph", h, 1)
         self._sys = {}
         self.max_systems = int(max_systems)
 

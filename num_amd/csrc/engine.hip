@@ -17,7 +17,9 @@
 // A step (k iterations) ends with two more launches: the mixed form's
 // finished systems fold x64 + x32 into the caller's x, and one record per
 // live system (phase, iterations, residual) is packed and copied to pinned
-// host memory behind an event.  Steps are pipelined: num_engine_launch
+// host memory behind an event.  Optionally a step's kernels are captured
+// once into a hipGraph and replayed (num_engine_set_graph), re-captured
+// whenever the live set changes.  Steps are pipelined: num_engine_launch
 // returns as soon as the step is queued, and the host learns which systems
 // finished when the step's event completes, while the next step runs.
 //
@@ -723,6 +725,13 @@ struct num_engine {
     EngRec* h_rec[2] = {nullptr, nullptr};
     hipEvent_t rec_ev[2] = {nullptr, nullptr};
     std::deque<Flight> flights;
+    // optional hipGraph of one step (captured on a private stream, replayed
+    // on the caller's; re-captured when the launch lists or iters change)
+    bool use_graph = false;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int32_t g_iters = -1;
+    bool g_valid = false;
 };
 
 namespace numamd {
@@ -853,6 +862,43 @@ static int eng_rebuild(num_engine* E, hipStream_t st)
     E->nact = na;
     E->act_host.assign(act, act + na);
     E->dirty = false;
+    E->g_valid = false;  // grids and list contents changed
+    return NUM_OK;
+}
+
+// The kernels of one step: `iters` engine iterations, then the fold and
+// the record pack (device work only: capturable into a hipGraph).
+static int eng_enqueue_step(num_engine* E, int32_t iters, hipStream_t st)
+{
+    const int32_t* act = eng_act(E);
+    const int32_t* toff = eng_toff(E);
+    const int32_t* coff = eng_coff(E);
+    const int chunk = tile_chunk(E->nbt);
+    const int grid_a = chunk_grid(E->nbt, chunk);
+    for (int32_t k = 0; k < iters; ++k) {
+        if (E->nt64) {
+            if (E->nt32)
+                eng_ka<true, true><<<grid_a, kSpmvBS, 0, st>>>(E->d_bt, E->nbt, chunk, E->d_desc,
+                                                               E->d_state, E->d_pa, E->d_pb);
+            else
+                eng_ka<true, false><<<grid_a, kSpmvBS, 0, st>>>(
+                    E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
+        } else {
+            if (E->nt32)
+                eng_ka<false, true><<<grid_a, kSpmvBS, 0, st>>>(
+                    E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
+            else
+                eng_ka<false, false><<<grid_a, kSpmvBS, 0, st>>>(
+                    E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
+        }
+        eng_kb<<<E->nact, kSpmvBS, 0, st>>>(act, toff, E->d_desc, E->d_state, E->d_pa, E->d_pb);
+        if (E->nch > 0)
+            eng_kc<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state, E->d_pc);
+        eng_kd<<<E->nact, kSpmvBS, 0, st>>>(act, coff, E->d_desc, E->d_state, E->d_pc);
+    }
+    if (E->nch > 0) eng_fold<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state);
+    eng_pack<<<(E->nact + 255) / 256, 256, 0, st>>>(act, E->nact, E->d_state, E->d_rec);
+    NUM_LAUNCH_CHECK();
     return NUM_OK;
 }
 
@@ -965,11 +1011,23 @@ int num_engine_create(int32_t max_systems, int32_t nt64, int32_t nt32, num_engin
     return NUM_OK;
 }
 
+int num_engine_set_graph(num_engine* E, int32_t on)
+{
+    if (!E) return fail(NUM_EINVAL, "engine is NULL");
+    if (on && !E->cap_stream)
+        NUM_HIP(hipStreamCreateWithFlags(&E->cap_stream, hipStreamNonBlocking));
+    E->use_graph = on != 0;
+    E->g_valid = false;
+    return NUM_OK;
+}
+
 int num_engine_destroy(num_engine* E)
 {
     if (!E) return NUM_OK;
     // queued steps and installs still read the engine's arrays
     (void)hipDeviceSynchronize();
+    if (E->gexec) (void)hipGraphExecDestroy(E->gexec);
+    if (E->cap_stream) (void)hipStreamDestroy(E->cap_stream);
     eng_free(E);
     if (E->d_desc) (void)hipFree(E->d_desc);
     if (E->d_state) (void)hipFree(E->d_state);
@@ -1131,35 +1189,32 @@ int num_engine_launch(num_engine* E, int32_t iters, void* stream)
         if (rc) return rc;
     }
     if (E->nact == 0) return NUM_OK;
-    const int32_t* act = eng_act(E);
-    const int32_t* toff = eng_toff(E);
-    const int32_t* coff = eng_coff(E);
-    const int chunk = tile_chunk(E->nbt);
-    const int grid_a = chunk_grid(E->nbt, chunk);
-    for (int32_t k = 0; k < iters; ++k) {
-        if (E->nt64) {
-            if (E->nt32)
-                eng_ka<true, true><<<grid_a, kSpmvBS, 0, st>>>(E->d_bt, E->nbt, chunk, E->d_desc,
-                                                               E->d_state, E->d_pa, E->d_pb);
-            else
-                eng_ka<true, false><<<grid_a, kSpmvBS, 0, st>>>(
-                    E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
-        } else {
-            if (E->nt32)
-                eng_ka<false, true><<<grid_a, kSpmvBS, 0, st>>>(
-                    E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
-            else
-                eng_ka<false, false><<<grid_a, kSpmvBS, 0, st>>>(
-                    E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
+    if (E->use_graph) {
+        if (!E->g_valid || E->g_iters != iters) {
+            if (E->gexec) {
+                (void)hipGraphExecDestroy(E->gexec);
+                E->gexec = nullptr;
+            }
+            NUM_HIP(hipStreamBeginCapture(E->cap_stream, hipStreamCaptureModeThreadLocal));
+            rc = eng_enqueue_step(E, iters, E->cap_stream);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(E->cap_stream, &g);
+            if (rc) {
+                if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            NUM_HIP(e);
+            const hipError_t ei = hipGraphInstantiate(&E->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            NUM_HIP(ei);
+            E->g_valid = true;
+            E->g_iters = iters;
         }
-        eng_kb<<<E->nact, kSpmvBS, 0, st>>>(act, toff, E->d_desc, E->d_state, E->d_pa, E->d_pb);
-        if (E->nch > 0)
-            eng_kc<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state, E->d_pc);
-        eng_kd<<<E->nact, kSpmvBS, 0, st>>>(act, coff, E->d_desc, E->d_state, E->d_pc);
+        NUM_HIP(hipGraphLaunch(E->gexec, st));
+    } else {
+        rc = eng_enqueue_step(E, iters, st);
+        if (rc) return rc;
     }
-    if (E->nch > 0) eng_fold<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state);
-    eng_pack<<<(E->nact + 255) / 256, 256, 0, st>>>(act, E->nact, E->d_state, E->d_rec);
-    NUM_LAUNCH_CHECK();
     // the record buffer not held by the (at most one) step still in flight
     const int buf = (!E->flights.empty() && E->flights.front().buf == 0) ? 1 : 0;
     NUM_HIP(hipMemcpyAsync(E->h_rec[buf], E->d_rec, (size_t)E->nact * sizeof(EngRec),

@@ -136,3 +136,35 @@ def test_edge_cases_per_system(ks):
     assert r3["converged"]
     true = np.linalg.norm(b3 - A3.to_scipy() @ x3.cpu().numpy()) / np.linalg.norm(b3)
     assert true < 2e-10
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graph_replay_equals_eager(ks, mixed):
+    """Steps replayed from a captured hipGraph give the eager engine's
+    iterates after EVERY step (FP64 x is the live iterate), with eager torch
+    work and synchronisations between replays, admissions into the running
+    batch (re-capture) and systems finishing (re-capture)."""
+    from num_amd.batched import CGEngine
+    grids = [((40, 30), 0.01), ((9, 8, 7), 0.2), ((300,), 0.05), ((64, 16), 0.5)]
+    late = [((25, 25), 0.02), ((5, 6, 7), 0.3)]
+    engines = [CGEngine(max_systems=8), CGEngine(max_systems=8, graph=True)]
+    slots = [[e.add_stencil(d, sh, 20 + i, max_iter=4000, mixed=mixed)
+              for i, (d, sh) in enumerate(grids)] for e in engines]
+    for step in range(400):
+        live = [e.step(6) for e in engines]
+        assert live[0] == live[1]
+        junk = torch.randn(1 << 16, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        for s0, s1 in zip(slots[0], slots[1]):
+            assert torch.equal(engines[0].solution(s0), engines[1].solution(s1)), (step, s0)
+        if step == 3:
+            for k, e in enumerate(engines):
+                slots[k] += [e.add_stencil(d, sh, 90 + i, max_iter=4000, mixed=mixed)
+                             for i, (d, sh) in enumerate(late)]
+        if live[0] == 0 and step > 3:
+            break
+    assert live == [0, 0]
+    for s0, s1 in zip(slots[0], slots[1]):
+        r0, r1 = engines[0].result(s0), engines[1].result(s1)
+        assert r0 == r1 and r0["converged"]
+    del junk
