@@ -84,9 +84,17 @@ def _write_csv(prefix: str, rows: list) -> Path:
     return path
 
 
-def run_spmv(args) -> list:
+def _matrix(name: str, N: int):
+    """get_matrix, plus 'zipf': the irregular power-law rows of the Zipf(2.0)
+    workload (n = N, rows clipped to [1, 4096], seed 2)."""
     from . import generators as g
-    A = g.get_matrix(args.matrix, args.N)
+    if name == "zipf":
+        return g.zipf_matrix(N, 2.0, 4096, seed=2)
+    return g.get_matrix(name, N)
+
+
+def run_spmv(args) -> list:
+    A = _matrix(args.matrix, args.N)
     N, nnz = A.nrows, A.nnz
     x = np.random.default_rng(42).standard_normal(A.ncols)
     print(f"\nSpMV Benchmark - {args.matrix}  N={N:,}  nnz={nnz:,}  nnz/row={nnz / N:.1f}")
@@ -195,7 +203,7 @@ def main(argv=None) -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     sp_ = sub.add_parser("spmv", help="SpMV benchmark (python/benchmark_spmv.py)")
     sp_.add_argument("--N", type=int, default=100_000)
-    sp_.add_argument("--matrix", default="laplacian_3d", choices=list(MATRIX_FAMILIES))
+    sp_.add_argument("--matrix", default="laplacian_3d", choices=list(MATRIX_FAMILIES) + ["zipf"])
     sp_.add_argument("--batch", type=int, default=0)
     sp_.add_argument("--repeat", type=int, default=100)
     sp_.add_argument("--backend", nargs="+", default=["scipy", "num_amd", "num_amd_custom"])
