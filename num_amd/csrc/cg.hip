@@ -2,38 +2,45 @@
 // FP32-inner / FP64-refined form, with fused kernels and device-resident
 // scalars.
 //
-// Both solvers run as a uniform sequence of 4-launch slots; every scalar
+// Both solvers run as a uniform sequence of 2-launch slots; every scalar
 // (alpha, beta, r.r, iteration count, stop flags) lives in a device state
 // record, so the host only queues slots and polls the `done` flag
 // asynchronously (two chunks in flight) instead of reading back two scalars
-// per iteration like the reference.  Global dot products finish in a
-// one-workgroup kernel that sums the per-workgroup partials in index order
-// (deterministic).  A grid-wide "last workgroup finishes" reduction instead
-// measured slower on this GPU: with a device-scope release per workgroup (an
-// L2 write-back) 5x, and with agent-scope atomic stores and ticket counters
-// (no write-back) still 25% per slot, from the per-workgroup atomic round
-// trips.
+// per iteration like the reference.  No launch is spent on a bare global
+// sum: alpha's p.Ap is summed by EVERY workgroup of the update kernel from
+// the SpMV kernel's per-tile partials (one fixed order, so all hold the same
+// bits), and the update kernel's own r.r is finished by its last workgroup
+// (common.hpp: publish / take_ticket), which also applies the convergence
+// test and beta.  Separate one-workgroup finalize kernels cost ~4.5 us each
+// per iteration on this GPU (two dependent memory round trips plus a
+// launch); folding the r.r into every workgroup of the next SpMV kernel
+// instead costs that kernel ~8% on 192^3 (a reduction and barrier per tile);
+// a last-workgroup ticket in the SpMV kernel itself (27k tickets per launch
+// on 192^3) measured 25% slower.  The update kernel has at most 256
+// workgroups, so its tickets are cheap.
 //
 // FP64 CG slot (the reference's arithmetic in its order):
 //   KA  p_new = 1*r + beta*p_old is recomputed inside the SpMV gather (the
 //       reference's axpy2, cg_solver.hpp:190, fused into the next SpMV), the
 //       tile writes its own rows of p_new to the other p buffer, Ap = A p_new,
 //       per-tile partials of p_new_i * Ap_i
-//   KB  pAp = sum of partials (fixed order); |pAp| < 1e-300 guard; alpha = rr/pAp
-//   KC  x = alpha*p_new + x ; r = (-alpha)*Ap + r ; partials of r_i * r_i
-//   KD  rrn = sum; rel = sqrt(rrn)/||b||; stop if rel < tol (strict), else
+//   KC  pAp = sum of KA's partials (fixed order, every workgroup); |pAp| <
+//       1e-300 guard; alpha = rr/pAp; x = alpha*p_new + x ;
+//       r = (-alpha)*Ap + r ; partials of r_i * r_i; the last workgroup:
+//       rrn = sum; rel = sqrt(rrn)/||b||; stop if rel < tol (strict), else
 //       beta = rrn/rr, rr = rrn; flip the p buffers
 //
-// Mixed slot (reliable updates): the same four kernels run the CG step in
+// Mixed slot (reliable updates): the same two kernels run the CG step in
 // FP32 (FP32 matrix copy and vectors, dot products accumulated in FP64,
 // scalars in FP64).  The FP32 solution correction x32 accumulates since the
 // last refinement.  When the FP32 recursive residual has dropped by a factor
-// delta since the last refinement, or looks converged, KD raises `ru` and the
-// next slot refines in FP64 instead of stepping: KA computes
-// x64' = x64 + x32 and the true residual r = b - A x64' in FP64 (re-seeding
-// the FP32 residual from it and restarting x32 at 0), KB tests the FP64 true
-// residual against tol and recomputes beta from the new residual, keeping the
-// search direction (the Krylov space survives the refinement).
+// delta since the last refinement, or looks converged, KC's last workgroup
+// raises `ru` and the next slot refines in FP64 instead of stepping: KA
+// computes x64' = x64 + x32 and the true residual r = b - A x64' in FP64
+// (re-seeding the FP32 residual from it and restarting x32 at 0), KC's last
+// workgroup tests the FP64 true residual against tol and recomputes beta from
+// the new residual, keeping the search direction (the Krylov space survives
+// the refinement).
 // When every row holds <= 32 entries and every column - row offset fits in
 // int16 (num_csr_attach_rel16), KA is mix_pair_kernel: one workgroup stages
 // two schedule tiles of FP32 values and int16 offsets at once (6 bytes per
@@ -130,8 +137,8 @@ struct EpiPAp {
 
 __global__ __launch_bounds__(kFin) void cg_init_finalize(const double* __restrict__ pa,
                                                          const double* __restrict__ pb, int np,
-                                                         CgState* st, double tol,
-                                                         int32_t max_iter)
+                                                         CgState* st, uint32_t* ticket,
+                                                         double tol, int32_t max_iter)
 {
     __shared__ double red[kFin / kWave];
     const double a = sum_partials_block<kFin>(pa, np, red);
@@ -139,6 +146,7 @@ __global__ __launch_bounds__(kFin) void cg_init_finalize(const double* __restric
     if (threadIdx.x == 0) {
         double bn = sqrt(b);
         if (bn == 0.0) bn = 1.0;  // src/cg_solver.hpp:153
+        *ticket = 0u;
         st->tol = tol;
         st->max_iter = max_iter;
         st->bnorm = bn;
@@ -159,77 +167,134 @@ __global__ __launch_bounds__(kFin) void cg_init_finalize(const double* __restric
     }
 }
 
-__global__ __launch_bounds__(kFin) void cg_alpha(const double* __restrict__ pa, int np,
-                                                 CgState* st)
+// Grid of the update kernels: at most 256 workgroups of kFin threads (their
+// tickets and the last workgroup's sum stay cheap).
+constexpr int kMaxUpdateGroups = 256;
+static int update_grid(int64_t n)
+{
+    int64_t g = (n + (int64_t)kFin * 4 - 1) / ((int64_t)kFin * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, kMaxUpdateGroups));
+}
+
+// KC: alpha from KA's partials (every workgroup, one fixed order), then
+// x = alpha*p_new + x, r = (-alpha)*Ap + r and r.r partials; the last
+// workgroup folds the r.r into the state.  A thread's first two elements are
+// loaded together with the partials, so alpha's sum overlaps their latency.
+__global__ __launch_bounds__(kFin) void cg_kc(int64_t n, const double* __restrict__ p0,
+                                              const double* __restrict__ p1,
+                                              const double* __restrict__ Ap, double* x,
+                                              double* r, CgState* st,
+                                              const double* __restrict__ pa, int npa,
+                                              double* pb, uint32_t* ticket)
 {
     __shared__ double red[kFin / kWave];
-    if (st->done) return;
-    const double pAp = sum_partials_block<kFin>(pa, np, red);
-    if (threadIdx.x == 0) {
-        if (!(fabs(pAp) >= 1e-300)) {  // src/cg_solver.hpp:84-85 (NaN included)
+    __shared__ double s_alpha;
+    __shared__ int s_brk, s_last;
+    if (st->done) return;  // read by every workgroup before any ticket is drawn
+    const int t = threadIdx.x;
+    // the fields the last workgroup's fold needs, read up front
+    const double rr = st->rr;
+    const double bnorm = st->bnorm, tol = st->tol;
+    const int it = st->it + 1, max_iter = st->max_iter, psel = st->psel;
+    const double* __restrict__ p = psel ? p0 : p1;  // p_new was written to !psel
+    const int64_t stride = (int64_t)gridDim.x * kFin;
+    int64_t i = (int64_t)blockIdx.x * kFin + t;
+    const int64_t j = i + stride;
+    double pi = 0.0, ai = 0.0, xi = 0.0, ri = 0.0, pj = 0.0, aj = 0.0, xj = 0.0, rj = 0.0;
+    if (i < n) {
+        pi = p[i];
+        ai = Ap[i];
+        xi = x[i];
+        ri = r[i];
+    }
+    if (j < n) {
+        pj = p[j];
+        aj = Ap[j];
+        xj = x[j];
+        rj = r[j];
+    }
+    const double pAp = sum_partials_block<kFin, 32>(pa, npa, red);
+    if (t == 0) {
+        const int brk = !(fabs(pAp) >= 1e-300);  // src/cg_solver.hpp:84-85 (NaN included)
+        s_brk = brk;
+        s_alpha = brk ? 0.0 : rr / pAp;
+    }
+    __syncthreads();
+    const bool brk = s_brk != 0;
+    const double alpha = s_alpha;
+    if (!brk) {
+        const double nalpha = -alpha;
+        double acc = 0.0;
+        if (i < n) {
+            x[i] = alpha * pi + xi;
+            const double rn = nalpha * ai + ri;
+            r[i] = rn;
+            acc += rn * rn;
+        }
+        if (j < n) {
+            x[j] = alpha * pj + xj;
+            const double rn = nalpha * aj + rj;
+            r[j] = rn;
+            acc += rn * rn;
+        }
+        for (i += 2 * stride; i + stride < n; i += 2 * stride) {
+            const int64_t k = i + stride;
+            const double pv = p[i], pw = p[k], av = Ap[i], aw = Ap[k];
+            const double xv = x[i], xw = x[k], rv = r[i], rw = r[k];
+            x[i] = alpha * pv + xv;
+            x[k] = alpha * pw + xw;
+            const double rnv = nalpha * av + rv;
+            const double rnw = nalpha * aw + rw;
+            r[i] = rnv;
+            r[k] = rnw;
+            acc += rnv * rnv;
+            acc += rnw * rnw;
+        }
+        if (i < n) {
+            x[i] = alpha * p[i] + x[i];
+            const double rn = nalpha * Ap[i] + r[i];
+            r[i] = rn;
+            acc += rn * rn;
+        }
+        const double tot = block_sum<kFin>(acc, red);
+        if (t == 0) publish(pb + blockIdx.x, tot);
+    }
+    if (!take_ticket(ticket, &s_last)) return;
+    // the last workgroup: every other one has read the state
+    if (brk) {
+        if (t == 0) {
             st->breakdown = 1;
             st->done = 1;
-        } else {
-            st->alpha = st->rr / pAp;
         }
+        return;
     }
-}
-
-__global__ __launch_bounds__(kRedBlock) void cg_update(int64_t n, const double* __restrict__ p0,
-                                                       const double* __restrict__ p1,
-                                                       const double* __restrict__ Ap, double* x,
-                                                       double* r, const CgState* st,
-                                                       double* __restrict__ partial)
-{
-    __shared__ double red[kRedBlock / kWave];
-    if (st->done) return;
-    const double alpha = st->alpha;
-    const double nalpha = -alpha;
-    const double* __restrict__ p = st->psel ? p0 : p1;  // p_new was written to !psel
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        x[i] = alpha * p[i] + x[i];
-        const double ri = nalpha * Ap[i] + r[i];
-        r[i] = ri;
-        acc += ri * ri;
-    }
-    const double s = block_sum<kRedBlock>(acc, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-__global__ __launch_bounds__(kFin) void cg_beta(const double* __restrict__ pb, int np,
-                                                CgState* st)
-{
-    __shared__ double red[kFin / kWave];
-    if (st->done) return;
-    const double rrn = sum_partials_block<kFin>(pb, np, red);
-    if (threadIdx.x == 0) {
-        const int it = st->it + 1;
+    const double rrn = sum_published_block<kFin>(pb, gridDim.x, red);
+    if (t == 0) {
+        st->alpha = alpha;
         st->it = it;
-        st->psel ^= 1;
-        const double rel = sqrt(rrn) / st->bnorm;
+        st->psel = psel ^ 1;
+        const double rel = sqrt(rrn) / bnorm;
         st->rel = rel;
-        if (rel < st->tol) {  // strict <, src/cg_solver.hpp:180-183
+        if (rel < tol) {  // strict <, src/cg_solver.hpp:180-183
             st->converged = 1;
             st->done = 1;
         } else {
-            st->beta = rrn / st->rr;
+            st->beta = rrn / rr;
             st->rr = rrn;
-            if (it >= st->max_iter) st->done = 1;
+            if (it >= max_iter) st->done = 1;
         }
     }
 }
 
 struct CgLayout {
-    size_t off_r, off_p0, off_p1, off_Ap, off_pa, off_pb, off_state, total;
+    size_t off_r, off_p0, off_p1, off_Ap, off_pa, off_pb, off_state, off_ticket, total;
 };
 
 static CgLayout cg_layout(const num_csr* A)
 {
     CgLayout L;
     const size_t n = (size_t)A->nrows;
-    const int np = std::max(A->sched.nblocks, dot_grid(A->nrows));
+    const int np = std::max(A->sched.nblocks, kMaxUpdateGroups);
     size_t o = 0;
     L.off_r = o;  o += align256(n * 8);
     L.off_p0 = o; o += align256(n * 8);
@@ -238,6 +303,7 @@ static CgLayout cg_layout(const num_csr* A)
     L.off_pa = o; o += align256((size_t)np * 8);
     L.off_pb = o; o += align256((size_t)np * 8);
     L.off_state = o; o += 256;
+    L.off_ticket = o; o += 256;
     L.total = o;
     return L;
 }
@@ -322,12 +388,11 @@ struct MixState {
     int32_t max_iter;
     int32_t done, converged, breakdown;
     int32_t ru;       // 1: this slot refines in FP64 instead of stepping
-    int32_t post_ru;  // 1: this slot refined (KC / KD skip)
     int32_t nref;     // refinements done (FP64 residual evaluations)
     int32_t psel;     // p buffer holding the current search direction
     int32_t xsel;     // x64 / x32 buffer pair holding the current iterate
     int32_t final_;   // 1: the pending refinement ends the solve
-    int32_t pad_;
+    int32_t pad_[2];
 };
 static_assert(sizeof(MixState) <= 256, "state fits its slot");
 
@@ -341,9 +406,10 @@ struct MixArgs {
     float* p0;
     float* p1;
     float* q;
-    double* pa;
-    double* pb;
+    double* pa;     // SpMV kernel partials: p.q, or the refinement's FP64 r.r
+    double* pb;     // the refinement's FP32 r32.r32, then the update kernel's r32.r32
     MixState* st;
+    uint32_t* ticket;
 };
 
 // FP64 gather of the refined iterate x64 + x32
@@ -399,41 +465,30 @@ struct EpiPq {
     }
 };
 
-// The scalar step after KA (thread 0): the refinement's FP64 convergence test
-// and new beta (ru), else alpha = rr / p.q.
-__device__ __forceinline__ void mix_after_ka(bool ru, double sa, double sb, MixState* st)
+// The refinement's FP64 convergence test and new beta (KC's last workgroup
+// after a refinement slot).
+__device__ __forceinline__ void mix_after_refine(double rr64, double rr32, MixState* st)
 {
-    if (ru) {
-        const double rr64 = sa, rr32 = sb;
-        const double rel = sqrt(rr64) / st->bnorm;
-        st->rel = rel;
-        st->xsel ^= 1;
-        st->nref += 1;
-        st->ru = 0;
-        st->post_ru = 1;
-        if (rel < st->tol) {
-            st->converged = 1;
-            st->done = 1;
-        } else if (st->final_ || !(rel == rel)) {
-            st->done = 1;
-        } else {
-            st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
-            st->rr = rr32;
-            st->rr_ref = rr32;
-        }
-        return;
-    }
-    const double pq = sa;
-    if (!(fabs(pq) >= 1e-300)) {
-        st->breakdown = 1;
+    const double rel = sqrt(rr64) / st->bnorm;
+    st->rel = rel;
+    st->xsel ^= 1;
+    st->nref += 1;
+    st->ru = 0;
+    if (rel < st->tol) {
+        st->converged = 1;
+        st->done = 1;
+    } else if (st->final_ || !(rel == rel)) {
         st->done = 1;
     } else {
-        st->alpha = st->rr / pq;
+        st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
+        st->rr = rr32;
+        st->rr_ref = rr32;
     }
 }
 
-// The scalar step after KC (thread 0): beta, or schedule a refinement.
-__device__ __forceinline__ void mix_after_kc(double rrn, MixState* st)
+// The scalar step after an FP32 update (KC's last workgroup): beta, or
+// schedule a refinement.
+__device__ __forceinline__ void mix_after_step(double rrn, MixState* st)
 {
     const int it = st->it + 1;
     st->it = it;
@@ -450,36 +505,19 @@ __device__ __forceinline__ void mix_after_kc(double rrn, MixState* st)
     }
 }
 
-// KB of the mixed slot: alpha, or the refinement's convergence test and beta.
-__global__ __launch_bounds__(kFin) void mix_kb(const double* __restrict__ pa,
-                                               const double* __restrict__ pb, int np,
-                                               MixState* st)
-{
-    __shared__ double red[kFin / kWave];
-    if (st->done) return;
-    const bool ru = st->ru != 0;
-    const double sa = sum_partials_block<kFin>(pa, np, red);
-    const double sb = ru ? sum_partials_block<kFin>(pb, np, red) : 0.0;
-    if (threadIdx.x == 0) mix_after_ka(ru, sa, sb, st);
-}
-
-// KD of the mixed slot: beta, or schedule a refinement.
-__global__ __launch_bounds__(kFin) void mix_kd(const double* __restrict__ pb, int np,
-                                               MixState* st)
-{
-    __shared__ double red[kFin / kWave];
-    if (st->done) return;
-    if (st->post_ru) {
-        if (threadIdx.x == 0) st->post_ru = 0;
-        return;
-    }
-    const double rrn = sum_partials_block<kFin>(pb, np, red);
-    if (threadIdx.x == 0) mix_after_kc(rrn, st);
-}
-
 // KA of the mixed slot: FP32 step SpMV, or the FP64 refinement SpMV.
+// Waves per SIMD the LDS of a mixed SpMV workgroup allows (160 KB per CU),
+// at most 7: the register budget is held to it (the FP64 refinement path
+// would otherwise set it for the FP32 steps too).
+template <int NPT>
+constexpr int mix_waves()
+{
+    constexpr int w = (160 * 1024) / (kSpmvBS * NPT * 12 + 1024) * (kSpmvBS / kWave) / 4;
+    return w > 7 ? 7 : (w < 1 ? 1 : w);
+}
+
 template <int NPT, bool NT32, bool NT64>
-__global__ __launch_bounds__(kSpmvBS) void mix_spmv_kernel(
+__global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_spmv_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const double* __restrict__ v64, const float* __restrict__ v32,
     const TileEntry* __restrict__ sched, int ntiles, int chunk, MixArgs a)
@@ -519,7 +557,7 @@ __global__ __launch_bounds__(kSpmvBS) void mix_spmv_kernel(
 // staged at once (FP32 values + int16 offsets fill the LDS of one FP64
 // tile); FP64 refinement: the two tiles in turn.  Partials: one per pair.
 template <int NPT, bool NT32, bool NT64>
-__global__ __launch_bounds__(kSpmvBS) void mix_pair_kernel(
+__global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const double* __restrict__ v64, const float* __restrict__ v32,
     const int16_t* __restrict__ rel, const TileEntry* __restrict__ sched, int ntiles,
@@ -563,10 +601,11 @@ __global__ __launch_bounds__(kSpmvBS) void mix_pair_kernel(
     }
 }
 
-__global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol, double delta,
-                         int32_t max_iter)
+__global__ void mix_init(const double* __restrict__ bb, MixState* st, uint32_t* ticket,
+                         double tol, double delta, int32_t max_iter)
 {
     if (threadIdx.x != 0) return;
+    *ticket = 0u;
     double bn = sqrt(bb[0]);
     if (bn == 0.0) bn = 1.0;
     st->bnorm = bn;
@@ -583,60 +622,108 @@ __global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol
     st->converged = 0;
     st->breakdown = 0;
     st->ru = 1;  // slot 0 evaluates r = b - A x0 in FP64
-    st->post_ru = 0;
     st->nref = 0;
     st->psel = 0;
     st->xsel = 0;
     st->final_ = max_iter <= 0 ? 1 : 0;
 }
 
-// KC of the mixed slot: x32 += alpha p_new, r32 -= alpha q (FP32) over float4
-// groups (n4 = n / 4 of them; the n % 4 tail by thread 0 of workgroup 0),
-// per-workgroup partials of r32.r32 (FP64).
-__global__ __launch_bounds__(kRedBlock) void mix_update4(int64_t n, MixArgs a)
+// KC of the mixed slot.  After a refinement: the last workgroup applies its
+// convergence test and beta (the others only draw tickets).  After an FP32
+// step: alpha = rr / p.q from KA's partials (every workgroup), then
+// x32 += alpha p_new, r32 -= alpha q (FP32) over float4 groups (n4 = n / 4 of
+// them; the n % 4 tail by thread 0 of workgroup 0) and r32.r32 partials
+// (FP64); the last workgroup folds them into the state.
+__global__ __launch_bounds__(kFin) void mix_kc(int64_t n, int npa, MixArgs a)
 {
-    __shared__ double red[kRedBlock / kWave];
-    const MixState* st = a.st;
-    if (st->done || st->post_ru) return;
-    const float al = (float)st->alpha;
-    const float nal = -al;
+    __shared__ double red[kFin / kWave];
+    __shared__ float s_alpha;
+    __shared__ int s_brk, s_last;
+    MixState* st = a.st;
+    if (st->done) return;  // read by every workgroup before any ticket is drawn
+    const int t = threadIdx.x;
+    if (st->ru) {
+        if (!take_ticket(a.ticket, &s_last)) return;
+        const double rr64 = sum_partials_block<kFin, 32>(a.pa, npa, red);
+        const double rr32 = sum_partials_block<kFin, 32>(a.pb, npa, red);
+        if (t == 0) mix_after_refine(rr64, rr32, st);
+        return;
+    }
+    const double rr = st->rr;
     const float* __restrict__ p = st->psel ? a.p0 : a.p1;
     float* __restrict__ x32 = st->xsel ? a.x32b : a.x32a;
     const float* __restrict__ q = a.q;
     float* __restrict__ r = a.r32;
     const int64_t n4 = n >> 2;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        const float4 pv = reinterpret_cast<const float4*>(p)[i];
-        const float4 qv = reinterpret_cast<const float4*>(q)[i];
-        float4 xv = reinterpret_cast<float4*>(x32)[i];
-        float4 rv = reinterpret_cast<float4*>(r)[i];
-        xv.x = al * pv.x + xv.x;
-        xv.y = al * pv.y + xv.y;
-        xv.z = al * pv.z + xv.z;
-        xv.w = al * pv.w + xv.w;
-        rv.x = nal * qv.x + rv.x;
-        rv.y = nal * qv.y + rv.y;
-        rv.z = nal * qv.z + rv.z;
-        rv.w = nal * qv.w + rv.w;
-        reinterpret_cast<float4*>(x32)[i] = xv;
-        reinterpret_cast<float4*>(r)[i] = rv;
-        acc += (double)rv.x * (double)rv.x;
-        acc += (double)rv.y * (double)rv.y;
-        acc += (double)rv.z * (double)rv.z;
-        acc += (double)rv.w * (double)rv.w;
+    const int64_t stride = (int64_t)gridDim.x * kFin;
+    const int64_t i0 = (int64_t)blockIdx.x * kFin + t;
+    // this thread's first float4 group, loaded together with the partials
+    float4 pv = {}, qv = {}, xv = {}, rv = {};
+    if (i0 < n4) {
+        pv = reinterpret_cast<const float4*>(p)[i0];
+        qv = reinterpret_cast<const float4*>(q)[i0];
+        xv = reinterpret_cast<const float4*>(x32)[i0];
+        rv = reinterpret_cast<const float4*>(r)[i0];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int64_t i = n4 * 4; i < n; ++i) {
-            x32[i] = al * p[i] + x32[i];
-            const float ri = nal * q[i] + r[i];
-            r[i] = ri;
-            acc += (double)ri * (double)ri;
+    const double pq = sum_partials_block<kFin, 32>(a.pa, npa, red);
+    if (t == 0) {
+        const int brk = !(fabs(pq) >= 1e-300);
+        s_brk = brk;
+        s_alpha = brk ? 0.0f : (float)(rr / pq);
+    }
+    __syncthreads();
+    const bool brk = s_brk != 0;
+    if (!brk) {
+        const float al = s_alpha;
+        const float nal = -al;
+        double acc = 0.0;
+        for (int64_t i = i0; i < n4; i += stride) {
+            if (i != i0) {
+                pv = reinterpret_cast<const float4*>(p)[i];
+                qv = reinterpret_cast<const float4*>(q)[i];
+                xv = reinterpret_cast<const float4*>(x32)[i];
+                rv = reinterpret_cast<const float4*>(r)[i];
+            }
+            xv.x = al * pv.x + xv.x;
+            xv.y = al * pv.y + xv.y;
+            xv.z = al * pv.z + xv.z;
+            xv.w = al * pv.w + xv.w;
+            rv.x = nal * qv.x + rv.x;
+            rv.y = nal * qv.y + rv.y;
+            rv.z = nal * qv.z + rv.z;
+            rv.w = nal * qv.w + rv.w;
+            reinterpret_cast<float4*>(x32)[i] = xv;
+            reinterpret_cast<float4*>(r)[i] = rv;
+            acc += (double)rv.x * (double)rv.x;
+            acc += (double)rv.y * (double)rv.y;
+            acc += (double)rv.z * (double)rv.z;
+            acc += (double)rv.w * (double)rv.w;
         }
+        if (blockIdx.x == 0 && t == 0) {
+            for (int64_t k = n4 * 4; k < n; ++k) {
+                x32[k] = al * p[k] + x32[k];
+                const float rk = nal * q[k] + r[k];
+                r[k] = rk;
+                acc += (double)rk * (double)rk;
+            }
+        }
+        const double tot = block_sum<kFin>(acc, red);
+        if (t == 0) publish(a.pb + blockIdx.x, tot);
     }
-    const double s = block_sum<kRedBlock>(acc, red);
-    if (threadIdx.x == 0) a.pb[blockIdx.x] = s;
+    if (!take_ticket(a.ticket, &s_last)) return;
+    // the last workgroup: every other one has read the state
+    if (brk) {
+        if (t == 0) {
+            st->breakdown = 1;
+            st->done = 1;
+        }
+        return;
+    }
+    const double rrn = sum_published_block<kFin>(a.pb, gridDim.x, red);
+    if (t == 0) {
+        st->alpha = rr / pq;
+        mix_after_step(rrn, st);
+    }
 }
 
 __global__ __launch_bounds__(kRedBlock) void mix_finish(int64_t n, const double* __restrict__ x64,
@@ -657,14 +744,15 @@ __global__ void zero_f32(int64_t n, float* __restrict__ a)
 
 struct MixLayout {
     size_t off_x64b, off_x32a, off_x32b, off_r, off_p0, off_p1, off_q, off_pa, off_pb, off_bb,
-        off_state, total;
+        off_state, off_ticket, total;
 };
 
 static MixLayout mix_layout(const num_csr* A)
 {
     MixLayout L;
     const size_t n = (size_t)A->nrows;
-    const int np = std::max(A->sched.nblocks, dot_grid(A->nrows));
+    // pb also holds the b.b partials of the set-up dot product
+    const int np = std::max({A->sched.nblocks, dot_grid(A->nrows), kMaxUpdateGroups});
     size_t o = 0;
     L.off_x64b = o; o += align256(n * 8);
     L.off_x32a = o; o += align256(n * 4);
@@ -677,6 +765,7 @@ static MixLayout mix_layout(const num_csr* A)
     L.off_pb = o; o += align256((size_t)np * 8);
     L.off_bb = o; o += 256;
     L.off_state = o; o += 256;
+    L.off_ticket = o; o += 256;
     L.total = o;
     return L;
 }
@@ -791,25 +880,24 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol, i
     double* pa = (double*)(w + L.off_pa);
     double* pb = (double*)(w + L.off_pb);
     CgState* state = (CgState*)(w + L.off_state);
+    uint32_t* ticket = (uint32_t*)(w + L.off_ticket);
     const int64_t n = A->nrows;
     const int np_spmv = A->sched.nblocks;
-    const int gu = dot_grid(n);
+    const int gk = update_grid(n);
 
     // p_old of the first step must be finite (it is multiplied by beta = 0)
     rc = num_fill(n, 0.0, p0, stream);
     if (rc) return rc;
     launch_tiles<double>(A, A->sched, A->vals, GatherX<double>{d_x},
                          EpiInit{d_b, r, pa, pb}, st);
-    cg_init_finalize<<<1, kFin, 0, st>>>(pa, pb, np_spmv, state, tol, max_iter);
+    cg_init_finalize<<<1, kFin, 0, st>>>(pa, pb, np_spmv, state, ticket, tol, max_iter);
     NUM_LAUNCH_CHECK();
 
     auto slot = [&]() -> int {
         GatherPNew<double> g{r, p0, p1, &state->beta, &state->psel, nullptr, 0.0};
         launch_tiles<double>(A, A->sched, A->vals, g, EpiPAp{p0, p1, Ap, state, pa, nullptr},
                              st);
-        cg_alpha<<<1, kFin, 0, st>>>(pa, np_spmv, state);
-        cg_update<<<gu, kRedBlock, 0, st>>>(n, p0, p1, Ap, d_x, r, state, pb);
-        cg_beta<<<1, kFin, 0, st>>>(pb, gu, state);
+        cg_kc<<<gk, kFin, 0, st>>>(n, p0, p1, Ap, d_x, r, state, pa, np_spmv, pb, ticket);
         NUM_LAUNCH_CHECK();
         return NUM_OK;
     };
@@ -866,22 +954,21 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     a.pb = (double*)(w + L.off_pb);
     double* bb = (double*)(w + L.off_bb);
     a.st = (MixState*)(w + L.off_state);
+    a.ticket = (uint32_t*)(w + L.off_ticket);
     const int64_t n = A->nrows;
-    const int gu = dot_grid(n);
+    const int gk = update_grid(n);
     const int ge = elementwise_grid(n);
 
     zero_f32<<<ge, kRedBlock, 0, st>>>(n, a.x32a);
     zero_f32<<<ge, kRedBlock, 0, st>>>(n, a.p0);
     rc = launch_dot(n, d_b, d_b, bb, a.pb, 0, st);
     if (rc) return rc;
-    mix_init<<<1, 64, 0, st>>>(bb, a.st, tol, delta, max_iter);
+    mix_init<<<1, 64, 0, st>>>(bb, a.st, a.ticket, tol, delta, max_iter);
     NUM_LAUNCH_CHECK();
     const int np_spmv = mix_paired(A) ? (A->sched.nblocks + 1) / 2 : A->sched.nblocks;
     auto slot = [&]() -> int {
         launch_mix_spmv(A, a, st);
-        mix_kb<<<1, kFin, 0, st>>>(a.pa, a.pb, np_spmv, a.st);
-        mix_update4<<<gu, kRedBlock, 0, st>>>(n, a);
-        mix_kd<<<1, kFin, 0, st>>>(a.pb, gu, a.st);
+        mix_kc<<<gk, kFin, 0, st>>>(n, np_spmv, a);
         NUM_LAUNCH_CHECK();
         return NUM_OK;
     };

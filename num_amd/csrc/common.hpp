@@ -107,22 +107,67 @@ __device__ __forceinline__ double block_sum(double v, double* red)
 }
 
 // Sum of n per-tile partials by one workgroup of BS threads in a fixed order:
-// thread t adds p[t], p[t + BS], ... with 8 independent loads in flight, then
-// the block tree.  Result valid in thread 0.  `red` holds BS / 64 doubles.
-template <int BS>
+// thread t adds p[t], p[t + BS], ... with U independent loads in flight (a
+// batch; predicated past n, so up to U * BS partials take one memory round
+// trip), then the block tree.  Result valid in thread 0.  `red` holds BS / 64
+// doubles.  The per-thread order does not depend on U.
+template <int BS, int U = 8>
 __device__ __forceinline__ double sum_partials_block(const double* __restrict__ p, int n,
                                                      double* red)
 {
     double acc = 0.0;
-    int i = threadIdx.x;
-    for (; i + 7 * BS < n; i += 8 * BS) {
-        double v[8];
+    for (int base = threadIdx.x; base < n; base += U * BS) {
+        double v[U];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = p[i + u * BS];
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * BS;
+            v[u] = i < n ? p[i] : 0.0;
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += v[u];
+        for (int u = 0; u < U; ++u)
+            if (base + u * BS < n) acc += v[u];
     }
-    for (; i < n; i += BS) acc += p[i];
+    return block_sum<BS>(acc, red);
+}
+
+// ---- finalize by the last workgroup of a launch ---------------------------
+// Each workgroup publishes its partial sum with an agent-scope store (written
+// through to the level all XCDs share, so none of the workgroup's other
+// output has to be written back early), waits for that store to complete and
+// draws a ticket with an agent-scope atomic.  The workgroup that draws the
+// last ticket re-arms the counter for the next launch, reads every partial
+// with agent-scope loads and finalizes.  No workgroup ever waits for another,
+// so nothing spins.  Every workgroup must read whatever the finalizer writes
+// before it draws its ticket.
+__device__ __forceinline__ void publish(double* slot, double v)
+{
+    __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by every thread after thread 0 has published (if it publishes);
+// returns, to every thread, whether this workgroup drew the last of
+// gridDim.x tickets.  `flag` is a __shared__ int.
+__device__ __forceinline__ bool take_ticket(uint32_t* counter, int* flag)
+{
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_s_waitcnt(0);  // the published store has completed
+        const uint32_t t =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int last = (t == gridDim.x - 1) ? 1 : 0;
+        if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *flag = last;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
+
+// sum_partials_block over published partials (agent-scope loads), same order.
+template <int BS>
+__device__ __forceinline__ double sum_published_block(const double* p, int n, double* red)
+{
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += BS)
+        acc += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return block_sum<BS>(acc, red);
 }
 
