@@ -2,22 +2,23 @@
 // FP32-inner / FP64-refined form, with fused kernels and device-resident
 // scalars.
 //
-// Both solvers run as a uniform sequence of 2-launch slots; every scalar
-// (alpha, beta, r.r, iteration count, stop flags) lives in a device state
-// record, so the host only queues slots and polls the `done` flag
-// asynchronously (two chunks in flight) instead of reading back two scalars
-// per iteration like the reference.  No launch is spent on a bare global
-// sum: alpha's p.Ap is summed by EVERY workgroup of the update kernel from
-// the SpMV kernel's per-tile partials (one fixed order, so all hold the same
-// bits), and the update kernel's own r.r is finished by its last workgroup
-// (common.hpp: publish / take_ticket), which also applies the convergence
-// test and beta.  Separate one-workgroup finalize kernels cost ~4.5 us each
-// per iteration on this GPU (two dependent memory round trips plus a
-// launch); folding the r.r into every workgroup of the next SpMV kernel
-// instead costs that kernel ~8% on 192^3 (a reduction and barrier per tile);
-// a last-workgroup ticket in the SpMV kernel itself (27k tickets per launch
-// on 192^3) measured 25% slower.  The update kernel has at most 256
-// workgroups, so its tickets are cheap.
+// Both solvers run as a uniform sequence of 2-launch slots (an SpMV kernel
+// KA, an update kernel KC); every scalar (alpha, beta, r.r, iteration count,
+// stop flags) lives in a device state record, so the host only queues slots
+// and polls the `done` flag asynchronously (two chunks in flight) instead of
+// reading back two scalars per iteration like the reference.  No launch is
+// spent on a bare global sum: alpha's p.Ap is summed by EVERY workgroup of KC
+// from KA's per-tile partials (one fixed order, so all hold the same bits).
+// KC's own r.r (<= 256 partials) is folded into the state (convergence test,
+// beta) in one of two ways, chosen by measurement:
+//  * FP64 CG: by KC's last workgroup (common.hpp: publish / take_ticket),
+//    ~2-3 us at the end of KC;
+//  * mixed CG: by every workgroup of the next KA (GatherMix's hook), ~0.2 ns
+//    of throughput per KA workgroup: cheaper for the 4096 pair workgroups of
+//    an FP32 step on 128^3, dearer for the 27k tiles of the FP64 KA on 192^3.
+// Separate one-workgroup finalize kernels cost ~4.5 us each per iteration on
+// this GPU (two dependent memory round trips plus a launch); a last-workgroup
+// ticket in KA itself (27k tickets per launch on 192^3) measured 25% slower.
 //
 // FP64 CG slot (the reference's arithmetic in its order):
 //   KA  p_new = 1*r + beta*p_old is recomputed inside the SpMV gather (the
@@ -33,14 +34,17 @@
 // Mixed slot (reliable updates): the same two kernels run the CG step in
 // FP32 (FP32 matrix copy and vectors, dot products accumulated in FP64,
 // scalars in FP64).  The FP32 solution correction x32 accumulates since the
-// last refinement.  When the FP32 recursive residual has dropped by a factor
-// delta since the last refinement, or looks converged, KC's last workgroup
-// raises `ru` and the next slot refines in FP64 instead of stepping: KA
-// computes x64' = x64 + x32 and the true residual r = b - A x64' in FP64
-// (re-seeding the FP32 residual from it and restarting x32 at 0), KC's last
-// workgroup tests the FP64 true residual against tol and recomputes beta from
-// the new residual, keeping the search direction (the Krylov space survives
-// the refinement).
+// last refinement.  KA first folds the previous update's r32.r32; when the
+// FP32 recursive residual has dropped by a factor delta since the last
+// refinement, or looks converged, the fold raises `ru` and that same KA
+// refines in FP64 instead of stepping: it computes x64' = x64 + x32 and the
+// true residual r = b - A x64' in FP64 (re-seeding the FP32 residual from it
+// and restarting x32 at 0); KC then tests the FP64 true residual against tol
+// and recomputes beta from the new residual, keeping the search direction
+// (the Krylov space survives the refinement).  The mixed state is
+// double-buffered: KA reads record A and its workgroup 0 writes B, KC reads B
+// and its workgroup 0 writes A, so no workgroup reads a record another
+// workgroup of the same launch writes.
 // When every row holds <= 32 entries and every column - row offset fits in
 // int16 (num_csr_attach_rel16), KA is mix_pair_kernel: one workgroup stages
 // two schedule tiles of FP32 values and int16 offsets at once (6 bytes per
@@ -55,7 +59,7 @@
 
 namespace numamd {
 
-constexpr int kFin = 1024;  // threads of the single-workgroup finalize kernels
+constexpr int kFin = 1024;  // threads of the update and set-up finalize kernels
 
 static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -198,20 +202,19 @@ __global__ __launch_bounds__(kFin) void cg_kc(int64_t n, const double* __restric
     const int it = st->it + 1, max_iter = st->max_iter, psel = st->psel;
     const double* __restrict__ p = psel ? p0 : p1;  // p_new was written to !psel
     const int64_t stride = (int64_t)gridDim.x * kFin;
-    int64_t i = (int64_t)blockIdx.x * kFin + t;
-    const int64_t j = i + stride;
-    double pi = 0.0, ai = 0.0, xi = 0.0, ri = 0.0, pj = 0.0, aj = 0.0, xj = 0.0, rj = 0.0;
-    if (i < n) {
-        pi = p[i];
-        ai = Ap[i];
-        xi = x[i];
-        ri = r[i];
-    }
-    if (j < n) {
-        pj = p[j];
-        aj = Ap[j];
-        xj = x[j];
-        rj = r[j];
+    const int64_t i0 = (int64_t)blockIdx.x * kFin + t;
+    // this thread's first kPre elements, loaded together with the partials
+    constexpr int kPre = 4;
+    double pv[kPre], av[kPre], xv[kPre], rv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            pv[u] = p[i];
+            av[u] = Ap[i];
+            xv[u] = x[i];
+            rv[u] = r[i];
+        }
     }
     const double pAp = sum_partials_block<kFin, 32>(pa, npa, red);
     if (t == 0) {
@@ -225,29 +228,28 @@ __global__ __launch_bounds__(kFin) void cg_kc(int64_t n, const double* __restric
     if (!brk) {
         const double nalpha = -alpha;
         double acc = 0.0;
-        if (i < n) {
-            x[i] = alpha * pi + xi;
-            const double rn = nalpha * ai + ri;
-            r[i] = rn;
-            acc += rn * rn;
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) {
+            const int64_t i = i0 + u * stride;
+            if (i < n) {
+                x[i] = alpha * pv[u] + xv[u];
+                const double rn = nalpha * av[u] + rv[u];
+                r[i] = rn;
+                acc += rn * rn;
+            }
         }
-        if (j < n) {
-            x[j] = alpha * pj + xj;
-            const double rn = nalpha * aj + rj;
-            r[j] = rn;
-            acc += rn * rn;
-        }
-        for (i += 2 * stride; i + stride < n; i += 2 * stride) {
+        int64_t i = i0 + kPre * stride;
+        for (; i + stride < n; i += 2 * stride) {
             const int64_t k = i + stride;
-            const double pv = p[i], pw = p[k], av = Ap[i], aw = Ap[k];
-            const double xv = x[i], xw = x[k], rv = r[i], rw = r[k];
-            x[i] = alpha * pv + xv;
+            const double pu = p[i], pw = p[k], au = Ap[i], aw = Ap[k];
+            const double xu = x[i], xw = x[k], ru = r[i], rw = r[k];
+            x[i] = alpha * pu + xu;
             x[k] = alpha * pw + xw;
-            const double rnv = nalpha * av + rv;
+            const double rnu = nalpha * au + ru;
             const double rnw = nalpha * aw + rw;
-            r[i] = rnv;
+            r[i] = rnu;
             r[k] = rnw;
-            acc += rnv * rnv;
+            acc += rnu * rnu;
             acc += rnw * rnw;
         }
         if (i < n) {
@@ -375,6 +377,35 @@ static int drive(hipStream_t stream, const State* d_state, int64_t max_slots, F 
 }
 
 // ============================================================ mixed CG ===
+// A value every lane holds alike, moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Workgroup 0 publishes the folded state: the record goes through LDS as
+// 32-bit words and thread 0 folds it in place.
+template <class S, class Fold>
+__device__ __forceinline__ void publish_state(const S* in, S* out, Fold fold)
+{
+    __shared__ S s;
+    const int t = threadIdx.x;
+    constexpr int W = (int)(sizeof(S) / 4);
+    if (t < W) reinterpret_cast<uint32_t*>(&s)[t] = reinterpret_cast<const uint32_t*>(in)[t];
+    __syncthreads();
+    if (t == 0) fold(s);
+    __syncthreads();
+    if (t < W) reinterpret_cast<uint32_t*>(out)[t] = reinterpret_cast<const uint32_t*>(&s)[t];
+}
+
+// State records are copied between device memory and LDS as 32-bit words.
+template <class S>
+constexpr int state_words() { return (int)(sizeof(S) / 4); }
+
 struct MixState {
     double rr;       // FP32 residual r32.r32 (FP64 accumulation) of the current step
     double rr_ref;   // rr right after the last refinement
@@ -387,12 +418,13 @@ struct MixState {
     int32_t it;      // FP32 CG steps taken
     int32_t max_iter;
     int32_t done, converged, breakdown;
-    int32_t ru;       // 1: this slot refines in FP64 instead of stepping
+    int32_t ru;       // 1: the next SpMV kernel refines in FP64 instead of stepping
+    int32_t pend;     // 1: the r32.r32 partials of the last update are not folded in yet
     int32_t nref;     // refinements done (FP64 residual evaluations)
     int32_t psel;     // p buffer holding the current search direction
     int32_t xsel;     // x64 / x32 buffer pair holding the current iterate
     int32_t final_;   // 1: the pending refinement ends the solve
-    int32_t pad_[2];
+    int32_t pad_;
 };
 static_assert(sizeof(MixState) <= 256, "state fits its slot");
 
@@ -407,9 +439,11 @@ struct MixArgs {
     float* p1;
     float* q;
     double* pa;     // SpMV kernel partials: p.q, or the refinement's FP64 r.r
-    double* pb;     // the refinement's FP32 r32.r32, then the update kernel's r32.r32
-    MixState* st;
-    uint32_t* ticket;
+    double* pb;     // update kernel partials: r32.r32
+    double* pc;     // the refinement's FP32 r32.r32 partials
+    int npb;        // update kernel workgroups (<= kSpmvBS)
+    MixState* sa;   // read by the SpMV kernel, written by the update kernel
+    MixState* sb;   // read by the update kernel, written by the SpMV kernel
 };
 
 // FP64 gather of the refined iterate x64 + x32
@@ -447,68 +481,172 @@ struct EpiRefine {
     }
 };
 
+// The scalar step after the SpMV kernel: the refinement's FP64 convergence
+// test and new beta (ru), else alpha = rr / p.q.
+__device__ __forceinline__ void mix_after_ka(bool ru, double sa, double sb, MixState& s)
+{
+    if (ru) {
+        const double rr64 = sa, rr32 = sb;
+        const double rel = sqrt(rr64) / s.bnorm;
+        s.rel = rel;
+        s.xsel ^= 1;
+        s.nref += 1;
+        s.ru = 0;
+        if (rel < s.tol) {
+            s.converged = 1;
+            s.done = 1;
+        } else if (s.final_ || !(rel == rel)) {
+            s.done = 1;
+        } else {
+            s.beta = rr32 / s.rr;  // rr = r.r the last p was built with
+            s.rr = rr32;
+            s.rr_ref = rr32;
+        }
+        return;
+    }
+    const double pq = sa;
+    if (!(fabs(pq) >= 1e-300)) {
+        s.breakdown = 1;
+        s.done = 1;
+    } else {
+        s.alpha = s.rr / pq;
+    }
+}
+
+// Fold the r32.r32 of an update into the state: beta, or schedule a
+// refinement.  mix_fold_view gives what a workgroup needs (beta, psel, ru);
+// mix_after_kc applies the whole fold to the published record.
+__device__ __forceinline__ void mix_fold_view(double rrn, const MixState& s, double& beta,
+                                              int& psel, int& ru, int& fin)
+{
+    beta = s.beta;
+    psel = s.psel;
+    ru = s.ru;
+    fin = s.final_;
+    if (s.done || !s.pend) return;
+    psel ^= 1;
+    const double est = sqrt(rrn) / s.bnorm;
+    if (s.it + 1 >= s.max_iter) {
+        ru = 1;
+        fin = 1;
+    } else if (est < s.tol || sqrt(rrn) < s.delta * sqrt(s.rr_ref) || !(rrn == rrn)) {
+        ru = 1;  // beta comes from the refined residual
+    } else {
+        beta = rrn / s.rr;
+    }
+}
+
+__device__ __forceinline__ void mix_after_kc(double rrn, MixState& s)
+{
+    if (s.done || !s.pend) return;
+    double beta;
+    int psel, ru, fin;
+    mix_fold_view(rrn, s, beta, psel, ru, fin);
+    s.pend = 0;
+    s.it += 1;
+    s.psel = psel;
+    s.ru = ru;
+    s.final_ = fin;
+    if (!ru) {
+        s.beta = beta;
+        s.rr = rrn;
+    }
+}
+
+// FP32 step gather: p_new[c] = 1*r32[c] + beta*p_old[c].  Its hook folds the
+// update kernel's partials into the state read from `in` (every workgroup
+// alike, through LDS as in GatherCg; workgroup 0 writes the result to `out`)
+// and walks only when the result is an FP32 step (not done, no refinement
+// due).  `done`, `ru` and `xsel` tell the kernel what to do otherwise.
+struct GatherMix {
+    const float* __restrict__ r;
+    const float* __restrict__ p0;
+    const float* __restrict__ p1;
+    const MixState* in;
+    MixState* out;
+    const double* pb;
+    int npb;
+    mutable const float* __restrict__ p_old;
+    mutable float beta;
+    mutable double part;
+    mutable int psel, done, ru, xsel;
+    __device__ __forceinline__ void prepare() const {}
+    __device__ __forceinline__ void issue() const
+    {
+        const int t = threadIdx.x;
+        part = t < npb ? pb[t] : 0.0;
+    }
+    __device__ __forceinline__ bool resolve() const
+    {
+        __shared__ double red[kSpmvBS / kWave];
+        const double rrn = uniform_d(block_total_once<kSpmvBS>(part, red));
+        double b;
+        int ps, r_, fin;
+        mix_fold_view(rrn, *in, b, ps, r_, fin);
+        beta = (float)uniform_d(b);
+        psel = uniform_i(ps);
+        ru = uniform_i(r_);
+        done = uniform_i(in->done);
+        xsel = uniform_i(in->xsel);
+        p_old = psel ? p1 : p0;
+        if (blockIdx.x == 0) publish_state(in, out, [rrn](MixState& st) { mix_after_kc(rrn, st); });
+        return !done && !ru;
+    }
+    __device__ __forceinline__ float operator()(int c) const { return r[c] + beta * p_old[c]; }
+};
+
 // FP32 step epilogue: p_new row to the other buffer, q = A p_new, p_new.q
 struct EpiPq {
     static constexpr int kRed = 1;
-    float* __restrict__ pn_out;
+    float* __restrict__ p0;
+    float* __restrict__ p1;
     float* __restrict__ q;
     double* partial_a;
     double* partial_b;
     __device__ __forceinline__ bool skip() const { return false; }
     __device__ __forceinline__ void operator()(int row, float v, double& ca, double&,
-                                               const GatherPNew<float>& g) const
+                                               const GatherMix& g) const
     {
         const float pn = g(row);
-        pn_out[row] = pn;
+        (g.psel ? p0 : p1)[row] = pn;
         q[row] = v;
         ca = (double)pn * (double)v;
     }
 };
 
-// The refinement's FP64 convergence test and new beta (KC's last workgroup
-// after a refinement slot).
-__device__ __forceinline__ void mix_after_refine(double rr64, double rr32, MixState* st)
+// The refinement a mixed SpMV workgroup runs instead of its FP32 step over
+// tiles [t0, t1) (two at most), one after the other; partials to pa / pc[w].
+template <int NPT, bool NT64>
+__device__ __forceinline__ void mix_refine_tiles(int t0, int t1, int w,
+                                                 const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ colind,
+                                                 const double* __restrict__ v64,
+                                                 const TileEntry* __restrict__ sched,
+                                                 const MixArgs& a, int xs, double* lv,
+                                                 int32_t* lc, double* red)
 {
-    const double rel = sqrt(rr64) / st->bnorm;
-    st->rel = rel;
-    st->xsel ^= 1;
-    st->nref += 1;
-    st->ru = 0;
-    if (rel < st->tol) {
-        st->converged = 1;
-        st->done = 1;
-    } else if (st->final_ || !(rel == rel)) {
-        st->done = 1;
-    } else {
-        st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
-        st->rr = rr32;
-        st->rr_ref = rr32;
+    GatherXsum g{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
+    EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, nullptr, nullptr};
+    double ca = 0.0, cb = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        __syncthreads();  // the LDS is staged again for this tile
+        double xa, xb;
+        spmv_tile_body<double, kSpmvBS, NPT, NT64>(t, rowptr, colind, v64, g, sched, epi, lv, lc,
+                                                   xa, xb);
+        ca += xa;
+        cb += xb;
+    }
+    block_sum2<kSpmvBS>(ca, cb, red);
+    if (threadIdx.x == 0) {
+        a.pa[w] = ca;
+        a.pc[w] = cb;
     }
 }
 
-// The scalar step after an FP32 update (KC's last workgroup): beta, or
-// schedule a refinement.
-__device__ __forceinline__ void mix_after_step(double rrn, MixState* st)
-{
-    const int it = st->it + 1;
-    st->it = it;
-    st->psel ^= 1;
-    const double est = sqrt(rrn) / st->bnorm;
-    if (it >= st->max_iter) {
-        st->ru = 1;
-        st->final_ = 1;
-    } else if (est < st->tol || sqrt(rrn) < st->delta * sqrt(st->rr_ref) || !(rrn == rrn)) {
-        st->ru = 1;  // beta comes from the refined residual
-    } else {
-        st->beta = rrn / st->rr;
-        st->rr = rrn;
-    }
-}
-
-// KA of the mixed slot: FP32 step SpMV, or the FP64 refinement SpMV.
-// Waves per SIMD the LDS of a mixed SpMV workgroup allows (160 KB per CU),
-// at most 7: the register budget is held to it (the FP64 refinement path
-// would otherwise set it for the FP32 steps too).
+// SpMV kernel of the mixed slot: FP32 step, or the FP64 refinement.
+// Waves per SIMD the LDS of a mixed SpMV workgroup allows (160 KB per CU):
+// the register budget is held to it (the FP64 refinement path would
+// otherwise set it for the FP32 steps too).
 template <int NPT>
 constexpr int mix_waves()
 {
@@ -527,35 +665,26 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_spmv_kernel(
     __shared__ double red[2 * kSpmvBS / kWave];
     const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
     if (tile < 0) return;
-    const MixState* st = a.st;
-    if (st->done) return;
+    GatherMix g{a.r32, a.p0, a.p1, a.sa, a.sb, a.pb, a.npb};
+    EpiPq epi{a.p0, a.p1, a.q, nullptr, nullptr};
     double ca, cb;
-    if (st->ru) {
-        const int xs = st->xsel;
-        GatherXsum g{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
-        EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, a.pa, a.pb};
-        spmv_tile_body<double, kSpmvBS, NPT, NT64>(tile, rowptr, colind, v64, g, sched, epi, lv,
-                                                   lc, ca, cb);
-        block_sum2<kSpmvBS>(ca, cb, red);
-        if (threadIdx.x == 0) {
-            a.pa[tile] = ca;
-            a.pb[tile] = cb;
-        }
-    } else {
-        GatherPNew<float> g{a.r32, a.p0, a.p1, &st->beta, &st->psel, nullptr, 0.0f};
-        g.prepare();
-        EpiPq epi{st->psel ? a.p0 : a.p1, a.q, a.pa, a.pb};
-        spmv_tile_body<float, kSpmvBS, NPT, NT32>(tile, rowptr, colind, v32, g, sched, epi,
-                                                  reinterpret_cast<float*>(lv), lc, ca, cb);
-        const double s = block_sum<kSpmvBS>(ca, red);
-        if (threadIdx.x == 0) a.pa[tile] = s;
+    spmv_tile_body<float, kSpmvBS, NPT, NT32>(tile, rowptr, colind, v32, g, sched, epi,
+                                              reinterpret_cast<float*>(lv), lc, ca, cb);
+    if (g.done) return;
+    if (g.ru) {
+        mix_refine_tiles<NPT, NT64>(tile, tile + 1, tile, rowptr, colind, v64, sched, a, g.xsel,
+                                    lv, lc, red);
+        return;
     }
+    const double s = block_sum<kSpmvBS>(ca, red);
+    if (threadIdx.x == 0) a.pa[tile] = s;
 }
 
-// KA of the mixed slot for a matrix with int16 column offsets and short rows
-// only: one workgroup per PAIR of schedule tiles.  FP32 step: both tiles are
-// staged at once (FP32 values + int16 offsets fill the LDS of one FP64
-// tile); FP64 refinement: the two tiles in turn.  Partials: one per pair.
+// SpMV kernel of the mixed slot for a matrix with int16 column offsets and
+// short rows only: one workgroup per PAIR of schedule tiles.  FP32 step:
+// both tiles are staged at once (FP32 values + int16 offsets fill the LDS of
+// one FP64 tile); FP64 refinement: the two tiles in turn.  Partials: one per
+// pair.
 template <int NPT, bool NT32, bool NT64>
 __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
@@ -568,162 +697,131 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     __shared__ double red[2 * kSpmvBS / kWave];
     const int p = chunk_tile(blockIdx.x, npairs, chunk);
     if (p < 0) return;
-    const MixState* st = a.st;
-    if (st->done) return;
     const int t0 = 2 * p, t1 = min(2 * p + 2, ntiles);
-    double ca = 0.0, cb = 0.0;
-    if (st->ru) {
-        const int xs = st->xsel;
-        GatherXsum g{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
-        EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, a.pa, a.pb};
-        for (int t = t0; t < t1; ++t) {
-            double xa, xb;
-            spmv_tile_body<double, kSpmvBS, NPT, NT64>(t, rowptr, colind, v64, g, sched, epi, lv,
-                                                       lc, xa, xb);
-            ca += xa;
-            cb += xb;
-            __syncthreads();  // the LDS is staged again for the next tile
-        }
-        block_sum2<kSpmvBS>(ca, cb, red);
-        if (threadIdx.x == 0) {
-            a.pa[p] = ca;
-            a.pb[p] = cb;
-        }
-    } else {
-        GatherPNew<float> g{a.r32, a.p0, a.p1, &st->beta, &st->psel, nullptr, 0.0f};
-        g.prepare();
-        EpiPq epi{st->psel ? a.p0 : a.p1, a.q, a.pa, a.pb};
-        spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
-                                              reinterpret_cast<float*>(lv),
-                                              reinterpret_cast<int16_t*>(lc), ca, cb);
-        const double s = block_sum<kSpmvBS>(ca, red);
-        if (threadIdx.x == 0) a.pa[p] = s;
+    GatherMix g{a.r32, a.p0, a.p1, a.sa, a.sb, a.pb, a.npb};
+    EpiPq epi{a.p0, a.p1, a.q, nullptr, nullptr};
+    double ca, cb;
+    spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
+                                          reinterpret_cast<float*>(lv),
+                                          reinterpret_cast<int16_t*>(lc), ca, cb);
+    if (g.done) return;
+    if (g.ru) {
+        mix_refine_tiles<NPT, NT64>(t0, t1, p, rowptr, colind, v64, sched, a, g.xsel, lv, lc,
+                                    red);
+        return;
     }
+    const double s = block_sum<kSpmvBS>(ca, red);
+    if (threadIdx.x == 0) a.pa[p] = s;
 }
 
-__global__ void mix_init(const double* __restrict__ bb, MixState* st, uint32_t* ticket,
-                         double tol, double delta, int32_t max_iter)
+__global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol, double delta,
+                         int32_t max_iter)
 {
     if (threadIdx.x != 0) return;
-    *ticket = 0u;
     double bn = sqrt(bb[0]);
     if (bn == 0.0) bn = 1.0;
-    st->bnorm = bn;
-    st->tol = tol;
-    st->delta = delta;
-    st->max_iter = max_iter;
     st->rr = INFINITY;  // first refinement: beta = rr'/inf = 0
     st->rr_ref = 0.0;
     st->alpha = 0.0;
     st->beta = 0.0;
+    st->bnorm = bn;
+    st->tol = tol;
+    st->delta = delta;
     st->rel = 0.0;
     st->it = 0;
+    st->max_iter = max_iter;
     st->done = 0;
     st->converged = 0;
     st->breakdown = 0;
     st->ru = 1;  // slot 0 evaluates r = b - A x0 in FP64
+    st->pend = 0;
     st->nref = 0;
     st->psel = 0;
     st->xsel = 0;
     st->final_ = max_iter <= 0 ? 1 : 0;
 }
 
-// KC of the mixed slot.  After a refinement: the last workgroup applies its
-// convergence test and beta (the others only draw tickets).  After an FP32
-// step: alpha = rr / p.q from KA's partials (every workgroup), then
-// x32 += alpha p_new, r32 -= alpha q (FP32) over float4 groups (n4 = n / 4 of
-// them; the n % 4 tail by thread 0 of workgroup 0) and r32.r32 partials
-// (FP64); the last workgroup folds them into the state.
+// Update kernel of the mixed slot.  After a refinement: its convergence test
+// and beta from the SpMV kernel's partials, no update.  After an FP32 step:
+// alpha = rr / p.q, then x32 += alpha p_new, r32 -= alpha q (FP32) over
+// float4 groups (n4 = n / 4 of them; the n % 4 tail by thread 0 of
+// workgroup 0) and per-workgroup partials of r32.r32 (FP64).
 __global__ __launch_bounds__(kFin) void mix_kc(int64_t n, int npa, MixArgs a)
 {
     __shared__ double red[kFin / kWave];
-    __shared__ float s_alpha;
-    __shared__ int s_brk, s_last;
-    MixState* st = a.st;
-    if (st->done) return;  // read by every workgroup before any ticket is drawn
+    __shared__ MixState s;
     const int t = threadIdx.x;
-    if (st->ru) {
-        if (!take_ticket(a.ticket, &s_last)) return;
-        const double rr64 = sum_partials_block<kFin, 32>(a.pa, npa, red);
-        const double rr32 = sum_partials_block<kFin, 32>(a.pb, npa, red);
-        if (t == 0) mix_after_refine(rr64, rr32, st);
-        return;
-    }
-    const double rr = st->rr;
-    const float* __restrict__ p = st->psel ? a.p0 : a.p1;
-    float* __restrict__ x32 = st->xsel ? a.x32b : a.x32a;
+    const MixState* in = a.sb;
+    const bool live = in->done == 0;
+    const bool ru = in->ru != 0;
+    const bool step = live && !ru;
+    if (t < state_words<MixState>())
+        reinterpret_cast<uint32_t*>(&s)[t] = reinterpret_cast<const uint32_t*>(in)[t];
+    // a thread's first float4 group, loaded together with the partials
+    const float* __restrict__ p = in->psel ? a.p0 : a.p1;
+    float* __restrict__ x32 = in->xsel ? a.x32b : a.x32a;
     const float* __restrict__ q = a.q;
     float* __restrict__ r = a.r32;
     const int64_t n4 = n >> 2;
     const int64_t stride = (int64_t)gridDim.x * kFin;
-    const int64_t i0 = (int64_t)blockIdx.x * kFin + t;
-    // this thread's first float4 group, loaded together with the partials
+    int64_t i = (int64_t)blockIdx.x * kFin + t;
     float4 pv = {}, qv = {}, xv = {}, rv = {};
-    if (i0 < n4) {
-        pv = reinterpret_cast<const float4*>(p)[i0];
-        qv = reinterpret_cast<const float4*>(q)[i0];
-        xv = reinterpret_cast<const float4*>(x32)[i0];
-        rv = reinterpret_cast<const float4*>(r)[i0];
+    if (step && i < n4) {
+        pv = reinterpret_cast<const float4*>(p)[i];
+        qv = reinterpret_cast<const float4*>(q)[i];
+        xv = reinterpret_cast<const float4*>(x32)[i];
+        rv = reinterpret_cast<const float4*>(r)[i];
     }
-    const double pq = sum_partials_block<kFin, 32>(a.pa, npa, red);
-    if (t == 0) {
-        const int brk = !(fabs(pq) >= 1e-300);
-        s_brk = brk;
-        s_alpha = brk ? 0.0f : (float)(rr / pq);
+    double s1 = 0.0, s2 = 0.0;
+    if (live) {
+        s1 = sum_partials_block<kFin, 32>(a.pa, npa, red);  // its barriers publish s
+        if (ru) s2 = sum_partials_block<kFin, 32>(a.pc, npa, red);
+    } else {
+        __syncthreads();
+    }
+    if (t == 0 && live) {
+        mix_after_ka(ru, s1, s2, s);
+        if (!ru) s.pend = s.done ? 0 : 1;
     }
     __syncthreads();
-    const bool brk = s_brk != 0;
-    if (!brk) {
-        const float al = s_alpha;
-        const float nal = -al;
-        double acc = 0.0;
-        for (int64_t i = i0; i < n4; i += stride) {
-            if (i != i0) {
-                pv = reinterpret_cast<const float4*>(p)[i];
-                qv = reinterpret_cast<const float4*>(q)[i];
-                xv = reinterpret_cast<const float4*>(x32)[i];
-                rv = reinterpret_cast<const float4*>(r)[i];
-            }
-            xv.x = al * pv.x + xv.x;
-            xv.y = al * pv.y + xv.y;
-            xv.z = al * pv.z + xv.z;
-            xv.w = al * pv.w + xv.w;
-            rv.x = nal * qv.x + rv.x;
-            rv.y = nal * qv.y + rv.y;
-            rv.z = nal * qv.z + rv.z;
-            rv.w = nal * qv.w + rv.w;
-            reinterpret_cast<float4*>(x32)[i] = xv;
-            reinterpret_cast<float4*>(r)[i] = rv;
-            acc += (double)rv.x * (double)rv.x;
-            acc += (double)rv.y * (double)rv.y;
-            acc += (double)rv.z * (double)rv.z;
-            acc += (double)rv.w * (double)rv.w;
+    if (blockIdx.x == 0 && t < state_words<MixState>())
+        reinterpret_cast<uint32_t*>(a.sa)[t] = reinterpret_cast<const uint32_t*>(&s)[t];
+    if (!step || s.done) return;
+    const float al = (float)s.alpha;
+    const float nal = -al;
+    double acc = 0.0;
+    for (; i < n4; i += stride) {
+        if (i != (int64_t)blockIdx.x * kFin + t) {
+            pv = reinterpret_cast<const float4*>(p)[i];
+            qv = reinterpret_cast<const float4*>(q)[i];
+            xv = reinterpret_cast<const float4*>(x32)[i];
+            rv = reinterpret_cast<const float4*>(r)[i];
         }
-        if (blockIdx.x == 0 && t == 0) {
-            for (int64_t k = n4 * 4; k < n; ++k) {
-                x32[k] = al * p[k] + x32[k];
-                const float rk = nal * q[k] + r[k];
-                r[k] = rk;
-                acc += (double)rk * (double)rk;
-            }
+        xv.x = al * pv.x + xv.x;
+        xv.y = al * pv.y + xv.y;
+        xv.z = al * pv.z + xv.z;
+        xv.w = al * pv.w + xv.w;
+        rv.x = nal * qv.x + rv.x;
+        rv.y = nal * qv.y + rv.y;
+        rv.z = nal * qv.z + rv.z;
+        rv.w = nal * qv.w + rv.w;
+        reinterpret_cast<float4*>(x32)[i] = xv;
+        reinterpret_cast<float4*>(r)[i] = rv;
+        acc += (double)rv.x * (double)rv.x;
+        acc += (double)rv.y * (double)rv.y;
+        acc += (double)rv.z * (double)rv.z;
+        acc += (double)rv.w * (double)rv.w;
+    }
+    if (blockIdx.x == 0 && t == 0) {
+        for (int64_t k = n4 * 4; k < n; ++k) {
+            x32[k] = al * p[k] + x32[k];
+            const float rk = nal * q[k] + r[k];
+            r[k] = rk;
+            acc += (double)rk * (double)rk;
         }
-        const double tot = block_sum<kFin>(acc, red);
-        if (t == 0) publish(a.pb + blockIdx.x, tot);
     }
-    if (!take_ticket(a.ticket, &s_last)) return;
-    // the last workgroup: every other one has read the state
-    if (brk) {
-        if (t == 0) {
-            st->breakdown = 1;
-            st->done = 1;
-        }
-        return;
-    }
-    const double rrn = sum_published_block<kFin>(a.pb, gridDim.x, red);
-    if (t == 0) {
-        st->alpha = rr / pq;
-        mix_after_step(rrn, st);
-    }
+    const double tot = block_sum<kFin>(acc, red);
+    if (t == 0) a.pb[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(kRedBlock) void mix_finish(int64_t n, const double* __restrict__ x64,
@@ -743,8 +841,8 @@ __global__ void zero_f32(int64_t n, float* __restrict__ a)
 }
 
 struct MixLayout {
-    size_t off_x64b, off_x32a, off_x32b, off_r, off_p0, off_p1, off_q, off_pa, off_pb, off_bb,
-        off_state, off_ticket, total;
+    size_t off_x64b, off_x32a, off_x32b, off_r, off_p0, off_p1, off_q, off_pa, off_pb, off_pc,
+        off_bb, off_sa, off_sb, total;
 };
 
 static MixLayout mix_layout(const num_csr* A)
@@ -752,7 +850,7 @@ static MixLayout mix_layout(const num_csr* A)
     MixLayout L;
     const size_t n = (size_t)A->nrows;
     // pb also holds the b.b partials of the set-up dot product
-    const int np = std::max({A->sched.nblocks, dot_grid(A->nrows), kMaxUpdateGroups});
+    const int np = std::max({A->sched.nblocks, dot_grid(A->nrows), kSpmvBS});
     size_t o = 0;
     L.off_x64b = o; o += align256(n * 8);
     L.off_x32a = o; o += align256(n * 4);
@@ -763,9 +861,10 @@ static MixLayout mix_layout(const num_csr* A)
     L.off_q = o;  o += align256(n * 4);
     L.off_pa = o; o += align256((size_t)np * 8);
     L.off_pb = o; o += align256((size_t)np * 8);
+    L.off_pc = o; o += align256((size_t)np * 8);
     L.off_bb = o; o += 256;
-    L.off_state = o; o += 256;
-    L.off_ticket = o; o += 256;
+    L.off_sa = o; o += 256;
+    L.off_sb = o; o += 256;
     L.total = o;
     return L;
 }
@@ -952,18 +1051,20 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     a.q = (float*)(w + L.off_q);
     a.pa = (double*)(w + L.off_pa);
     a.pb = (double*)(w + L.off_pb);
+    a.pc = (double*)(w + L.off_pc);
     double* bb = (double*)(w + L.off_bb);
-    a.st = (MixState*)(w + L.off_state);
-    a.ticket = (uint32_t*)(w + L.off_ticket);
+    a.sa = (MixState*)(w + L.off_sa);
+    a.sb = (MixState*)(w + L.off_sb);
     const int64_t n = A->nrows;
     const int gk = update_grid(n);
     const int ge = elementwise_grid(n);
+    a.npb = gk;
 
     zero_f32<<<ge, kRedBlock, 0, st>>>(n, a.x32a);
     zero_f32<<<ge, kRedBlock, 0, st>>>(n, a.p0);
     rc = launch_dot(n, d_b, d_b, bb, a.pb, 0, st);
     if (rc) return rc;
-    mix_init<<<1, 64, 0, st>>>(bb, a.st, a.ticket, tol, delta, max_iter);
+    mix_init<<<1, 64, 0, st>>>(bb, a.sa, tol, delta, max_iter);
     NUM_LAUNCH_CHECK();
     const int np_spmv = mix_paired(A) ? (A->sched.nblocks + 1) / 2 : A->sched.nblocks;
     auto slot = [&]() -> int {
@@ -974,11 +1075,11 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     };
     // FP32 steps plus one slot per refinement (at most one per step, plus
     // the first evaluation and a final one)
-    rc = drive(st, a.st, 2 * (int64_t)max_iter + 2, slot,
+    rc = drive(st, a.sa, 2 * (int64_t)max_iter + 2, slot,
                [](const MixState& h) { return h.done != 0; });
     if (rc) return rc;
     MixState h;
-    NUM_HIP(hipMemcpyAsync(&h, a.st, sizeof(MixState), hipMemcpyDeviceToHost, st));
+    NUM_HIP(hipMemcpyAsync(&h, a.sa, sizeof(MixState), hipMemcpyDeviceToHost, st));
     NUM_HIP(hipStreamSynchronize(st));
     // the iterate lives in x64[xsel] + x32[xsel]; bring it to the caller's x
     const double* xcur = h.xsel ? a.x64b : a.x64a;

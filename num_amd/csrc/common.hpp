@@ -130,6 +130,22 @@ __device__ __forceinline__ double sum_partials_block(const double* __restrict__ 
     return block_sum<BS>(acc, red);
 }
 
+// Block sum returned to every thread (each adds the wave totals in the same
+// order, so all hold the same bits; thread 0 matches block_sum).  No trailing
+// barrier: `red` must not be written again in this launch.
+template <int BS>
+__device__ __forceinline__ double block_total_once(double v, double* red)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < BS / kWave; ++i) s += red[i];
+    return s;
+}
+
 // ---- finalize by the last workgroup of a launch ---------------------------
 // Each workgroup publishes its partial sum with an agent-scope store (written
 // through to the level all XCDs share, so none of the workgroup's other

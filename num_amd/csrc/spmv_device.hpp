@@ -26,9 +26,24 @@
 // Tiles are dealt to the 8 XCDs in chunks of consecutive tiles (common.hpp:
 // chunk_tile), so each XCD's L2 sees neighbouring x windows.
 #pragma once
+#include <type_traits>
+#include <utility>
+
 #include "common.hpp"
 
 namespace numamd {
+
+// Optional per-workgroup hook of an x accessor (the mixed CG step kernels
+// compute their scalars from the previous kernel's partial sums inside every
+// workgroup): issue() runs right after the stage loads are issued (its loads
+// overlap the stage latency), resolve() after the stage barrier, before any
+// gather; when it returns false the workgroup walks no row.  Both are const
+// and keep what they compute in mutable members.
+template <class G, class = void>
+struct has_hook : std::false_type {};
+template <class G>
+struct has_hook<G, std::void_t<decltype(std::declval<const G&>().resolve())>>
+    : std::true_type {};
 
 // Matrix streams: non-temporal (NT) loads keep a matrix that is read once
 // per call from evicting x out of L2 / the Infinity Cache; a matrix small
@@ -85,6 +100,10 @@ __device__ __forceinline__ void spmv_tile_run(
         // long row: one row, whole workgroup; 8 independent loads in flight
         // per thread (clamped, no per-element branches), then a fixed-shape
         // tree (wave butterfly, waves in order).
+        if constexpr (has_hook<Gather>::value) {
+            x.issue();
+            if (!x.resolve()) return;
+        }
         T acc = T(0);
         const int last = j1 - 1;
         for (int j = j0 + t; j < j1; j += 8 * BS) {
@@ -130,6 +149,7 @@ __device__ __forceinline__ void spmv_tile_run(
             v[s] = ld_mat<NT>(vals + j);
             c[s] = ld_mat<NT>(colind + j);
         }
+        if constexpr (has_hook<Gather>::value) x.issue();
 #pragma unroll
         for (int s = 0; s < NPT; ++s) {
             const int k = t + s * BS;
@@ -138,8 +158,13 @@ __device__ __forceinline__ void spmv_tile_run(
                 lc[k] = c[s];
             }
         }
+    } else if constexpr (has_hook<Gather>::value) {
+        x.issue();
     }
     __syncthreads();
+    if constexpr (has_hook<Gather>::value) {
+        if (!x.resolve()) return;
+    }
     const int rb = rb_raw - j0, re = re_raw - j0;
     const bool wide = e0.wide != 0;  // uniform over the workgroup
     // 2a. short rows: one thread per row, entries in index order from 0, 8
@@ -252,6 +277,7 @@ __device__ __forceinline__ void spmv_rel16_run(
             v[s] = ld_mat<NT>(vals + j);
             c[s] = ld_mat<NT>(rel + j);
         }
+        if constexpr (has_hook<Gather>::value) x.issue();
 #pragma unroll
         for (int s = 0; s < M; ++s) {
             const int k = t + s * BS;
@@ -260,10 +286,15 @@ __device__ __forceinline__ void spmv_rel16_run(
                 lc[k] = c[s];
             }
         }
+    } else if constexpr (has_hook<Gather>::value) {
+        x.issue();
     }
     __syncthreads();
     ca = 0.0;
     cb = 0.0;
+    if constexpr (has_hook<Gather>::value) {
+        if (!x.resolve()) return;
+    }
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int lr = t + k * BS;
