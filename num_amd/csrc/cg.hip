@@ -119,6 +119,12 @@ struct EpiInit {
     }
 };
 
+// Output vectors of the CG kernels are stored non-temporally: the next
+// kernel finds them in the Infinity Cache or HBM either way (each XCD's L2 is
+// private), and streaming them out avoids a burst of L2 write-backs at the
+// end of the kernel (measured: 192^3 CG -2%, 128^3 mixed -4%).
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 // KA epilogue: own row of p_new to the other buffer, Ap, p_new_i * Ap_i
 struct EpiPAp {
     static constexpr int kRed = 1;
@@ -133,8 +139,8 @@ struct EpiPAp {
                                                const GatherPNew<double>& g) const
     {
         const double pn = g(row);
-        (st->psel ? p0 : p1)[row] = pn;
-        Ap[row] = v;
+        __builtin_nontemporal_store(pn, (st->psel ? p0 : p1) + row);
+        __builtin_nontemporal_store(v, Ap + row);
         ca = pn * v;
     }
 };
@@ -232,9 +238,9 @@ __global__ __launch_bounds__(kFin) void cg_kc(int64_t n, const double* __restric
         for (int u = 0; u < kPre; ++u) {
             const int64_t i = i0 + u * stride;
             if (i < n) {
-                x[i] = alpha * pv[u] + xv[u];
+                __builtin_nontemporal_store(alpha * pv[u] + xv[u], x + i);
                 const double rn = nalpha * av[u] + rv[u];
-                r[i] = rn;
+                __builtin_nontemporal_store(rn, r + i);
                 acc += rn * rn;
             }
         }
@@ -243,12 +249,12 @@ __global__ __launch_bounds__(kFin) void cg_kc(int64_t n, const double* __restric
             const int64_t k = i + stride;
             const double pu = p[i], pw = p[k], au = Ap[i], aw = Ap[k];
             const double xu = x[i], xw = x[k], ru = r[i], rw = r[k];
-            x[i] = alpha * pu + xu;
-            x[k] = alpha * pw + xw;
+            __builtin_nontemporal_store(alpha * pu + xu, x + i);
+            __builtin_nontemporal_store(alpha * pw + xw, x + k);
             const double rnu = nalpha * au + ru;
             const double rnw = nalpha * aw + rw;
-            r[i] = rnu;
-            r[k] = rnw;
+            __builtin_nontemporal_store(rnu, r + i);
+            __builtin_nontemporal_store(rnw, r + k);
             acc += rnu * rnu;
             acc += rnw * rnw;
         }
@@ -608,8 +614,8 @@ struct EpiPq {
                                                const GatherMix& g) const
     {
         const float pn = g(row);
-        (g.psel ? p0 : p1)[row] = pn;
-        q[row] = v;
+        __builtin_nontemporal_store(pn, (g.psel ? p0 : p1) + row);
+        __builtin_nontemporal_store(v, q + row);
         ca = (double)pn * (double)v;
     }
 };
@@ -757,20 +763,25 @@ __global__ __launch_bounds__(kFin) void mix_kc(int64_t n, int npa, MixArgs a)
     const bool step = live && !ru;
     if (t < state_words<MixState>())
         reinterpret_cast<uint32_t*>(&s)[t] = reinterpret_cast<const uint32_t*>(in)[t];
-    // a thread's first float4 group, loaded together with the partials
+    // a thread's first kPre float4 groups, loaded together with the partials
     const float* __restrict__ p = in->psel ? a.p0 : a.p1;
     float* __restrict__ x32 = in->xsel ? a.x32b : a.x32a;
     const float* __restrict__ q = a.q;
     float* __restrict__ r = a.r32;
     const int64_t n4 = n >> 2;
     const int64_t stride = (int64_t)gridDim.x * kFin;
-    int64_t i = (int64_t)blockIdx.x * kFin + t;
-    float4 pv = {}, qv = {}, xv = {}, rv = {};
-    if (step && i < n4) {
-        pv = reinterpret_cast<const float4*>(p)[i];
-        qv = reinterpret_cast<const float4*>(q)[i];
-        xv = reinterpret_cast<const float4*>(x32)[i];
-        rv = reinterpret_cast<const float4*>(r)[i];
+    const int64_t i0 = (int64_t)blockIdx.x * kFin + t;
+    constexpr int kPre = 2;
+    float4 pv[kPre], qv[kPre], xv[kPre], rv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int64_t i = i0 + u * stride;
+        if (step && i < n4) {
+            pv[u] = reinterpret_cast<const float4*>(p)[i];
+            qv[u] = reinterpret_cast<const float4*>(q)[i];
+            xv[u] = reinterpret_cast<const float4*>(x32)[i];
+            rv[u] = reinterpret_cast<const float4*>(r)[i];
+        }
     }
     double s1 = 0.0, s2 = 0.0;
     if (live) {
@@ -790,28 +801,30 @@ __global__ __launch_bounds__(kFin) void mix_kc(int64_t n, int npa, MixArgs a)
     const float al = (float)s.alpha;
     const float nal = -al;
     double acc = 0.0;
-    for (; i < n4; i += stride) {
-        if (i != (int64_t)blockIdx.x * kFin + t) {
-            pv = reinterpret_cast<const float4*>(p)[i];
-            qv = reinterpret_cast<const float4*>(q)[i];
-            xv = reinterpret_cast<const float4*>(x32)[i];
-            rv = reinterpret_cast<const float4*>(r)[i];
-        }
-        xv.x = al * pv.x + xv.x;
-        xv.y = al * pv.y + xv.y;
-        xv.z = al * pv.z + xv.z;
-        xv.w = al * pv.w + xv.w;
-        rv.x = nal * qv.x + rv.x;
-        rv.y = nal * qv.y + rv.y;
-        rv.z = nal * qv.z + rv.z;
-        rv.w = nal * qv.w + rv.w;
-        reinterpret_cast<float4*>(x32)[i] = xv;
-        reinterpret_cast<float4*>(r)[i] = rv;
-        acc += (double)rv.x * (double)rv.x;
-        acc += (double)rv.y * (double)rv.y;
-        acc += (double)rv.z * (double)rv.z;
-        acc += (double)rv.w * (double)rv.w;
+    auto update = [&](int64_t i, const float4& pw, const float4& qw, float4 xw, float4 rw) {
+        xw.x = al * pw.x + xw.x;
+        xw.y = al * pw.y + xw.y;
+        xw.z = al * pw.z + xw.z;
+        xw.w = al * pw.w + xw.w;
+        rw.x = nal * qw.x + rw.x;
+        rw.y = nal * qw.y + rw.y;
+        rw.z = nal * qw.z + rw.z;
+        rw.w = nal * qw.w + rw.w;
+        __builtin_nontemporal_store(f4v{xw.x, xw.y, xw.z, xw.w}, reinterpret_cast<f4v*>(x32) + i);
+        __builtin_nontemporal_store(f4v{rw.x, rw.y, rw.z, rw.w}, reinterpret_cast<f4v*>(r) + i);
+        acc += (double)rw.x * (double)rw.x;
+        acc += (double)rw.y * (double)rw.y;
+        acc += (double)rw.z * (double)rw.z;
+        acc += (double)rw.w * (double)rw.w;
+    };
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int64_t i = i0 + u * stride;
+        if (i < n4) update(i, pv[u], qv[u], xv[u], rv[u]);
     }
+    for (int64_t i = i0 + kPre * stride; i < n4; i += stride)
+        update(i, reinterpret_cast<const float4*>(p)[i], reinterpret_cast<const float4*>(q)[i],
+               reinterpret_cast<const float4*>(x32)[i], reinterpret_cast<const float4*>(r)[i]);
     if (blockIdx.x == 0 && t == 0) {
         for (int64_t k = n4 * 4; k < n; ++k) {
             x32[k] = al * p[k] + x32[k];
