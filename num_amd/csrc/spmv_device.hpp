@@ -384,7 +384,9 @@ struct EpiY {
     template <class Gather>
     __device__ __forceinline__ void operator()(int row, T v, double&, double&, const Gather&) const
     {
-        y[row] = v;
+        // streamed out: no burst of L2 write-backs at the end of the kernel
+        // (192^3: 120.9 -> 118.3 us)
+        __builtin_nontemporal_store(v, y + row);
     }
 };
 
