@@ -161,15 +161,16 @@ __device__ __forceinline__ void publish(double* slot, double v)
 }
 
 // Called by every thread after thread 0 has published (if it publishes);
-// returns, to every thread, whether this workgroup drew the last of
-// gridDim.x tickets.  `flag` is a __shared__ int.
-__device__ __forceinline__ bool take_ticket(uint32_t* counter, int* flag)
+// returns, to every thread, whether this workgroup drew the last of `total`
+// tickets (default: one per workgroup of the launch).  `flag` is a
+// __shared__ int.
+__device__ __forceinline__ bool take_ticket(uint32_t* counter, int* flag, uint32_t total = 0)
 {
     if (threadIdx.x == 0) {
         __builtin_amdgcn_s_waitcnt(0);  // the published store has completed
         const uint32_t t =
             __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int last = (t == gridDim.x - 1) ? 1 : 0;
+        const int last = (t == (total ? total : gridDim.x) - 1) ? 1 : 0;
         if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         *flag = last;
     }

@@ -3,17 +3,17 @@
 // "batch" is k right-hand sides on one matrix, src/batch_spmv_kernel.hpp:4).
 //
 // Each admitted system keeps its matrix (its own row tiles), its vectors and
-// a per-system state record resident in HBM.  One engine iteration is four
+// a per-system state record resident in HBM.  One engine iteration is two
 // launches over ALL live systems:
 //   KA  row tiles of every live system (one workgroup per tile): the SpMV
 //       of the system's current phase (set-up, FP64 step, FP64 refinement or
 //       FP32 step) with its fused epilogue; per-tile partial sums
-//   KB  one workgroup per live system: sums that system's tile partials in
-//       index order -> alpha (or the set-up / refinement finish)
-//   KC  2048-row chunks of every live system: the vector updates, per-chunk
-//       partials
-//   KD  one workgroup per live system: beta, convergence, the refinement
-//       trigger of the mixed form
+//   KC  chunks (>= 2048 rows, <= 256 per system) of every live system: each
+//       sums its system's tile partials in index order -> alpha, updates its
+//       rows and publishes a partial; the system's last chunk (an
+//       agent-scope ticket per system, common.hpp) sums the chunk partials
+//       -> beta, convergence, the refinement trigger of the mixed form (or
+//       finishes the set-up / a refinement, when KA evaluated a residual)
 // A step (k iterations) ends with two more launches: the mixed form's
 // finished systems fold x64 + x32 into the caller's x, and one record per
 // live system (phase, iterations, residual) is packed and copied to pinned
@@ -57,7 +57,8 @@ enum : int32_t {
 };
 
 constexpr int kEngNpt = 7;        // LDS entries per thread of the engine's tile kernel
-constexpr int kChunkRows = 2048;  // rows per KC workgroup (256 threads x 8)
+constexpr int kChunkRows = 2048;  // rows per KC workgroup (256 threads x 8), at least
+constexpr int kMaxChunks = 256;   // chunks per system, at most (each sums the tile partials)
 // Rows per tile of an assembled stencil (<= 7 entries per row): 256, or 512
 // in the mixed form, whose FP32 steps stage a 512-row tile at once (FP32
 // values and int16 column offsets fill the LDS of one 256-row FP64 tile) and
@@ -86,11 +87,15 @@ struct SysDesc {
     float* p32b;
     const int16_t* rel16;  // column - row (FP32 phases), with 512-row tiles; or nullptr
     int32_t n, ntiles, mode, rpt;
+    int32_t crows;         // rows per update-kernel chunk
+    int32_t pad_;
 };
 
 struct SysState {
     double rr, rr_ref, alpha, beta, bnorm, tol, delta, rel;
-    int32_t it, max_iter, phase, post, converged, breakdown, psel, xsel, nref, final_, gen;
+    int32_t it, max_iter, phase;
+    uint32_t tick;  // tickets of the update kernel's chunks (common.hpp: take_ticket)
+    int32_t converged, breakdown, psel, xsel, nref, final_, gen;
     int32_t pad[5];
 };
 static_assert(sizeof(SysState) == 128, "state record is 128 bytes");
@@ -108,8 +113,11 @@ static_assert(sizeof(EngRec) == 32, "record is 32 bytes");
 struct BTile {
     int32_t sys, tile, r0, r1, nz0, nz1, wide, pad;
 };
+// One entry of the chunk list: the system, its rows [row0, row1), and where
+// the system's tile partials [b0, b0 + nb) and chunk partials [c0, c0 + nc)
+// sit in the launch's partial arrays.
 struct Chunk {
-    int32_t sys, row0, row1, pad;
+    int32_t sys, row0, row1, b0, nb, c0, nc, pad;
 };
 
 // ---------------------------------------------------------------- gathers
@@ -158,8 +166,8 @@ struct EEpiStep {  // p_new row to the other buffer, Ap, p_new . Ap
                                                const EGatherPNew<T>& g) const
     {
         const T pn = g(row);
-        pn_out[row] = pn;
-        Ap[row] = v;
+        __builtin_nontemporal_store(pn, pn_out + row);  // streamed (see cg.hip)
+        __builtin_nontemporal_store(v, Ap + row);
         ca = (double)pn * (double)v;
     }
 };
@@ -275,22 +283,96 @@ __global__ __launch_bounds__(kSpmvBS) void eng_ka(const BTile* __restrict__ bts,
     }
 }
 
-__global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ act,
-                                                  const int32_t* __restrict__ toff,
-                                                  const SysDesc* __restrict__ sd,
-                                                  SysState* __restrict__ ss,
-                                                  const double* __restrict__ pa,
-                                                  const double* __restrict__ pb)
+// One chunk's CG update x += alpha p, r -= alpha q (T = double: q = Ap; T =
+// float: the FP32 step), alpha = rr / p.q from the system's tile partials.
+// kPre rows per thread are loaded before alpha's sum, the rest kPre at a time
+// (all loads of a group in flight together).  acc gets this thread's r.r.
+template <typename T>
+__device__ __forceinline__ void chunk_update(const Chunk& e, const T* __restrict__ p,
+                                             const T* __restrict__ q, T* __restrict__ x,
+                                             T* __restrict__ r, double rr,
+                                             const double* __restrict__ pa, double* red,
+                                             double& s_alpha, int& s_brk, double& acc)
 {
-    __shared__ double red[kSpmvBS / kWave];
-    const int s = act[blockIdx.x];
-    SysState* st = ss + s;
+    constexpr int kPre = 4;
+    const int t = threadIdx.x;
+    T pv[kPre], qv[kPre], xv[kPre], rv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int i = e.row0 + t + u * kRedBlock;
+        if (i < e.row1) {
+            pv[u] = p[i];
+            qv[u] = q[i];
+            xv[u] = x[i];
+            rv[u] = r[i];
+        }
+    }
+    const double pq = sum_partials_block<kRedBlock, 32>(pa + e.b0, e.nb, red);
+    if (t == 0) {
+        const int brk = !(fabs(pq) >= 1e-300);  // src/cg_solver.hpp:84-85
+        s_brk = brk;
+        s_alpha = brk ? 0.0 : rr / pq;
+    }
+    __syncthreads();
+    if (s_brk) return;
+    const T al = (T)s_alpha, nal = -al;
+    for (int base = e.row0 + t;; base += kPre * kRedBlock) {
+        if (base != e.row0 + t) {
+            if (base >= e.row1) break;
+#pragma unroll
+            for (int u = 0; u < kPre; ++u) {
+                const int i = base + u * kRedBlock;
+                if (i < e.row1) {
+                    pv[u] = p[i];
+                    qv[u] = q[i];
+                    xv[u] = x[i];
+                    rv[u] = r[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) {
+            const int i = base + u * kRedBlock;
+            if (i < e.row1) {
+                __builtin_nontemporal_store(al * pv[u] + xv[u], x + i);
+                const T ri = nal * qv[u] + rv[u];
+                __builtin_nontemporal_store(ri, r + i);
+                acc += (double)ri * (double)ri;
+            }
+        }
+    }
+}
+
+// The update kernel, one workgroup per chunk of every live system.
+//  * PH_INIT / PH_REFINE (the SpMV kernel evaluated a residual; no update):
+//    the system's last chunk (ticket) sums its tile partials and finishes the
+//    set-up or the refinement (convergence test, beta).
+//  * PH_RUN64 / PH_RUN32: every chunk sums the system's tile partials (same
+//    order, same bits) -> alpha, updates its rows and publishes its r.r
+//    partial; the system's last chunk folds them: beta, convergence, the
+//    mixed form's refinement trigger.
+// Every chunk reads the state before it draws its ticket, and only the last
+// writes it.
+__global__ __launch_bounds__(kRedBlock) void eng_kc(const Chunk* __restrict__ chs,
+                                                    const SysDesc* __restrict__ sd,
+                                                    SysState* __restrict__ ss,
+                                                    const double* __restrict__ pa,
+                                                    const double* __restrict__ pb,
+                                                    double* __restrict__ pc)
+{
+    __shared__ double red[kRedBlock / kWave];
+    __shared__ double s_alpha;
+    __shared__ int s_brk, s_last;
+    const Chunk e = chs[blockIdx.x];
+    const SysDesc* d = sd + e.sys;
+    SysState* st = ss + e.sys;
     const int ph = st->phase;
-    const int b0 = toff[blockIdx.x], nb = toff[blockIdx.x + 1] - b0;
+    const int t = threadIdx.x;
     if (ph == PH_INIT) {
-        const double rr = sum_partials_block<kSpmvBS>(pa + b0, nb, red);
-        const double bb = sum_partials_block<kSpmvBS>(pb + b0, nb, red);
-        if (threadIdx.x == 0) {
+        if (!take_ticket(&st->tick, &s_last, e.nc)) return;
+        const double rr = sum_partials_block<kRedBlock>(pa + e.b0, e.nb, red);
+        const double bb = sum_partials_block<kRedBlock>(pb + e.b0, e.nb, red);
+        if (t == 0) {
             double bn = sqrt(bb);
             if (bn == 0.0) bn = 1.0;  // src/cg_solver.hpp:153
             st->bnorm = bn;
@@ -299,7 +381,6 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
             st->beta = 0.0;
             st->psel = 0;
             st->it = 0;
-            st->post = 1;
             if (sqrt(rr) / bn < st->tol) {  // early exit, src/cg_solver.hpp:158-159
                 st->converged = 1;
                 st->phase = PH_DONE;
@@ -309,10 +390,13 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
                 st->phase = PH_RUN64;
             }
         }
-    } else if (ph == PH_REFINE) {
-        const double rr64 = sum_partials_block<kSpmvBS>(pa + b0, nb, red);
-        double rr32 = sum_partials_block<kSpmvBS>(pb + b0, nb, red);
-        if (threadIdx.x == 0) {
+        return;
+    }
+    if (ph == PH_REFINE) {
+        if (!take_ticket(&st->tick, &s_last, e.nc)) return;
+        const double rr64 = sum_partials_block<kRedBlock>(pa + e.b0, e.nb, red);
+        double rr32 = sum_partials_block<kRedBlock>(pb + e.b0, e.nb, red);
+        if (t == 0) {
             if (st->nref == 0) {
                 // first refinement: the second partial was b.b
                 double bn = sqrt(rr32);
@@ -324,12 +408,11 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
             st->rel = rel;
             st->xsel ^= 1;
             st->nref += 1;
-            st->post = 1;
             if (rel < st->tol) {
                 st->converged = 1;
-                st->phase = done_phase(sd + s, st);
+                st->phase = done_phase(d, st);
             } else if (st->final_ || !(rel == rel)) {
-                st->phase = done_phase(sd + s, st);
+                st->phase = done_phase(d, st);
             } else {
                 st->beta = rr32 / st->rr;  // rr = r.r the last p was built with
                 st->rr = rr32;
@@ -337,81 +420,42 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kb(const int32_t* __restrict__ ac
                 st->phase = PH_RUN32;
             }
         }
-    } else if (ph == PH_RUN64 || ph == PH_RUN32) {
-        const double pq = sum_partials_block<kSpmvBS>(pa + b0, nb, red);
-        if (threadIdx.x == 0) {
-            if (!(fabs(pq) >= 1e-300)) {  // src/cg_solver.hpp:84-85
-                st->breakdown = 1;
-                st->phase = done_phase(sd + s, st);
-            } else {
-                st->alpha = st->rr / pq;
-            }
-        }
-    }
-}
-
-__global__ __launch_bounds__(kRedBlock) void eng_kc(const Chunk* __restrict__ chs,
-                                                    const SysDesc* __restrict__ sd,
-                                                    const SysState* __restrict__ ss,
-                                                    double* __restrict__ pc)
-{
-    __shared__ double red[kRedBlock / kWave];
-    const Chunk e = chs[blockIdx.x];
-    const SysDesc* d = sd + e.sys;
-    const SysState* st = ss + e.sys;
-    const int ph = st->phase;
-    if (st->post || (ph != PH_RUN64 && ph != PH_RUN32)) return;
-    double acc = 0.0;
-    if (ph == PH_RUN64) {
-        const double al = st->alpha, nal = -al;
-        const double* __restrict__ p = st->psel ? d->p0 : d->p1;  // p_new
-        double* __restrict__ x = d->x;
-        double* __restrict__ r = d->r;
-        const double* __restrict__ Ap = d->Ap;
-        for (int i = e.row0 + threadIdx.x; i < e.row1; i += kRedBlock) {
-            x[i] = al * p[i] + x[i];
-            const double ri = nal * Ap[i] + r[i];
-            r[i] = ri;
-            acc += ri * ri;
-        }
-    } else {
-        const float al = (float)st->alpha, nal = -al;
-        const float* __restrict__ p = st->psel ? d->p32a : d->p32b;
-        float* __restrict__ x32 = st->xsel ? d->x32b : d->x32a;
-        float* __restrict__ r = d->r32;
-        const float* __restrict__ q = d->q32;
-        for (int i = e.row0 + threadIdx.x; i < e.row1; i += kRedBlock) {
-            x32[i] = al * p[i] + x32[i];
-            const float ri = nal * q[i] + r[i];
-            r[i] = ri;
-            acc += (double)ri * (double)ri;
-        }
-    }
-    const double s = block_sum<kRedBlock>(acc, red);
-    if (threadIdx.x == 0) pc[blockIdx.x] = s;
-}
-
-__global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ act,
-                                                  const int32_t* __restrict__ coff,
-                                                  const SysDesc* __restrict__ sd,
-                                                  SysState* __restrict__ ss,
-                                                  const double* __restrict__ pc)
-{
-    __shared__ double red[kSpmvBS / kWave];
-    const int s = act[blockIdx.x];
-    SysState* st = ss + s;
-    const int ph = st->phase;
-    if (ph != PH_RUN64 && ph != PH_RUN32) return;
-    if (st->post) {
-        if (threadIdx.x == 0) st->post = 0;
         return;
     }
-    const int c0 = coff[blockIdx.x], nc = coff[blockIdx.x + 1] - c0;
-    const double rrn = sum_partials_block<kSpmvBS>(pc + c0, nc, red);
-    if (threadIdx.x != 0) return;
+    if (ph != PH_RUN64 && ph != PH_RUN32) return;
+    const double rr = st->rr;
+    const int psel = st->psel, xsel = st->xsel;
+    double acc = 0.0;
+    bool brk;
+    double alpha;
+    if (ph == PH_RUN64) {
+        chunk_update<double>(e, psel ? d->p0 : d->p1, d->Ap, d->x, d->r, rr, pa, red, s_alpha,
+                             s_brk, acc);
+    } else {
+        chunk_update<float>(e, psel ? d->p32a : d->p32b, d->q32, xsel ? d->x32b : d->x32a,
+                            d->r32, rr, pa, red, s_alpha, s_brk, acc);
+    }
+    brk = s_brk != 0;
+    alpha = s_alpha;
+    if (!brk) {
+        const double tot = block_sum<kRedBlock>(acc, red);
+        if (t == 0) publish(pc + blockIdx.x, tot);
+    }
+    if (!take_ticket(&st->tick, &s_last, e.nc)) return;
+    // the system's last chunk
+    if (brk) {
+        if (t == 0) {
+            st->breakdown = 1;
+            st->phase = done_phase(d, st);
+        }
+        return;
+    }
+    const double rrn = sum_published_block<kRedBlock>(pc + e.c0, e.nc, red);
+    if (t != 0) return;
+    st->alpha = alpha;
     const int it = st->it + 1;
     st->it = it;
-    st->psel ^= 1;
+    st->psel = psel ^ 1;
     if (ph == PH_RUN64) {
         const double rel = sqrt(rrn) / st->bnorm;
         st->rel = rel;
@@ -419,7 +463,7 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ ac
             st->converged = 1;
             st->phase = PH_DONE;
         } else {
-            st->beta = rrn / st->rr;
+            st->beta = rrn / rr;
             st->rr = rrn;
             if (it >= st->max_iter) st->phase = PH_DONE;
         }
@@ -432,7 +476,7 @@ __global__ __launch_bounds__(kSpmvBS) void eng_kd(const int32_t* __restrict__ ac
                    !(rrn == rrn)) {
             st->phase = PH_REFINE;  // beta comes from the refined residual
         } else {
-            st->beta = rrn / st->rr;
+            st->beta = rrn / rr;
             st->rr = rrn;
         }
     }
@@ -522,8 +566,10 @@ __global__ void eng_expand(const int32_t* __restrict__ act, const int32_t* __res
     if (g < nch) {
         const int i = owner_of(coff, nact, g);
         const int s = act[i];
-        const int r0 = (g - coff[i]) * kChunkRows;
-        ch[g] = {s, r0, min(sd[s].n, r0 + kChunkRows), 0};
+        const int cr = sd[s].crows;
+        const int r0 = (g - coff[i]) * cr;
+        ch[g] = {s, r0, min(sd[s].n, r0 + cr), toff[i], toff[i + 1] - toff[i], coff[i],
+                 coff[i + 1] - coff[i], 0};
     }
 }
 
@@ -871,8 +917,6 @@ static int eng_rebuild(num_engine* E, hipStream_t st)
 static int eng_enqueue_step(num_engine* E, int32_t iters, hipStream_t st)
 {
     const int32_t* act = eng_act(E);
-    const int32_t* toff = eng_toff(E);
-    const int32_t* coff = eng_coff(E);
     const int chunk = tile_chunk(E->nbt);
     const int grid_a = chunk_grid(E->nbt, chunk);
     for (int32_t k = 0; k < iters; ++k) {
@@ -891,10 +935,9 @@ static int eng_enqueue_step(num_engine* E, int32_t iters, hipStream_t st)
                 eng_ka<false, false><<<grid_a, kSpmvBS, 0, st>>>(
                     E->d_bt, E->nbt, chunk, E->d_desc, E->d_state, E->d_pa, E->d_pb);
         }
-        eng_kb<<<E->nact, kSpmvBS, 0, st>>>(act, toff, E->d_desc, E->d_state, E->d_pa, E->d_pb);
         if (E->nch > 0)
-            eng_kc<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state, E->d_pc);
-        eng_kd<<<E->nact, kSpmvBS, 0, st>>>(act, coff, E->d_desc, E->d_state, E->d_pc);
+            eng_kc<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state, E->d_pa,
+                                                 E->d_pb, E->d_pc);
     }
     if (E->nch > 0) eng_fold<<<E->nch, kRedBlock, 0, st>>>(E->d_ch, E->d_desc, E->d_state);
     eng_pack<<<(E->nact + 255) / 256, 256, 0, st>>>(act, E->nact, E->d_state, E->d_rec);
@@ -938,6 +981,15 @@ static int eng_check(const num_engine* E)
     NUM_HIP(hipGetDevice(&dev));
     if (dev != E->device) return fail(NUM_EINVAL, "engine lives on device %d", E->device);
     return NUM_OK;
+}
+
+// Rows per update-kernel chunk of an n-row system: kChunkRows, or more (a
+// multiple of kRedBlock) so that no system has more than kMaxChunks chunks.
+static int32_t chunk_rows(int64_t n)
+{
+    const int64_t per = (n + kMaxChunks - 1) / kMaxChunks;
+    const int64_t r = (per + kRedBlock - 1) / kRedBlock * kRedBlock;
+    return (int32_t)std::max<int64_t>(kChunkRows, r);
 }
 
 static SysState initial_state(double tol, int32_t max_iter, int32_t mode, double delta)
@@ -1098,7 +1150,8 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
                                                                         d.p32a);
         NUM_LAUNCH_CHECK();
     }
-    rc = eng_admit(E, s, d, h, (int32_t)((n + kChunkRows - 1) / kChunkRows), st);
+    d.crows = chunk_rows(n);
+    rc = eng_admit(E, s, d, h, (int32_t)((n + d.crows - 1) / d.crows), st);
     if (rc) return rc;
     *slot_out = s;
     return NUM_OK;
@@ -1167,7 +1220,8 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
         mode == 1 ? d.x32a : nullptr, mode == 1 ? d.p32a : nullptr);
     NUM_LAUNCH_CHECK();
     SysState h = initial_state(tol, max_iter, mode, delta);
-    rc = eng_admit(E, s, d, h, (int32_t)((n + kChunkRows - 1) / kChunkRows), st);
+    d.crows = chunk_rows(n);
+    rc = eng_admit(E, s, d, h, (int32_t)((n + d.crows - 1) / d.crows), st);
     if (rc) return rc;
     *slot_out = s;
     return NUM_OK;
