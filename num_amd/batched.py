@@ -3,9 +3,9 @@
 ``CGEngine`` wraps the native engine (include/num_amd.h, num_engine_*): each
 admitted system keeps its CSR handle, right-hand side, iterate and workspace
 resident in HBM, and every ``step(k)`` runs k CG iterations of all live
-systems with four launches per iteration (row tiles of every system in one
-SpMV launch, one workgroup per system for its scalars, 2048-row chunks for
-the vector updates).  A system that converges leaves the launch lists at the
+systems with two launches per iteration (row tiles of every system in one
+SpMV launch; row chunks of every system for the vector updates, whose last
+chunk per system also finishes that system's scalars).  A system that converges leaves the launch lists at the
 next step; a system admitted between steps joins the running batch.  Each
 system's arithmetic depends only on its own data, so its result is
 bit-identical whether it runs alone or with any other systems.
