@@ -168,3 +168,29 @@ def test_graph_replay_equals_eager(ks, mixed):
         r0, r1 = engines[0].result(s0), engines[1].result(s1)
         assert r0 == r1 and r0["converged"]
     del junk
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_large_system_wide_chunks(ks, mixed):
+    """A system of 819,200 rows gets update chunks wider than 2048 rows (at
+    most 256 chunks per system): its result is converged, meets the FP64
+    true residual, and is bit-identical alone and next to a small system."""
+    from num_amd.batched import CGEngine
+    big, small = ((1024, 800), 0.05, 11), ((40, 30), 0.3, 12)
+
+    def run(specs):
+        eng = CGEngine(max_systems=4)
+        slots = [eng.add_stencil(d, sh, sd, tol=1e-10, max_iter=5000, mixed=mixed)
+                 for d, sh, sd in specs]
+        eng.run(8)
+        return [(eng.result(s), eng.solution(s).clone()) for s in slots]
+
+    (r_alone, x_alone), = run([big])
+    (r_small, _), (r_mix, x_mix) = run([small, big])
+    assert r_alone["converged"] and r_mix["converged"] and r_small["converged"]
+    assert r_alone["iters"] == r_mix["iters"]
+    assert torch.equal(x_alone, x_mix)
+    A = g.stencil(big[0], big[1])
+    b = g.hash_uniform(A.nrows, big[2])
+    true = np.linalg.norm(b - A.to_scipy() @ x_alone.cpu().numpy()) / np.linalg.norm(b)
+    assert true < (1e-10 if mixed else 2e-10)

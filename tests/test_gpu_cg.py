@@ -212,3 +212,33 @@ def test_mixed_paths(ks, orc, case):
     assert rel(xh, x64) < 1e-7
     # the FP32 steps stay close to the FP64 iteration count
     assert r["inner_total"] <= 1.5 * r64["iters"] + 10
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_full_width_update_groups(ks, orc, mixed):
+    """n >= 2^20 rows: the update kernel runs its full 256 workgroups, so the
+    FP64 solver's last-workgroup fold draws 256 tickets per iteration and
+    every workgroup of the mixed SpMV kernel folds 256 partials.  Same
+    iteration count as the oracle (+-2%), the true residual, and bit-identical
+    repeats (the fold's order does not depend on which workgroup is last)."""
+    A = g.stencil((1024, 1024), 0.05)
+    b = np.random.default_rng(3).uniform(-1, 1, A.nrows)
+    dA = _up(ks, A)
+    if mixed:
+        dA.ensure_f32()
+    xs = []
+    for _ in range(2):
+        x = _dev(np.zeros(A.nrows))
+        if mixed:
+            r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+        else:
+            r = ks.cg_solve_local(dA, _dev(b), x, tol=1e-10, max_iter=5000)
+        assert r["converged"] and r["residual"] < 1e-10
+        xs.append(x.cpu().numpy())
+    assert np.array_equal(xs[0], xs[1])
+    true = np.linalg.norm(b - A.to_scipy() @ xs[0]) / np.linalg.norm(b)
+    assert true < (1e-10 if mixed else 2e-10)
+    xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=5000)
+    assert rel(xs[0], xo) < 1e-7
+    if not mixed:
+        assert abs(r["iters"] - ro["iters"]) <= max(1, int(0.02 * ro["iters"]))
