@@ -12,7 +12,10 @@ bit-identical whether it runs alone or with any other systems.
 
 Modes: FP64 (the reference's CG sequence, src/cg_solver.hpp:133-196) or mixed
 (FP32 CG steps with FP64-accumulated dots inside FP64 reliable-update
-refinement; convergence on the FP64 true residual).
+refinement; convergence on the FP64 true residual).  The mixed form refines
+once the FP32 residual has dropped by ``delta`` (default 0.003: on the
+4096-system batch 80.8k solves/s against 73.1k with 0.1, since every
+refinement costs its system an FP64 iteration).
 
 Systems come either as a CrsMatrix with caller-owned b / x (``add``) or as a
 Dirichlet stencil assembled on the device in one buffer (``add_stencil``,
@@ -84,7 +87,7 @@ class CGEngine:
         return int(_lib.load().num_engine_work_bytes(int(n), 1 if mixed else 0))
 
     def add(self, A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000,
-            mixed: bool = False, delta: float = 0.1, work=None) -> int:
+            mixed: bool = False, delta: float = 0.003, work=None) -> int:
         """Admit A x = b (x holds x0, receives the solution).  Returns the slot."""
         b = _dev_tensor(b, "b", 1)
         x = _dev_tensor(x, "x", 1)
@@ -111,7 +114,7 @@ class CGEngine:
         return b
 
     def add_stencil(self, dims, shift: float, seed: int, tol: float = 1e-10,
-                    max_iter: int = 1000, mixed: bool = False, delta: float = 0.1,
+                    max_iter: int = 1000, mixed: bool = False, delta: float = 0.003,
                     buf=None, nbytes: int | None = None) -> int:
         """Admit the Dirichlet stencil system on grid ``dims`` (x fastest) with
         diagonal shift ``shift`` and b = generators.hash_uniform(n, seed),
@@ -202,7 +205,7 @@ class CGEngine:
 
 
 def solve_batch(systems, tol: float = 1e-10, max_iter: int = 1000, mixed: bool = False,
-                delta: float = 0.1, iters_per_step: int = 16, engine: CGEngine | None = None):
+                delta: float = 0.003, iters_per_step: int = 16, engine: CGEngine | None = None):
     """Solve every (A, b, x) of ``systems`` together; x is overwritten.
     Returns the per-system result dicts in input order."""
     eng = engine or CGEngine(max_systems=max(1, len(systems)))

@@ -405,7 +405,7 @@ def cg_solve_local(A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000)
 
 
 def cg_solve_mixed(A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 100000,
-                   delta: float = 0.1) -> dict:
+                   delta: float = 0.01) -> dict:
     """FP32-inner / FP64-refined CG (no reference counterpart).
 
     CG steps run in FP32 (FP32 matrix copy and vectors, FP64-accumulated

@@ -69,11 +69,16 @@ class CGServer:
     ``timeout_s``: a request still running that long after admission is
     cancelled (its slot and memory are freed) and reported with
     ``result["timeout"] = True``.  A request larger than the whole budget is
-    rejected at once (``result["rejected"] = True``)."""
+    rejected at once (``result["rejected"] = True``).  ``delta``: refine once
+    the FP32 residual has dropped by this factor (0.003 measured best for
+    the served stream: 3.0k solves/s at lambda = 10k against 2.7k with 0.1,
+    as each refinement costs an FP64 iteration of its system); ``nt32``:
+    stream the FP32 matrix copies non-temporally (measured slower on the
+    served stream: 2.68k against 2.74k solves/s)."""
 
     def __init__(self, budget_bytes: float = 200e9, max_systems: int = 4096,
                  mixed: bool = True, iters_per_step: int = 8, max_iter: int = 20000,
-                 delta: float = 0.1, nt64: bool = True, nt32: bool = False,
+                 delta: float = 0.003, nt64: bool = True, nt32: bool = False,
                  timeout_s: float | None = None, admit_per_pump: int = 32):
         self.budget = float(budget_bytes)
         self.in_use = 0.0
