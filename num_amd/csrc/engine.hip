@@ -56,6 +56,9 @@ enum : int32_t {
     PH_FIN = 6,     // mixed, finished: x = x64 + x32 still to be folded
 };
 
+// 16-byte vector for non-temporal float4 stores
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 constexpr int kEngNpt = 7;        // LDS entries per thread of the engine's tile kernel
 constexpr int kChunkRows = 2048;  // rows per KC workgroup (256 threads x 8), at least
 constexpr int kMaxChunks = 256;   // chunks per system, at most (each sums the tile partials)
@@ -343,6 +346,75 @@ __device__ __forceinline__ void chunk_update(const Chunk& e, const T* __restrict
     }
 }
 
+// chunk_update for the FP32 step over float4 groups (row0 is a multiple of
+// kRedBlock, the vectors are 256-byte aligned); the rows of a last partial
+// group go to thread 0 after its groups.
+__device__ __forceinline__ void chunk_update_f4(const Chunk& e, const float* __restrict__ p,
+                                                const float* __restrict__ q,
+                                                float* __restrict__ x, float* __restrict__ r,
+                                                double rr, const double* __restrict__ pa,
+                                                double* red, double& s_alpha, int& s_brk,
+                                                double& acc)
+{
+    constexpr int kPre = 2;
+    const int t = threadIdx.x;
+    const int g0 = e.row0 >> 2, g1 = e.row1 >> 2;  // whole float4 groups [g0, g1)
+    const float4* __restrict__ p4 = reinterpret_cast<const float4*>(p);
+    const float4* __restrict__ q4 = reinterpret_cast<const float4*>(q);
+    float4* __restrict__ x4 = reinterpret_cast<float4*>(x);
+    float4* __restrict__ r4 = reinterpret_cast<float4*>(r);
+    float4 pv[kPre], qv[kPre], xv[kPre], rv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int i = g0 + t + u * kRedBlock;
+        if (i < g1) {
+            pv[u] = p4[i];
+            qv[u] = q4[i];
+            xv[u] = x4[i];
+            rv[u] = r4[i];
+        }
+    }
+    const double pq = sum_partials_block<kRedBlock, 32>(pa + e.b0, e.nb, red);
+    if (t == 0) {
+        const int brk = !(fabs(pq) >= 1e-300);  // src/cg_solver.hpp:84-85
+        s_brk = brk;
+        s_alpha = brk ? 0.0 : rr / pq;
+    }
+    __syncthreads();
+    if (s_brk) return;
+    const float al = (float)s_alpha, nal = -al;
+    auto upd = [&](int i, const float4& pw, const float4& qw, float4 xw, float4 rw) {
+        xw.x = al * pw.x + xw.x;
+        xw.y = al * pw.y + xw.y;
+        xw.z = al * pw.z + xw.z;
+        xw.w = al * pw.w + xw.w;
+        rw.x = nal * qw.x + rw.x;
+        rw.y = nal * qw.y + rw.y;
+        rw.z = nal * qw.z + rw.z;
+        rw.w = nal * qw.w + rw.w;
+        __builtin_nontemporal_store(f4v{xw.x, xw.y, xw.z, xw.w}, reinterpret_cast<f4v*>(x4 + i));
+        __builtin_nontemporal_store(f4v{rw.x, rw.y, rw.z, rw.w}, reinterpret_cast<f4v*>(r4 + i));
+        acc += (double)rw.x * (double)rw.x;
+        acc += (double)rw.y * (double)rw.y;
+        acc += (double)rw.z * (double)rw.z;
+        acc += (double)rw.w * (double)rw.w;
+    };
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int i = g0 + t + u * kRedBlock;
+        if (i < g1) upd(i, pv[u], qv[u], xv[u], rv[u]);
+    }
+    for (int i = g0 + t + kPre * kRedBlock; i < g1; i += kRedBlock) upd(i, p4[i], q4[i], x4[i], r4[i]);
+    if (t == 0) {
+        for (int i = g1 * 4; i < e.row1; ++i) {
+            x[i] = al * p[i] + x[i];
+            const float ri = nal * q[i] + r[i];
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+    }
+}
+
 // The update kernel, one workgroup per chunk of every live system.
 //  * PH_INIT / PH_REFINE (the SpMV kernel evaluated a residual; no update):
 //    the system's last chunk (ticket) sums its tile partials and finishes the
@@ -432,8 +504,8 @@ __global__ __launch_bounds__(kRedBlock) void eng_kc(const Chunk* __restrict__ ch
         chunk_update<double>(e, psel ? d->p0 : d->p1, d->Ap, d->x, d->r, rr, pa, red, s_alpha,
                              s_brk, acc);
     } else {
-        chunk_update<float>(e, psel ? d->p32a : d->p32b, d->q32, xsel ? d->x32b : d->x32a,
-                            d->r32, rr, pa, red, s_alpha, s_brk, acc);
+        chunk_update_f4(e, psel ? d->p32a : d->p32b, d->q32, xsel ? d->x32b : d->x32a, d->r32,
+                        rr, pa, red, s_alpha, s_brk, acc);
     }
     brk = s_brk != 0;
     alpha = s_alpha;
