@@ -560,10 +560,12 @@ __device__ __forceinline__ void mix_after_kc(double rrn, MixState& s)
 }
 
 // FP32 step gather: p_new[c] = 1*r32[c] + beta*p_old[c].  Its hook folds the
-// update kernel's partials into the state read from `in` (every workgroup
-// alike, through LDS as in GatherCg; workgroup 0 writes the result to `out`)
-// and walks only when the result is an FP32 step (not done, no refinement
-// due).  `done`, `ru` and `xsel` tell the kernel what to do otherwise.
+// update kernel's partials into the state read from `in`: every thread sums
+// the <= 256 partials (one barrier) and applies mix_fold_view to the record,
+// and the results move to scalar registers; workgroup 0 also writes the
+// folded record to `out` (publish_state).  It walks only when the result is
+// an FP32 step (not done, no refinement due); `done`, `ru` and `xsel` tell
+// the kernel what to do otherwise.
 struct GatherMix {
     const float* __restrict__ r;
     const float* __restrict__ p0;
