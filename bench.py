@@ -9,9 +9,12 @@ nnz = 49,324,032), inputs resident in HBM.  Prints ONE JSON line:
   cpu_baseline = the C oracle's SpMV (OpenMP, host cores) on a bounded sample
   cg           = plain FP64 CG on the same matrix to 1e-10: solves/s, p50/p99
   spmv_slab_r3of8 / spmv_zipf4m = the slab-share and Zipf SpMV
+  batch_spmv_lap100_k128 = the multi-RHS SpMV at the reference harness's
+                 batch configuration (3-D Laplacian N = 1,000,000, k = 128)
   cg_mixed_diff128 / cg_batched_4096 / cg_served = the three solver forms
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu]
-                       [--no-extra] [--no-solvers] [--served-seconds S]
+                       [--no-extra | --extras slab,zipf,batch] [--no-solvers]
+                       [--served-seconds S]
 """
 from __future__ import annotations
 
@@ -138,6 +141,29 @@ def zipf_bench(ks, steps):
     return {"n": Z.nrows, "nnz": Z.nnz, "max_row": int(np.diff(Z.rowptr).max()),
             "kernel_us": round(avg * 1e6, 2), "GBps": round(B / avg / 1e9, 1),
             "frac": round(B / avg / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def batch_bench(ks, steps, L=100, k=128):
+    """Multi-RHS SpMV Y = A X at python/benchmark_spmv.py's batch setting
+    (--N 1000000 --batch 128: the 3-D Laplacian 100^3, X (N, 128) standard
+    normal, Y (N, 128)).  Bytes 12 nnz + 4(n+1) + 8k(ncols + nrows); gflops
+    2 nnz k / t as benchmark_spmv.py:107-113.  Column 0 of Y must equal the
+    single-vector SpMV of X[:, 0] bit for bit (the same row order)."""
+    import torch
+    from num_amd import generators as g
+    A = g.laplacian_3d(L)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    gen = torch.Generator(device="cuda").manual_seed(42)
+    X = torch.randn((A.ncols, k), dtype=torch.float64, device="cuda", generator=gen)
+    Y = torch.zeros((A.nrows, k), dtype=torch.float64, device="cuda")
+    avg = time_launches(lambda: ks.batch_spmv(dA, X, Y, sync=False), steps, 5)
+    B = 12 * A.nnz + 4 * (A.nrows + 1) + 8 * k * (A.ncols + A.nrows)
+    y0 = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+    ks.spmv(dA, X[:, 0].contiguous(), y0)
+    return {"n": A.nrows, "nnz": A.nnz, "k": k, "kernel_us": round(avg * 1e6, 2),
+            "GBps": round(B / avg / 1e9, 1), "frac": round(B / avg / 1e9 / HBM_PEAK_GBS, 4),
+            "gflops": round(2 * A.nnz * k / avg / 1e9, 1), "bytes": B,
+            "col0_equals_spmv": bool(torch.equal(Y[:, 0], y0))}
 
 
 def slab_bench(ks, A, x, y_ref, steps):
@@ -320,7 +346,9 @@ def main():
                     help="duration of the lambda = 10k served stream (60 = full; it then "
                          "drains for a few minutes)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the slab-share and Zipf SpMV measurements")
+                    help="skip the slab-share, Zipf and multi-RHS SpMV measurements")
+    ap.add_argument("--extras", default="slab,zipf,batch",
+                    help="which of those to measure (comma list)")
     ap.add_argument("--cg-solves", type=int, default=5)
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB copy used to scale PMC byte counters")
@@ -367,12 +395,17 @@ def main():
         cb, y_ref = cpu_baseline(A, x)
         out["cpu_baseline"] = cb
         out["parity_max_abs"] = float(np.max(np.abs(y_gpu - y_ref)))
-    if not args.no_extra:
+    extras = set() if args.no_extra else set(args.extras.split(","))
+    if "slab" in extras:
         log("bench: slab share (rank 3 of 8)")
         out["spmv_slab_r3of8"] = slab_bench(ks, A, x, y_ref if not args.no_cpu else None,
                                             args.steps)
+    if "zipf" in extras:
         log("bench: Zipf n=4M")
         out["spmv_zipf4m"] = zipf_bench(ks, max(10, args.steps // 4))
+    if "batch" in extras:
+        log("bench: multi-RHS SpMV, 100^3, k = 128")
+        out["batch_spmv_lap100_k128"] = batch_bench(ks, max(10, args.steps // 4))
     if not args.no_cg:
         log("bench: CG 1e-10")
         out["cg"] = cg_bench(ks, A, dA, args.cg_solves)

@@ -1,5 +1,6 @@
 // spmv.hip -- y = A x (FP64 / FP32), row-range SpMV, multi-RHS SpMV, halo
 // pack/scatter and the host round-trip helpers.
+#include <algorithm>
 #include <vector>
 
 #include "spmv_launch.hpp"
@@ -30,36 +31,277 @@ template int launch_spmv<float>(const num_csr*, const Schedule&, const float*, c
 // ---------------------------------------------------------------------------
 // Multi-RHS: Y(i, b) = sum_j A_ij X(c_j, b), row-major X (ncols, k), Y (nrows, k)
 // (src/batch_spmv_kernel.hpp:94-102 order: sequential over the row's
-// non-zeros from 0.0).  KP lanes per row, lane b covers rhs b, b+KP, ...; the
-// KP lanes of a row read the same value/column (one broadcast load) and
-// consecutive X entries (one coalesced load per KP lanes).
+// non-zeros from 0.0, one rounded product and one rounded sum per entry).
+//
+// Two kernels, both memory-bound on X and Y (8k bytes each per row):
+// * k < 64, batch_spmv_kernel: a group of G lanes owns a row; lane l covers
+//   the V consecutive right-hand sides b = l*V + m*G*V (m = 0, 1, ...), so one
+//   X row is read as G*V*8 contiguous bytes (V = 2: 16-byte loads).  The
+//   row's entries are taken U at a time: U column/value loads (the same
+//   address for the G lanes of a group, one request), then U independent X
+//   loads, all issued before the first sum.
+// * k >= 64, batch_wave_kernel: a wave per row, R rows per wave, the rows'
+//   entries held in registers and the X loads of row r + 1 issued before row
+//   r is summed (below).
+// In both the sums run in entry order.
+//
+// X rows are re-read by every row that references them (7 times for a 3-D
+// stencil), and at k = 128 one X row is 1 KB, so the reuse must come from the
+// XCD's own L2: row blocks are dealt to the 8 XCDs in chunks of consecutive
+// blocks (chunk_tile), each chunk spanning about kBatchChunkRows rows, so the
+// rows a block shares X rows with (the neighbouring rows and, for a 2-D
+// grid, the next grid line) mostly run on the same XCD.  On a 3-D grid the
+// next plane's rows (10,000 X rows = 10 MB at k = 128 on 100^3) do not fit in
+// the 4 MB L2 and come from the Infinity Cache.  Y is written once and never
+// read here: non-temporal stores.
 // ---------------------------------------------------------------------------
-template <int KP>
-__global__ __launch_bounds__(256) void batch_spmv_kernel(
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-    const double* __restrict__ vals, const double* __restrict__ X, double* __restrict__ Y,
-    int nrows, int k)
+constexpr int kBatchBS = 256;
+constexpr int kBatchU = 8;
+constexpr int kBatchChunkRows = 2048;
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <int V> struct BatchVec;
+template <> struct BatchVec<1> {
+    typedef double T;
+    static __device__ __forceinline__ T load(const double* p) { return *p; }
+    static __device__ __forceinline__ double get(const T& v, int) { return v; }
+    static __device__ __forceinline__ void store_nt(double* p, const double* a)
+    {
+        __builtin_nontemporal_store(a[0], p);
+    }
+};
+template <> struct BatchVec<2> {
+    typedef d2v T;
+    static __device__ __forceinline__ T load(const double* p) { return *(const d2v*)p; }
+    static __device__ __forceinline__ double get(const T& v, int i) { return v[i]; }
+    static __device__ __forceinline__ void store_nt(double* p, const double* a)
+    {
+        const d2v o = {a[0], a[1]};
+        __builtin_nontemporal_store(o, (d2v*)p);
+    }
+};
+
+// One row's Y(row, b..) for the right-hand sides b = lane*V + m*G*V, entries
+// read straight from colind / vals, kBatchU at a time.
+template <int G, int V>
+__device__ __forceinline__ void batch_row_direct(const int32_t* __restrict__ colind,
+                                                 const double* __restrict__ vals,
+                                                 const double* __restrict__ X,
+                                                 double* __restrict__ Y, int row, int js, int je,
+                                                 int k, int lane)
 {
-    constexpr int RPB = 256 / KP;
-    const int row = blockIdx.x * RPB + threadIdx.x / KP;
-    const int lane = threadIdx.x % KP;
-    if (row >= nrows) return;
-    const int js = rowptr[row], je = rowptr[row + 1];
-    for (int b = lane; b < k; b += KP) {
-        double acc = 0.0;
-        for (int j = js; j < je; ++j) acc += vals[j] * X[(int64_t)colind[j] * k + b];
-        Y[(int64_t)row * k + b] = acc;
+    typedef BatchVec<V> Vec;
+    for (int b = lane * V; b < k; b += G * V) {
+        double acc[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) acc[i] = 0.0;
+        for (int j = js; j < je; j += kBatchU) {
+            int c[kBatchU];
+            double v[kBatchU];
+            typename Vec::T xv[kBatchU];
+#pragma unroll
+            for (int u = 0; u < kBatchU; ++u) {
+                if (j + u < je) {
+                    c[u] = colind[j + u];
+                    v[u] = vals[j + u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBatchU; ++u)
+                if (j + u < je) xv[u] = Vec::load(X + (int64_t)c[u] * k + b);
+#pragma unroll
+            for (int u = 0; u < kBatchU; ++u) {
+                if (j + u < je) {
+#pragma unroll
+                    for (int i = 0; i < V; ++i) acc[i] = acc[i] + v[u] * Vec::get(xv[u], i);
+                }
+            }
+        }
+        Vec::store_nt(Y + (int64_t)row * k + b, acc);
     }
 }
 
-template <int KP>
+template <int G, int V>
+__global__ __launch_bounds__(kBatchBS) void batch_spmv_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ vals, const double* __restrict__ X, double* __restrict__ Y,
+    int nrows, int k, int nblocks, int chunk)
+{
+    constexpr int RPB = kBatchBS / G;  // rows per workgroup
+    const int t = chunk_tile(blockIdx.x, nblocks, chunk);
+    if (t < 0) return;
+    const int row = t * RPB + (int)threadIdx.x / G;
+    if (row >= nrows) return;
+    batch_row_direct<G, V>(colind, vals, X, Y, row, rowptr[row], rowptr[row + 1], k,
+                           (int)threadIdx.x % G);
+}
+
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// The pipelined walk of batch_wave_kernel.  FULL: all R rows exist and every
+// lane has a right-hand side (k a multiple of 64 * V), so no store is guarded;
+// a guarded store lets the compiler sink the loads feeding it into the guard,
+// behind the next row's loads, which undoes the pipelining.
+template <int V, int R, bool FULL>
+__device__ __forceinline__ void batch_wave_rows(const double* __restrict__ X,
+                                                double* __restrict__ Y, int rpl, int e0,
+                                                int nr, int cl, double vl, int r0, int k,
+                                                int lane)
+{
+    typedef BatchVec<V> Vec;
+    for (int cb = 0; cb < k; cb += kWave * V) {
+        const int b = cb + lane * V;
+        const bool act = FULL || b < k;
+        const int bx = act ? b : 0;  // idle lanes load an in-bounds column
+        typename Vec::T xb[2][kBatchU];
+#pragma unroll
+        for (int r = 0; r <= R; ++r) {
+            if (r < R) {  // issue row r (clamped to the last row) into buffer r & 1
+                const int rr = FULL ? r : min(r, nr - 1);
+                const int s = readlane_i(rpl, rr) - e0;
+                const int len = readlane_i(rpl, rr + 1) - e0 - s;
+#pragma unroll
+                for (int u = 0; u < kBatchU; ++u) {
+                    const int e = u < len ? s + u : max(s + len - 1, 0);
+                    const int c = readlane_i(cl, e);
+                    xb[r & 1][u] = Vec::load(X + (int64_t)c * k + bx);
+                }
+            }
+            if (r > 0 && (FULL || r - 1 < nr)) {  // sum row r - 1
+                const int q = r - 1;
+                const int s = readlane_i(rpl, q) - e0;
+                const int len = readlane_i(rpl, q + 1) - e0 - s;
+                double acc[V];
+#pragma unroll
+                for (int i = 0; i < V; ++i) acc[i] = 0.0;
+                // Branch-free: every slot's sum is formed and kept only for
+                // u < len, so no load is sunk into a branch and waited for.
+#pragma unroll
+                for (int u = 0; u < kBatchU; ++u) {
+                    const double v = readlane_d(vl, min(s + u, kWave - 1));
+#pragma unroll
+                    for (int i = 0; i < V; ++i) {
+                        const double nxt = acc[i] + v * Vec::get(xb[q & 1][u], i);
+                        acc[i] = u < len ? nxt : acc[i];
+                    }
+                }
+                if (act) Vec::store_nt(Y + (int64_t)(r0 + q) * k + b, acc);
+            }
+        }
+    }
+}
+
+// Wide k, one wave (64 lanes) per row: each wave takes R consecutive rows.
+// Their bounds are one load (lane i holds rowptr[r0 + i]); when the R rows
+// hold at most 64 entries and none more than kBatchU, their columns and
+// values are one more load (lane e holds entry e) and every X load after that
+// takes its column from a register (readlane), with no index load in between.
+// The X loads of row r + 1 are issued before row r is summed (two register
+// buffers), so each wave keeps a row's X rows in flight while it adds.  Every
+// load slot is filled (a short row repeats its last column, the last row of
+// the matrix is loaded again), so the compiler's wait counts stay exact.
+// Rows outside that shape take batch_row_direct.
+template <int V, int R>
+__global__ __launch_bounds__(kBatchBS) void batch_wave_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ vals, const double* __restrict__ X, double* __restrict__ Y,
+    int nrows, int k, int nblocks, int chunk)
+{
+    static_assert(R * kBatchU <= kWave, "entries of R rows must fit one wave");
+    const int t = chunk_tile(blockIdx.x, nblocks, chunk);
+    if (t < 0) return;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int r0 = (t * (kBatchBS / kWave) + (int)threadIdx.x / kWave) * R;
+    if (r0 >= nrows) return;
+    const int nr = min(R, nrows - r0);
+    const int rpl = rowptr[r0 + min(lane, nr)];
+    const int e0 = readlane_i(rpl, 0);
+    int maxlen = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (r < nr) maxlen = max(maxlen, readlane_i(rpl, r + 1) - readlane_i(rpl, r));
+    if (maxlen > kBatchU) {
+        for (int r = 0; r < nr; ++r)
+            batch_row_direct<kWave, V>(colind, vals, X, Y, r0 + r, readlane_i(rpl, r),
+                                       readlane_i(rpl, r + 1), k, lane);
+        return;
+    }
+    // nr <= R rows of <= kBatchU entries: at most R * kBatchU <= 64 entries.
+    const int ne = readlane_i(rpl, nr) - e0;
+    int cl = 0;
+    double vl = 0.0;
+    if (lane < ne) {
+        cl = colind[e0 + lane];
+        vl = vals[e0 + lane];
+    }
+    if (nr == R && k % (kWave * V) == 0)
+        batch_wave_rows<V, R, true>(X, Y, rpl, e0, nr, cl, vl, r0, k, lane);
+    else
+        batch_wave_rows<V, R, false>(X, Y, rpl, e0, nr, cl, vl, r0, k, lane);
+}
+
+template <int G, int V>
 static void launch_batch(const num_csr* A, const double* X, double* Y, int k,
                          hipStream_t stream)
 {
-    constexpr int RPB = 256 / KP;
-    const int grid = (A->nrows + RPB - 1) / RPB;
-    batch_spmv_kernel<KP><<<grid, 256, 0, stream>>>(A->rowptr, A->colind, A->vals, X, Y,
-                                                    A->nrows, k);
+    constexpr int RPB = kBatchBS / G;
+    const int nblocks = (A->nrows + RPB - 1) / RPB;
+    const int chunk = nblocks >= 8 * 64 ? std::max(1, kBatchChunkRows / RPB) : 1;
+    const int grid = chunk_grid(nblocks, chunk);
+    batch_spmv_kernel<G, V><<<grid, kBatchBS, 0, stream>>>(A->rowptr, A->colind, A->vals, X,
+                                                           Y, A->nrows, k, nblocks, chunk);
+}
+
+template <int V, int R>
+static void launch_batch_wave(const num_csr* A, const double* X, double* Y, int k,
+                              hipStream_t stream)
+{
+    constexpr int RPB = (kBatchBS / kWave) * R;
+    const int nblocks = (A->nrows + RPB - 1) / RPB;
+    const int chunk = nblocks >= 8 * 64 ? std::max(1, kBatchChunkRows / RPB) : 1;
+    const int grid = chunk_grid(nblocks, chunk);
+    batch_wave_kernel<V, R><<<grid, kBatchBS, 0, stream>>>(A->rowptr, A->colind, A->vals, X,
+                                                           Y, A->nrows, k, nblocks, chunk);
+}
+
+// k >= kBatchWaveK: a wave per row (16-byte X loads when k > 64 is even and
+// X, Y are 16-byte aligned).  Smaller k: G lanes per row, G the smallest
+// power of two with G*V >= k (V = 2 when k is even and aligned).
+constexpr int kBatchWaveK = 64;
+constexpr int kBatchWaveRows = 4;
+static void dispatch_batch(const num_csr* A, const double* X, double* Y, int k,
+                           hipStream_t st)
+{
+    const bool v2 = (k % 2 == 0) && ((uintptr_t)X % 16 == 0) && ((uintptr_t)Y % 16 == 0);
+    if (k >= kBatchWaveK) {
+        if (v2 && k > kWave) launch_batch_wave<2, kBatchWaveRows>(A, X, Y, k, st);
+        else launch_batch_wave<1, kBatchWaveRows>(A, X, Y, k, st);
+        return;
+    }
+    const int w = v2 ? k / 2 : k;
+    if (v2) {
+        if (w <= 1) launch_batch<1, 2>(A, X, Y, k, st);
+        else if (w <= 2) launch_batch<2, 2>(A, X, Y, k, st);
+        else if (w <= 4) launch_batch<4, 2>(A, X, Y, k, st);
+        else if (w <= 8) launch_batch<8, 2>(A, X, Y, k, st);
+        else if (w <= 16) launch_batch<16, 2>(A, X, Y, k, st);
+        else launch_batch<32, 2>(A, X, Y, k, st);
+    } else {
+        if (w <= 2) launch_batch<2, 1>(A, X, Y, k, st);
+        else if (w <= 4) launch_batch<4, 1>(A, X, Y, k, st);
+        else if (w <= 8) launch_batch<8, 1>(A, X, Y, k, st);
+        else if (w <= 16) launch_batch<16, 1>(A, X, Y, k, st);
+        else if (w <= 32) launch_batch<32, 1>(A, X, Y, k, st);
+        else launch_batch<64, 1>(A, X, Y, k, st);
+    }
 }
 
 // One row of the Dirichlet stencil on an nx x ny x nz grid (x fastest):
@@ -168,12 +410,7 @@ int num_batch_spmv(const num_csr* A, const double* d_X, double* d_Y, int32_t k, 
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     if (k == 1) return launch_spmv<double>(A, A->sched, A->vals, d_X, d_Y, st);
-    if (k <= 2) launch_batch<2>(A, d_X, d_Y, k, st);
-    else if (k <= 4) launch_batch<4>(A, d_X, d_Y, k, st);
-    else if (k <= 8) launch_batch<8>(A, d_X, d_Y, k, st);
-    else if (k <= 16) launch_batch<16>(A, d_X, d_Y, k, st);
-    else if (k <= 32) launch_batch<32>(A, d_X, d_Y, k, st);
-    else launch_batch<64>(A, d_X, d_Y, k, st);
+    dispatch_batch(A, d_X, d_Y, k, st);
     NUM_LAUNCH_CHECK();
     return NUM_OK;
 }

@@ -214,6 +214,68 @@ def test_batch_spmv(ks, orc, k):
     assert np.array_equal(Y.cpu().numpy()[:, 0], y1)
 
 
+@pytest.mark.parametrize("k", [2, 5, 8, 64, 128])
+def test_batch_spmv_irregular(ks, orc, k):
+    # Zipf rows of 1..300 entries: rows longer than one 8-entry batch, and
+    # lane groups of one wave walking rows of different lengths.
+    A = g.zipf_matrix(4000, 2.0, 300, seed=11)
+    dA = _up(ks, A)
+    X = np.random.default_rng(100 + k).standard_normal((A.ncols, k))
+    Y = torch.zeros((A.nrows, k), dtype=torch.float64, device="cuda")
+    ks.batch_spmv(dA, _dev(X), Y)
+    assert np.array_equal(Y.cpu().numpy(), orc.batch_spmv(A.rowptr, A.colind, A.values, X))
+
+
+@pytest.mark.parametrize("k", [2, 16, 37, 128])
+def test_batch_spmv_xcd_chunks(ks, orc, k):
+    # 40^3 = 64,000 rows: enough row blocks for the chunked XCD order at
+    # k >= 16, and a partial last chunk.
+    A = g.laplacian_3d(40)
+    dA = _up(ks, A)
+    X = np.random.default_rng(200 + k).uniform(-1, 1, (A.ncols, k))
+    Y = torch.zeros((A.nrows, k), dtype=torch.float64, device="cuda")
+    ks.batch_spmv(dA, _dev(X), Y)
+    assert np.array_equal(Y.cpu().numpy(), orc.batch_spmv(A.rowptr, A.colind, A.values, X))
+
+
+@pytest.mark.parametrize("k", [64, 65, 128, 192, 256])
+def test_batch_spmv_wide(ks, orc, k):
+    # A wave per row: 37^2 = 1369 rows leaves a last wave with fewer rows
+    # than the others; k = 192 and 256 take two column passes (the first with
+    # idle lanes in its second pass, the second without); k = 65 the 8-byte
+    # form.  The appended Zipf rows (up to 40 entries) take the direct walk.
+    A2 = g.laplacian_2d(37)
+    Z = g.zipf_matrix(A2.nrows, 2.0, 40, seed=12)
+    rp = np.concatenate([A2.rowptr, A2.rowptr[-1] + Z.rowptr[1:]]).astype(np.int32)
+    ci = np.concatenate([A2.colind, Z.colind]).astype(np.int32)
+    va = np.concatenate([A2.values, Z.values])
+    n = 2 * A2.nrows - 1
+    rp = rp[: n + 1]
+    ci, va = ci[: rp[-1]], va[: rp[-1]]
+    dA = ks.upload_csr(rp, ci, va, n, A2.ncols)
+    X = np.random.default_rng(300 + k).standard_normal((A2.ncols, k))
+    Y = torch.zeros((n, k), dtype=torch.float64, device="cuda")
+    ks.batch_spmv(dA, _dev(X), Y)
+    assert np.array_equal(Y.cpu().numpy(), orc.batch_spmv(rp, ci, va, X))
+
+
+def test_batch_spmv_8byte_aligned(ks, orc):
+    # X and Y one element into their buffers: the 8-byte-load form for even k.
+    A = g.laplacian_2d(30)
+    dA = _up(ks, A)
+    k = 6
+    X = np.random.default_rng(5).standard_normal((A.ncols, k))
+    xb = torch.zeros(A.ncols * k + 1, dtype=torch.float64, device="cuda")
+    yb = torch.zeros(A.nrows * k + 1, dtype=torch.float64, device="cuda")
+    Xd = xb[1:].view(A.ncols, k)
+    Xd.copy_(_dev(X))
+    Yd = yb[1:].view(A.nrows, k)
+    assert Xd.data_ptr() % 16 == 8 and Yd.data_ptr() % 16 == 8
+    ks.batch_spmv(dA, Xd, Yd)
+    assert np.array_equal(Yd.cpu().numpy(), orc.batch_spmv(A.rowptr, A.colind, A.values, X))
+    assert yb[0].item() == 0.0
+
+
 def test_batch_shape_errors(ks):
     A = g.laplacian_2d(4)
     dA = _up(ks, A)
