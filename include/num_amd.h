@@ -83,6 +83,21 @@ int num_csr_attach_f32(num_csr* A, const float* d_vals32);
  * in int16 (stencils, banded matrices); otherwise the FP32 path keeps the
  * int32 columns.  Synchronises the stream. */
 int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* attached);
+/* Build a delta-coded copy of the column ids into d_codes [nnz + 2] (uint16,
+ * 4-byte aligned; the kernels read the codes in 4-byte pairs) for
+ * the FP64 row-tile kernels (no reference counterpart: a data format of this
+ * build; the reference reads int32 colind, crs_matrix.hpp:22-34).  Entry j
+ * of a row holds its column minus the previous entry's column; a row's first
+ * entry holds its column minus its tile's base column.  Attached
+ * (*attached = 1) only when every row holds <= 32 entries in ascending
+ * column order and every code fits in 16 bits; then the FP64 SpMV and CG on
+ * the whole-matrix schedule read 2 index bytes per entry instead of 4, with
+ * the same columns in the same order (bit-identical results).  The buffer
+ * stays the caller's and must outlive A; num_csr_set_schedule re-codes it
+ * (and detaches it if the new tiles do not fit).  Synchronises the stream. */
+int num_csr_attach_d16(num_csr* A, uint16_t* d_codes, void* stream, int32_t* attached);
+/* *delta16 = 1 if a delta-coded index is attached (num_csr_attach_d16). */
+int num_csr_index_info(const num_csr* A, int32_t* delta16);
 int num_csr_destroy(num_csr* A);
 /* nrows / ncols / nnz read-only attributes (src/bindings.cpp:79-88). */
 int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz);

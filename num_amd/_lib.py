@@ -41,6 +41,8 @@ SIGNATURES = {
     "num_csr_create_host": (C.c_int, [vp, vp, vp, i32, i32, C.POINTER(vp)]),
     "num_csr_attach_f32": (C.c_int, [vp, vp]),
     "num_csr_attach_rel16": (C.c_int, [vp, vp, vp, pi32]),
+    "num_csr_attach_d16": (C.c_int, [vp, vp, vp, pi32]),
+    "num_csr_index_info": (C.c_int, [vp, pi32]),
     "num_csr_destroy": (C.c_int, [vp]),
     "num_csr_shape": (C.c_int, [vp, pi32, pi32, pi64]),
     "num_csr_set_schedule": (C.c_int, [vp, i32, i32, i32, i32, i32]),

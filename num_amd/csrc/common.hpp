@@ -68,7 +68,7 @@ struct TileEntry {
     int32_t row;    // first row
     int32_t wide;   // walk mode: kWalkShort, kWalkMedium or kWalkLong
     int32_t nz0;    // rowptr[row]
-    int32_t pad;
+    int32_t base;   // delta-coded index only: the tile's base column, else 0
 };
 // Row classes.  A row of up to kShortRow entries is walked by one thread; a
 // longer row of a staged tile by a whole wave; a row of more than kLongRow

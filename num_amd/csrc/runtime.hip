@@ -196,6 +196,66 @@ __global__ void rel16_kernel(int32_t nrows, const int32_t* __restrict__ rowptr,
     if (flag) atomicOr(bad, 1);
 }
 
+// Delta codes, one workgroup per tile (rows_per_block <= blockDim): the
+// tile's base column is the smallest first column of its rows; each row's
+// first entry is coded against the base, every later entry against the
+// previous entry.  *bad = 1 if a code is negative (columns out of order) or
+// exceeds 16 bits.
+__global__ void d16_build_kernel(const int32_t* __restrict__ rowptr,
+                                 const int32_t* __restrict__ colind, TileEntry* __restrict__ tiles,
+                                 uint16_t* __restrict__ code, int32_t* __restrict__ bad)
+{
+    __shared__ int base;
+    const int r0 = tiles[blockIdx.x].row, r1 = tiles[blockIdx.x + 1].row;
+    if (threadIdx.x == 0) base = INT32_MAX;
+    __syncthreads();
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+        const int js = rowptr[r], je = rowptr[r + 1];
+        if (je > js) atomicMin(&base, colind[js]);
+    }
+    __syncthreads();
+    const int b = base == INT32_MAX ? 0 : base;
+    int flag = 0;
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+        int prev = b;
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            const int c = colind[j];
+            const int d = c - prev;
+            if (d < 0 || d > 65535) flag = 1;
+            code[j] = (uint16_t)d;
+            prev = c;
+        }
+    }
+    if (flag) atomicOr(bad, 1);
+    if (threadIdx.x == 0) tiles[blockIdx.x].base = b;
+}
+
+// (Re)code A->d16_buf against A->sched; attaches on success, detaches
+// otherwise.  Synchronises `st`.
+static int build_d16(num_csr* A, hipStream_t st)
+{
+    A->d16 = nullptr;
+    if (!A->d16_buf || A->nnz == 0 || A->max_row_nnz > kShortRow || A->sched.long_rows != 0 ||
+        A->sched.nblocks == 0 || A->sched.rows_per_block > kSpmvBS)
+        return NUM_OK;
+    int32_t* d_bad = nullptr;
+    NUM_HIP(hipMalloc(&d_bad, sizeof(int32_t)));
+    int32_t h_bad = 0;
+    hipError_t e = hipMemcpyAsync(d_bad, &h_bad, sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        d16_build_kernel<<<A->sched.nblocks, kSpmvBS, 0, st>>>(
+            A->rowptr, A->colind, static_cast<TileEntry*>(A->sched.d_tiles), A->d16_buf, d_bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&h_bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_bad);
+    if (e != hipSuccess) return fail(NUM_EHIP, "delta-code build failed: %s", hipGetErrorString(e));
+    if (h_bad == 0) A->d16 = A->d16_buf;
+    return NUM_OK;
+}
+
 }  // namespace numamd
 
 using namespace numamd;
@@ -342,6 +402,32 @@ int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* at
     return NUM_OK;
 }
 
+int num_csr_attach_d16(num_csr* A, uint16_t* d_codes, void* stream, int32_t* attached)
+{
+    if (!A || !attached) return fail(NUM_EINVAL, "matrix or attached is NULL");
+    *attached = 0;
+    if (!d_codes && A->nnz > 0) return fail(NUM_EINVAL, "d_codes is NULL");
+    if ((uintptr_t)d_codes % 4 != 0) return fail(NUM_EINVAL, "d_codes must be 4-byte aligned");
+    int rc = check_device(A);
+    if (rc) return rc;
+    A->d16_buf = d_codes;
+    rc = build_d16(A, (hipStream_t)stream);
+    if (rc) {
+        A->d16_buf = nullptr;
+        return rc;
+    }
+    if (!A->d16) A->d16_buf = nullptr;
+    *attached = A->d16 ? 1 : 0;
+    return NUM_OK;
+}
+
+int num_csr_index_info(const num_csr* A, int32_t* delta16)
+{
+    if (!A || !delta16) return fail(NUM_EINVAL, "matrix or delta16 is NULL");
+    *delta16 = A->d16 ? 1 : 0;
+    return NUM_OK;
+}
+
 int num_csr_destroy(num_csr* A)
 {
     if (!A) return NUM_OK;
@@ -374,6 +460,9 @@ int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_blo
     int rc = build_schedule(A, 0, A->nrows, rows_per_block, nnz_per_block, chunk, nt64, nt32,
                             A->sched);
     if (rc) return rc;
+    rc = build_d16(A, nullptr);  // the codes depend on the tiles
+    if (rc) return rc;
+    if (!A->d16) A->d16_buf = nullptr;
     std::lock_guard<std::mutex> lk(A->mu);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
     A->range_sched.clear();

@@ -333,6 +333,101 @@ __device__ __forceinline__ void spmv_rel16_run(
     }
 }
 
+// FP64 walk with delta-coded column ids (num_csr_attach_d16): a row's first
+// code is its column minus the tile's base column (TileEntry.base), every
+// later code its column minus the previous entry's, as uint16.  Half the
+// index bytes of int32 (192^3 Laplacian: 98.6 of its 733 MB).  Every row of
+// the matrix holds <= kShortRow entries in ascending column order, so every
+// tile is walked one thread per row: the stage of spmv_tile_run with 2-byte
+// codes, then the running column is rebuilt by integer adds in the walk,
+// entries summed in index order from 0.0 as in spmv_tile_run (bit-identical).
+template <int BS, int NPT>
+constexpr int d16_pairs() { return (NPT + 2) / 2; }  // 2*BS*pairs >= BS*NPT + 1 codes
+
+template <int BS, int NPT, bool NT, class Epi, class Gather>
+__device__ __forceinline__ void spmv_d16_run(
+    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
+    const uint16_t* __restrict__ code, const double* __restrict__ vals, const Gather& x,
+    const Epi& epi, double* __restrict__ lv, uint32_t* __restrict__ lc2, double& ca, double& cb)
+{
+    constexpr int NPP = d16_pairs<BS, NPT>();
+    const int t = threadIdx.x;
+    const int r0 = e0.row, j0 = e0.nz0;
+    const int r1 = e1.row, j1 = e1.nz0;
+    const int nz = j1 - j0, nr = r1 - r0;
+    ca = 0.0;
+    cb = 0.0;
+    // row bounds, branch-free, in flight with the stage loads (spmv_tile_run)
+    const bool active = t < nr;
+    const int tr = min(t, nr);
+    const int rb_raw = rowptr[r0 + tr];
+    const int re_raw = rowptr[r0 + tr + (active ? 1 : 0)];
+    // codes staged as 4-byte pairs from the even entry jb <= j0 (the code
+    // array holds nnz + 2 entries, so the last pair stays inside it): code
+    // of entry j at lc[j - jb], value at lv[j - j0]
+    const int jb = j0 & ~1;
+    const int off = j0 - jb;
+    if (nz > 0) {
+        const int last = j1 - 1;
+        const int lastp = (j1 - 1 - jb) >> 1;
+        const uint32_t* cp = reinterpret_cast<const uint32_t*>(code + jb);
+        double v[NPT];
+        uint32_t c[NPP];
+#pragma unroll
+        for (int s = 0; s < NPT; ++s) v[s] = ld_mat<NT>(vals + min(j0 + t + s * BS, last));
+#pragma unroll
+        for (int s = 0; s < NPP; ++s) c[s] = ld_mat<NT>(cp + min(t + s * BS, lastp));
+        if constexpr (has_hook<Gather>::value) x.issue();
+#pragma unroll
+        for (int s = 0; s < NPT; ++s) {
+            const int k = t + s * BS;
+            if (k < nz) lv[k] = v[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NPP; ++s) {
+            const int k = t + s * BS;
+            if (k <= lastp) lc2[k] = c[s];
+        }
+    } else if constexpr (has_hook<Gather>::value) {
+        x.issue();
+    }
+    __syncthreads();
+    if constexpr (has_hook<Gather>::value) {
+        if (!x.resolve()) return;
+    }
+    const uint16_t* __restrict__ lc = reinterpret_cast<const uint16_t*>(lc2) + off;
+    if (!active) return;
+    const int rb = rb_raw - j0, re = re_raw - j0;
+    double acc = 0.0;
+    int col = e0.base;  // a valid column: the smallest first column of the tile (or 0)
+    int k = rb;
+    for (; k + 7 < re; k += 8) {
+        double vv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            vv[u] = lv[k + u];
+            col += lc[k + u];
+            xv[u] = x(col);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += vv[u] * xv[u];
+    }
+    double vv[7], xv[7];
+#pragma unroll
+    for (int u = 0; u < 7; ++u) {
+        const int kk = k + u;
+        const bool ok = kk < re;
+        const int ks = ok ? kk : 0;  // slot 0 is always inside the LDS array
+        vv[u] = lv[ks];
+        col += ok ? (int)lc[ks] : 0;  // past the row: the last column again
+        xv[u] = x(col);
+    }
+#pragma unroll
+    for (int u = 0; u < 7; ++u)
+        if (k + u < re) acc += vv[u] * xv[u];
+    epi(r0 + t, acc, ca, cb, x);
+}
+
 template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_tile_body(
     int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
@@ -361,6 +456,35 @@ __global__ __launch_bounds__(BS) void spmv_tile_kernel(
     double ca, cb;
     spmv_tile_body<T, BS, NPT, NT, Epi, Gather>(tile, rowptr, colind, vals, x, sched, epi, lv, lc,
                                             ca, cb);
+    if constexpr (Epi::kRed == 1) {
+        const double s = block_sum<BS>(ca, red);
+        if (threadIdx.x == 0) epi.partial_a[tile] = s;
+    } else if constexpr (Epi::kRed == 2) {
+        block_sum2<BS>(ca, cb, red);
+        if (threadIdx.x == 0) {
+            epi.partial_a[tile] = ca;
+            epi.partial_b[tile] = cb;
+        }
+    }
+}
+
+// The same over delta-coded column ids (spmv_d16_run).
+template <int BS, int NPT, bool NT, class Epi, class Gather>
+__global__ __launch_bounds__(BS) void spmv_d16_kernel(
+    const int32_t* __restrict__ rowptr, const uint16_t* __restrict__ code,
+    const double* __restrict__ vals, Gather x, const TileEntry* __restrict__ sched, int ntiles,
+    int chunk, Epi epi)
+{
+    __shared__ double lv[BS * NPT];
+    __shared__ uint32_t lc2[BS * d16_pairs<BS, NPT>()];
+    __shared__ double red[2 * BS / kWave];
+    const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
+    if (tile < 0) return;
+    if (epi.skip()) return;
+    x.prepare();
+    double ca, cb;
+    spmv_d16_run<BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], rowptr, code, vals, x,
+                                           epi, lv, lc2, ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
         if (threadIdx.x == 0) epi.partial_a[tile] = s;

@@ -6,10 +6,25 @@
 
 namespace numamd {
 
+// FP64 values on the whole-matrix schedule take the delta-coded column ids
+// when they are attached (their tile bases live in that schedule) and the
+// matrix streams from HBM (NT): the codes save HBM bytes at the cost of a
+// longer walk, which a matrix served from the Infinity Cache does not repay
+// (128^3 diffusion, 117 MB of FP64 matrix: SpMV 32.6 -> 34.8 us with codes;
+// 192^3 Laplacian, 621 MB: 117.9 -> 110.4 us).
 template <typename T, int NPT, bool NT, class Epi, class Gather>
 inline void launch_tiles_npt(const num_csr* A, const Schedule& s, const T* vals,
                              const Gather& x, const Epi& epi, hipStream_t stream)
 {
+    if constexpr (sizeof(T) == 8 && NT) {
+        if (A->d16 && &s == &A->sched && vals == A->vals) {
+            spmv_d16_kernel<kSpmvBS, NPT, NT, Epi, Gather>
+                <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
+                    A->rowptr, A->d16, vals, x, static_cast<const TileEntry*>(s.d_tiles),
+                    s.nblocks, s.chunk, epi);
+            return;
+        }
+    }
     spmv_tile_kernel<T, kSpmvBS, NPT, NT, Epi, Gather>
         <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
             A->rowptr, A->colind, vals, x, static_cast<const TileEntry*>(s.d_tiles), s.nblocks,

@@ -131,20 +131,22 @@ class CrsMatrix:
     """Device CSR matrix (src/crs_matrix.hpp:37-73).  Owns its device arrays
     (torch tensors) and the native handle with its row-block schedule."""
 
-    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_rel16", "_nrows", "_ncols",
-                 "_nnz", "_h_rowptr", "__weakref__")
+    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_rel16", "_d16", "_nrows",
+                 "_ncols", "_nnz", "_h_rowptr", "__weakref__")
 
     def __init__(self):
         raise TypeError("CrsMatrix objects are created by upload_csr()")
 
     @classmethod
     def _from_device(cls, h_rowptr: np.ndarray, rowptr: torch.Tensor, colind: torch.Tensor,
-                     values: torch.Tensor, nrows: int, ncols: int) -> "CrsMatrix":
+                     values: torch.Tensor, nrows: int, ncols: int,
+                     index: str = "auto") -> "CrsMatrix":
         self = object.__new__(cls)
         self._h = None
         self._rowptr, self._colind, self._values = rowptr, colind, values
         self._values32 = None
         self._rel16 = None
+        self._d16 = None
         self._h_rowptr = h_rowptr
         h = C.c_void_p()
         call("num_csr_create", h_rowptr.ctypes.data_as(C.c_void_p), _ptr(rowptr), _ptr(colind),
@@ -153,7 +155,24 @@ class CrsMatrix:
         r, c, z = C.c_int32(), C.c_int32(), C.c_int64()
         call("num_csr_shape", h, C.byref(r), C.byref(c), C.byref(z))
         self._nrows, self._ncols, self._nnz = r.value, c.value, z.value
+        if index == "auto" and self._nnz > 0 and int(np.diff(h_rowptr).max()) <= 32:
+            # delta-coded column ids for the FP64 kernels where they fit
+            codes = torch.empty(self._nnz + 2, dtype=torch.int16, device=values.device)
+            ok = C.c_int32()
+            call("num_csr_attach_d16", h, _ptr(codes), _stream(), C.byref(ok))
+            self._d16 = codes if ok.value else None
         return self
+
+    @property
+    def index_format(self) -> str:
+        """'delta16' when the FP64 SpMV / CG kernels read the delta-coded
+        column ids: they are attached (every row <= 32 entries in ascending
+        column order, codes within 16 bits) and the schedule streams the
+        FP64 matrix from HBM (nt64; a matrix that stays in the Infinity
+        Cache reads its int32 columns faster).  Else 'int32'."""
+        if self._d16 is None or not self.schedule_info()["nt64"]:
+            return "int32"
+        return "delta16"
 
     @property
     def nrows(self) -> int:
@@ -190,6 +209,11 @@ class CrsMatrix:
         """Rebuild the row-tile schedule (0 / -1 = default for that knob)."""
         call("num_csr_set_schedule", self._h, int(rows_per_block), int(nnz_per_block),
              int(chunk), int(nt64), int(nt32))
+        if self._d16 is not None:  # re-coded for the new tiles, or detached
+            on = C.c_int32()
+            call("num_csr_index_info", self._h, C.byref(on))
+            if not on.value:
+                self._d16 = None
 
     def schedule_info(self) -> dict:
         v = [C.c_int32() for _ in range(8)]
@@ -208,13 +232,21 @@ class CrsMatrix:
             self._h = None
 
 
-def upload_csr(rowptr, colind, values, nrows: int, ncols: int) -> CrsMatrix:
+def upload_csr(rowptr, colind, values, nrows: int, ncols: int, *,
+               index: str = "auto") -> CrsMatrix:
     """Copy a host CSR matrix to the GPU (bindings.cpp:97-123).
 
     rowptr int32[nrows+1], colind int32[nnz], values float64[nnz] (host).
     Raises ValueError for non-positive dimensions (crs_matrix.hpp:52-53) and,
     beyond the reference, for inconsistent lengths or out-of-range columns.
+    index="auto" also builds the delta-coded column ids (2 bytes per entry
+    more on the device) that the FP64 SpMV and CG read instead of colind
+    where every row holds <= 32 entries in ascending column order;
+    index="int32" keeps the int32 columns only.  Results are the same
+    either way.
     """
+    if index not in ("auto", "int32"):
+        raise ValueError("index must be 'auto' or 'int32'")
     _ensure_init()
     nrows, ncols = int(nrows), int(ncols)
     rp = _host_array(rowptr, "rowptr", np.int32)
@@ -237,7 +269,7 @@ def upload_csr(rowptr, colind, values, nrows: int, ncols: int) -> CrsMatrix:
     d_rp = torch.from_numpy(rp).to(dev)
     d_ci = torch.from_numpy(ci).to(dev) if nnz else torch.zeros(1, dtype=torch.int32, device=dev)
     d_v = torch.from_numpy(vv).to(dev) if nnz else torch.zeros(1, dtype=torch.float64, device=dev)
-    return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, nrows, ncols)
+    return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, nrows, ncols, index=index)
 
 
 def upload_stencil(dims, shift: float = 0.0) -> CrsMatrix:
