@@ -912,7 +912,8 @@ static void launch_mix_pair_npt(const num_csr* A, const MixArgs& a, hipStream_t 
 {
     const Schedule& s = A->sched;
     const int npairs = (s.nblocks + 1) / 2;
-    const int chunk = tile_chunk(npairs);
+    const int pc = plane_chunk(A, 2.0 * A->nrows / std::max(1, s.nblocks));
+    const int chunk = pc > 0 && npairs >= 8 * pc ? pc : tile_chunk(npairs);
     const int grid = chunk_grid(npairs, chunk);
     const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
     if (s.nt32) {

@@ -48,6 +48,10 @@ struct num_csr {
     bool owns = false;                  // rowptr/colind/vals allocated here
     std::vector<int32_t> h_rowptr;      // host copy, drives scheduling
     int32_t max_row_nnz = 0;
+    // Dominant long-range column offset |col - row| (the largest offset of
+    // the highest power-of-two band that holds >= 10 % of the entries: the
+    // z-plane offset nx*ny of a 3-D stencil), 0 if none; see plane_chunk.
+    int32_t offset_d = 0;
     numamd::Schedule sched;             // whole-matrix schedule
     mutable std::mutex mu;              // guards range_sched
     mutable std::map<std::pair<int32_t, int32_t>, numamd::Schedule> range_sched;
@@ -63,6 +67,12 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
 // mean row length, times the block size.
 int32_t default_nnz_per_block(const num_csr* A);
 void free_schedule(Schedule& s);
+// XCD chunk (in units of rows_per_unit rows) that deals the rows offset_d
+// apart to the same XCD: offset_d / 8 rows, so rows offset_d apart sit
+// exactly 8 chunks apart and the x entries one chunk gathers across the
+// long-range offset were fetched into that XCD's L2 by its previous chunk.
+// 0 when there is no such offset or the chunk would fall outside [1, 64].
+int32_t plane_chunk(const num_csr* A, double rows_per_unit);
 // Schedule for a row range (cached per range); whole range -> A->sched.
 int range_schedule(const num_csr* A, int32_t r0, int32_t r1, const Schedule** out);
 int check_device(const num_csr* A);

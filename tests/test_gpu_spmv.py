@@ -138,6 +138,25 @@ def test_schedules_agree(ks, orc, sched):
     assert rel(_spmv(ks, dA, x), orc.spmv(A.rowptr, A.colind, A.values, x)) < 1e-12
 
 
+def test_plane_aligned_chunks(ks, orc):
+    # 64^3: the z-plane offset is 4096 rows, so the automatic XCD chunk is
+    # 4096 / 8 = 512 rows = 2 tiles of 256 rows; a 2-D 200^2 grid (offset
+    # 200 rows, under one tile per XCD) keeps the size-based chunk.  The
+    # order of the workgroups never changes a result.
+    A = g.laplacian_3d(64)
+    dA = _up(ks, A)
+    assert dA.schedule_info()["chunk"] == 2
+    x = np.random.default_rng(6).uniform(-1, 1, A.ncols)
+    y = _spmv(ks, dA, x)
+    assert np.array_equal(y, orc.spmv(A.rowptr, A.colind, A.values, x))
+    for ch in (1, 64):
+        dA.set_schedule(chunk=ch)
+        assert dA.schedule_info()["chunk"] == ch
+        assert np.array_equal(_spmv(ks, dA, x), y)
+    B = g.laplacian_2d(200)
+    assert _up(ks, B).schedule_info()["chunk"] == 1
+
+
 def test_empty_rows_and_rect(ks, orc):
     rng = np.random.default_rng(5)
     nrows, ncols = 3000, 777
