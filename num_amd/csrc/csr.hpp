@@ -71,8 +71,9 @@ void free_schedule(Schedule& s);
 // apart to the same XCD: offset_d / 8 rows, so rows offset_d apart sit
 // exactly 8 chunks apart and the x entries one chunk gathers across the
 // long-range offset were fetched into that XCD's L2 by its previous chunk.
-// 0 when there is no such offset or the chunk would fall outside [1, 64].
-int32_t plane_chunk(const num_csr* A, double rows_per_unit);
+// 0 when there is no such offset or the chunk would fall outside
+// [1, max_units].
+int32_t plane_chunk(const num_csr* A, double rows_per_unit, int32_t max_units = 64);
 // Schedule for a row range (cached per range); whole range -> A->sched.
 int range_schedule(const num_csr* A, int32_t r0, int32_t r1, const Schedule** out);
 int check_device(const num_csr* A);

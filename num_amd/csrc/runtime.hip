@@ -243,12 +243,12 @@ static int find_offset(num_csr* A)
     return NUM_OK;
 }
 
-int32_t plane_chunk(const num_csr* A, double rows_per_unit)
+int32_t plane_chunk(const num_csr* A, double rows_per_unit, int32_t max_units)
 {
     if (A->offset_d <= 0 || rows_per_unit <= 0) return 0;
     const double c = A->offset_d / (double)kNumXcd / rows_per_unit;
     const int32_t r = (int32_t)(c + 0.5);
-    return r >= 1 && r <= 64 ? r : 0;
+    return r >= 1 && r <= max_units ? r : 0;
 }
 
 static int finish_create(num_csr* A)
