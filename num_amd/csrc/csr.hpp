@@ -41,8 +41,9 @@ struct num_csr {
     const float* vals32 = nullptr;
     const int16_t* rel16 = nullptr;     // column - row as int16 (FP32 path), if attached
     // Delta-coded column ids (FP64 path on the whole-matrix schedule), if
-    // attached: see num_csr_attach_d16.  d16_buf is the caller's buffer,
-    // kept so a new schedule can re-code into it.
+    // attached: see num_csr_attach_d16.  d16_buf is the caller's buffer
+    // (nnz + 2 codes, then one length byte per row: d16_lens), kept so a
+    // new schedule can re-code into it.
     const uint16_t* d16 = nullptr;
     uint16_t* d16_buf = nullptr;
     bool owns = false;                  // rowptr/colind/vals allocated here

@@ -287,7 +287,8 @@ __global__ void rel16_kernel(int32_t nrows, const int32_t* __restrict__ rowptr,
 // exceeds 16 bits.
 __global__ void d16_build_kernel(const int32_t* __restrict__ rowptr,
                                  const int32_t* __restrict__ colind, TileEntry* __restrict__ tiles,
-                                 uint16_t* __restrict__ code, int32_t* __restrict__ bad)
+                                 uint16_t* __restrict__ code, uint8_t* __restrict__ lens,
+                                 int32_t* __restrict__ bad)
 {
     __shared__ int base;
     const int r0 = tiles[blockIdx.x].row, r1 = tiles[blockIdx.x + 1].row;
@@ -302,6 +303,7 @@ __global__ void d16_build_kernel(const int32_t* __restrict__ rowptr,
     int flag = 0;
     for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
         int prev = b;
+        lens[r] = (uint8_t)(rowptr[r + 1] - rowptr[r]);  // <= kShortRow
         for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
             const int c = colind[j];
             const int d = c - prev;
@@ -328,7 +330,8 @@ static int build_d16(num_csr* A, hipStream_t st)
     hipError_t e = hipMemcpyAsync(d_bad, &h_bad, sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         d16_build_kernel<<<A->sched.nblocks, kSpmvBS, 0, st>>>(
-            A->rowptr, A->colind, static_cast<TileEntry*>(A->sched.d_tiles), A->d16_buf, d_bad);
+            A->rowptr, A->colind, static_cast<TileEntry*>(A->sched.d_tiles), A->d16_buf,
+            d16_lens(A->d16_buf, A->nnz), d_bad);
         e = hipGetLastError();
     }
     if (e == hipSuccess)

@@ -346,22 +346,37 @@ constexpr int d16_pairs() { return (NPT + 2) / 2; }  // 2*BS*pairs >= BS*NPT + 1
 
 template <int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_d16_run(
-    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
+    const TileEntry e0, const TileEntry e1, const uint8_t* __restrict__ lens,
     const uint16_t* __restrict__ code, const double* __restrict__ vals, const Gather& x,
     const Epi& epi, double* __restrict__ lv, uint32_t* __restrict__ lc2, double& ca, double& cb)
 {
     constexpr int NPP = d16_pairs<BS, NPT>();
+    constexpr int NW = BS / kWave;
+    __shared__ int wtot[NW];
     const int t = threadIdx.x;
     const int r0 = e0.row, j0 = e0.nz0;
     const int r1 = e1.row, j1 = e1.nz0;
     const int nz = j1 - j0, nr = r1 - r0;
     ca = 0.0;
     cb = 0.0;
-    // row bounds, branch-free, in flight with the stage loads (spmv_tile_run)
+    // row bounds from one length byte per row (instead of two int32 bounds):
+    // a wave-level inclusive scan now, the waves' totals combined after the
+    // stage barrier; the length load is in flight with the stage loads
     const bool active = t < nr;
-    const int tr = min(t, nr);
-    const int rb_raw = rowptr[r0 + tr];
-    const int re_raw = rowptr[r0 + tr + (active ? 1 : 0)];
+    const uint8_t len_raw = lens[r0 + min(t, nr - 1)];  // branch-free (nr >= 1)
+    const int lane = t & (kWave - 1);
+    int len = 0, incl = 0;
+    // after the stage loads are issued: waits for the length byte only
+    auto scan = [&]() {
+        len = active ? (int)len_raw : 0;
+        incl = len;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int y = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += y;
+        }
+        if (lane == kWave - 1) wtot[t / kWave] = incl;
+    };
     // codes staged as 4-byte pairs from the even entry jb <= j0 (the code
     // array holds nnz + 2 entries, so the last pair stays inside it): code
     // of entry j at lc[j - jb], value at lv[j - j0]
@@ -378,6 +393,7 @@ __device__ __forceinline__ void spmv_d16_run(
 #pragma unroll
         for (int s = 0; s < NPP; ++s) c[s] = ld_mat<NT>(cp + min(t + s * BS, lastp));
         if constexpr (has_hook<Gather>::value) x.issue();
+        scan();
 #pragma unroll
         for (int s = 0; s < NPT; ++s) {
             const int k = t + s * BS;
@@ -388,8 +404,9 @@ __device__ __forceinline__ void spmv_d16_run(
             const int k = t + s * BS;
             if (k <= lastp) lc2[k] = c[s];
         }
-    } else if constexpr (has_hook<Gather>::value) {
-        x.issue();
+    } else {
+        if constexpr (has_hook<Gather>::value) x.issue();
+        scan();
     }
     __syncthreads();
     if constexpr (has_hook<Gather>::value) {
@@ -397,7 +414,11 @@ __device__ __forceinline__ void spmv_d16_run(
     }
     const uint16_t* __restrict__ lc = reinterpret_cast<const uint16_t*>(lc2) + off;
     if (!active) return;
-    const int rb = rb_raw - j0, re = re_raw - j0;
+    int before = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        if (w < t / kWave) before += wtot[w];
+    const int rb = before + incl - len, re = rb + len;
     double acc = 0.0;
     int col = e0.base;  // a valid column: the smallest first column of the tile (or 0)
     int k = rb;
@@ -471,7 +492,7 @@ __global__ __launch_bounds__(BS) void spmv_tile_kernel(
 // The same over delta-coded column ids (spmv_d16_run).
 template <int BS, int NPT, bool NT, class Epi, class Gather>
 __global__ __launch_bounds__(BS) void spmv_d16_kernel(
-    const int32_t* __restrict__ rowptr, const uint16_t* __restrict__ code,
+    const uint8_t* __restrict__ lens, const uint16_t* __restrict__ code,
     const double* __restrict__ vals, Gather x, const TileEntry* __restrict__ sched, int ntiles,
     int chunk, Epi epi)
 {
@@ -483,7 +504,7 @@ __global__ __launch_bounds__(BS) void spmv_d16_kernel(
     if (epi.skip()) return;
     x.prepare();
     double ca, cb;
-    spmv_d16_run<BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], rowptr, code, vals, x,
+    spmv_d16_run<BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], lens, code, vals, x,
                                            epi, lv, lc2, ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);

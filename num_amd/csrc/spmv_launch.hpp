@@ -20,7 +20,8 @@ inline void launch_tiles_npt(const num_csr* A, const Schedule& s, const T* vals,
         if (A->d16 && &s == &A->sched && vals == A->vals) {
             spmv_d16_kernel<kSpmvBS, NPT, NT, Epi, Gather>
                 <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
-                    A->rowptr, A->d16, vals, x, static_cast<const TileEntry*>(s.d_tiles),
+                    d16_lens(A->d16, A->nnz), A->d16, vals, x,
+                    static_cast<const TileEntry*>(s.d_tiles),
                     s.nblocks, s.chunk, epi);
             return;
         }
