@@ -168,8 +168,9 @@ def batch_bench(ks, steps, L=100, k=128):
 
 def slab_bench(ks, A, x, y_ref, steps):
     """One GPU's share of the 8-slab row partition of the 192^3 Laplacian
-    (rank 3: 24 owned z-planes, 2 ghost planes): interior rows on x_local,
-    boundary rows on [x_local | x_ghost]."""
+    (rank 3: 24 owned z-planes, 2 ghost planes) on [x_local | x_ghost]: one
+    launch over the share, and (us_interior_then_boundary) the interior rows
+    then the boundary rows, the order an exchange in flight needs."""
     import torch
     D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=3, size=8)
     s, e = D.local_row_start, D.local_row_end
@@ -177,11 +178,16 @@ def slab_bench(ks, A, x, y_ref, steps):
     xg = D.ghost_values(torch.from_numpy(x).cuda())
     yl = torch.zeros(e - s, dtype=torch.float64, device="cuda")
     avg = time_launches(lambda: ks.dist_spmv(D, xl, yl, xg, sync=False), steps, 5)
+    avg_ov = time_launches(lambda: ks.dist_spmv(D, xl, yl, xg, overlap=True, sync=False),
+                           steps, 5)
+    ks.dist_spmv(D, xl, yl, xg)  # the one-launch result is the one checked
     loc = D.local
     B = 12 * loc.nnz + 4 * (loc.nrows + 1) + 8 * loc.ncols + 8 * loc.nrows
     exact = y_ref is not None and bool(np.array_equal(yl.cpu().numpy(), y_ref[s:e]))
     return {"rows": [s, e], "n_local": loc.nrows, "n_ghost": D.n_ghost, "nnz": loc.nnz,
-            "launches": len(D.interior_runs) + len(D.boundary_runs),
+            "launches": 1,
+            "us_interior_then_boundary": round(avg_ov * 1e6, 2),
+            "launches_interior_then_boundary": len(D.interior_runs) + len(D.boundary_runs),
             "us": round(avg * 1e6, 2), "GBps": round(B / avg / 1e9, 1),
             "bytes": B, "cache_resident": B < 256 * 2**20,
             "bit_exact_vs_oracle": exact if y_ref is not None else None}

@@ -43,6 +43,9 @@ def test_share_rows_equal_global_rows(ks, orc, name, make, shares):
         yl = torch.full((e - s,), float("nan"), dtype=torch.float64, device="cuda")
         ks.dist_spmv(D, xl, yl, xg)
         got = yl.cpu().numpy()
+        yo = torch.full((e - s,), float("nan"), dtype=torch.float64, device="cuda")
+        ks.dist_spmv(D, xl, yo, xg, overlap=True)  # interior, then boundary runs
+        assert np.array_equal(yo.cpu().numpy(), got), (name, rank, size)
         if int(np.diff(A.rowptr).max()) <= 32:
             # one lane per row: the row is summed in entry order
             assert np.array_equal(got, y_ref[s:e]), (name, rank, size)
