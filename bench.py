@@ -128,19 +128,28 @@ def time_launches(fn, steps, warmup):
 
 def zipf_bench(ks, steps):
     """SpMV on the irregular Zipf(2.0) matrix, n = 4M, rows clipped to
-    [1, 4096] (long rows take the multi-lane and whole-workgroup paths)."""
+    [1, 4096], columns uniform: upload_csr picks the column-partitioned
+    layout (each XCD gathers from its own x blocks); the row layout of the
+    same matrix is timed beside it (us_row_layout).  kernel_us covers both
+    launches of the partitioned SpMV."""
     import torch
     from num_amd import generators as g
     Z = g.zipf_matrix(4_000_000, 2.0, 4096, seed=2)
-    dZ = ks.upload_csr(Z.rowptr, Z.colind, Z.values, Z.nrows, Z.ncols)
     x = np.random.default_rng(1).uniform(-1, 1, Z.ncols)
     dx = torch.from_numpy(x).cuda()
     dy = torch.zeros(Z.nrows, dtype=torch.float64, device="cuda")
+    dR = ks.upload_csr(Z.rowptr, Z.colind, Z.values, Z.nrows, Z.ncols, layout="rows")
+    avg_rows = time_launches(lambda: ks.spmv(dR, dx, dy, sync=False), steps, 5)
+    y_rows = dy.clone()
+    del dR
+    dZ = ks.upload_csr(Z.rowptr, Z.colind, Z.values, Z.nrows, Z.ncols)
     avg = time_launches(lambda: ks.spmv(dZ, dx, dy, sync=False), steps, 5)
     B = spmv_bytes(Z.nrows, Z.ncols, Z.nnz)
+    d = float(torch.linalg.norm(dy - y_rows) / torch.linalg.norm(y_rows))
     return {"n": Z.nrows, "nnz": Z.nnz, "max_row": int(np.diff(Z.rowptr).max()),
-            "kernel_us": round(avg * 1e6, 2), "GBps": round(B / avg / 1e9, 1),
-            "frac": round(B / avg / 1e9 / HBM_PEAK_GBS, 4)}
+            "layout": dZ.layout, "kernel_us": round(avg * 1e6, 2),
+            "GBps": round(B / avg / 1e9, 1), "frac": round(B / avg / 1e9 / HBM_PEAK_GBS, 4),
+            "us_row_layout": round(avg_rows * 1e6, 2), "rel_diff_vs_row_layout": d}
 
 
 def batch_bench(ks, steps, L=100, k=128):

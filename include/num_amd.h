@@ -98,6 +98,28 @@ int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* at
  * stays the caller's and must outlive A; num_csr_set_schedule re-codes it
  * (and detaches it if the new tiles do not fit).  Synchronises the stream. */
 int num_csr_attach_d16(num_csr* A, uint16_t* d_codes, void* stream, int32_t* attached);
+/* Attach a column-partitioned layout of A for num_spmv_xcd (no reference
+ * counterpart: a data layout of this build for matrices whose x gathers
+ * spread far beyond one XCD's L2, such as random columns).  The columns are
+ * cut into nblocks equal blocks; S is a segment matrix (num_csr_create over
+ * the caller's device arrays) with one row per (block, row) pair that holds
+ * entries, block-major and row-ascending within a block, each holding that
+ * row's entries of the block in their original order;
+ * h_block_seg_start [nblocks + 1] gives each block's first segment;
+ * d_row_seg_ptr [nrows + 1] / d_row_seg_idx [nseg] list each row's segments
+ * in block order.  Block j is served by XCD j % 8.  S and the maps stay the
+ * caller's and must outlive A; S's schedule must not change afterwards. */
+int num_csr_attach_xcd(num_csr* A, const num_csr* S, int32_t nblocks,
+                       const int32_t* h_block_seg_start, const int32_t* d_row_seg_ptr,
+                       const int32_t* d_row_seg_idx);
+/* y = A x through the column-partitioned layout (replaces the kernels behind
+ * launch_spmv_kokkoskernels / launch_spmv_custom, spmv_kernel.hpp:102-180,
+ * for such matrices): a partial sum per segment into d_work [nseg] doubles,
+ * then each row's partials summed in block order.  A row whose entries all
+ * lie in one block is bit-identical to the row loop; others agree to
+ * rounding.  d_work is per call (concurrent calls need their own). */
+int num_spmv_xcd(const num_csr* A, const double* d_x, double* d_y, double* d_work,
+                 void* stream);
 /* *delta16 = 1 if a delta-coded index is attached (num_csr_attach_d16). */
 int num_csr_index_info(const num_csr* A, int32_t* delta16);
 int num_csr_destroy(num_csr* A);

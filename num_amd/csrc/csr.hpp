@@ -53,6 +53,15 @@ struct num_csr {
     // the highest power-of-two band that holds >= 10 % of the entries: the
     // z-plane offset nx*ny of a 3-D stencil), 0 if none; see plane_chunk.
     int32_t offset_d = 0;
+    // Column-partitioned layout (num_csr_attach_xcd), if attached: the
+    // segment matrix (one row per (column block, row) pair holding entries,
+    // block-major), each row's segments in block order, and the segment
+    // matrix's tiles grouped by the XCD that serves their column block.
+    const num_csr* xcd_seg = nullptr;            // the caller's; outlives A
+    const int32_t* xcd_row_seg_ptr = nullptr;    // [nrows + 1]
+    const int32_t* xcd_row_seg_idx = nullptr;    // [nseg]
+    int32_t* xcd_tiles = nullptr;                // owned: [tiles], grouped by XCD
+    int32_t xcd_off[numamd::kNumXcd + 1] = {};   // group bounds in xcd_tiles
     numamd::Schedule sched;             // whole-matrix schedule
     mutable std::mutex mu;              // guards range_sched
     mutable std::map<std::pair<int32_t, int32_t>, numamd::Schedule> range_sched;

@@ -515,9 +515,66 @@ int num_csr_index_info(const num_csr* A, int32_t* delta16)
     return NUM_OK;
 }
 
+int num_csr_attach_xcd(num_csr* A, const num_csr* S, int32_t nblocks,
+                       const int32_t* h_block_seg_start, const int32_t* d_row_seg_ptr,
+                       const int32_t* d_row_seg_idx)
+{
+    if (!A || !S || !h_block_seg_start) return fail(NUM_EINVAL, "NULL argument");
+    if (nblocks < 1) return fail(NUM_EINVAL, "nblocks must be >= 1");
+    if (S->ncols != A->ncols || S->nnz != A->nnz)
+        return fail(NUM_EINVAL, "segment matrix does not match A (ncols / nnz)");
+    if (h_block_seg_start[0] != 0 || h_block_seg_start[nblocks] != S->nrows)
+        return fail(NUM_EINVAL, "block segment starts must run from 0 to the segment count");
+    for (int32_t j = 0; j < nblocks; ++j)
+        if (h_block_seg_start[j] > h_block_seg_start[j + 1])
+            return fail(NUM_EINVAL, "block segment starts must be non-decreasing");
+    if (!d_row_seg_ptr || (S->nrows > 0 && !d_row_seg_idx))
+        return fail(NUM_EINVAL, "row -> segment map is NULL");
+    int rc = check_device(A);
+    if (rc) return rc;
+    // tiles of the segment matrix by the XCD of their first segment's block
+    const int32_t nt = S->sched.nblocks;
+    std::vector<TileEntry> tiles((size_t)nt + 1);
+    if (nt > 0)
+        NUM_HIP(hipMemcpy(tiles.data(), S->sched.d_tiles, tiles.size() * sizeof(TileEntry),
+                          hipMemcpyDeviceToHost));
+    std::vector<int32_t> group[kNumXcd];
+    int32_t j = 0;
+    for (int32_t t = 0; t < nt; ++t) {
+        while (j + 1 < nblocks && tiles[t].row >= h_block_seg_start[j + 1]) ++j;
+        group[j % kNumXcd].push_back(t);
+    }
+    std::vector<int32_t> flat;
+    flat.reserve((size_t)nt);
+    int32_t off[kNumXcd + 1];
+    off[0] = 0;
+    for (int k = 0; k < kNumXcd; ++k) {
+        flat.insert(flat.end(), group[k].begin(), group[k].end());
+        off[k + 1] = (int32_t)flat.size();
+    }
+    int32_t* d_tiles = nullptr;
+    if (nt > 0) {
+        NUM_HIP(hipMalloc(&d_tiles, flat.size() * sizeof(int32_t)));
+        hipError_t e = hipMemcpy(d_tiles, flat.data(), flat.size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d_tiles);
+            return fail(NUM_EHIP, "XCD tile list upload failed: %s", hipGetErrorString(e));
+        }
+    }
+    if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
+    A->xcd_tiles = d_tiles;
+    std::copy(off, off + kNumXcd + 1, A->xcd_off);
+    A->xcd_row_seg_ptr = d_row_seg_ptr;
+    A->xcd_row_seg_idx = d_row_seg_idx;
+    A->xcd_seg = S;
+    return NUM_OK;
+}
+
 int num_csr_destroy(num_csr* A)
 {
     if (!A) return NUM_OK;
+    if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
     free_schedule(A->sched);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
     if (A->owns) {

@@ -316,6 +316,73 @@ static void dispatch_batch(const num_csr* A, const double* X, double* Y, int k,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Column-partitioned SpMV (num_csr_attach_xcd), for matrices whose x gathers
+// are spread far wider than an XCD's 4 MB L2 (random columns: the Zipf
+// power-law rows).  x is cut into column blocks of up to 4 MB, and block j is served
+// by XCD j % 8 only, so each XCD's L2 holds the one x block it gathers from
+// instead of filling a 128-byte line per random 8-byte gather.  Pass 1 runs
+// the row-tile kernel (trimmed short-row walk) over the segment matrix (a
+// row per (block, row) pair, block-major) with each workgroup's tile taken
+// from its own XCD's group
+// (workgroup b runs on XCD b % 8), writing one partial sum per segment.
+// Pass 2 sums each row's partials in block order.  A row whose entries all
+// fall in one block keeps the row loop's sum bit for bit (0.0 + s = s);
+// other rows are the sum of their blocks' partials.
+// ---------------------------------------------------------------------------
+struct XcdGroups {
+    int32_t off[kNumXcd + 1];
+};
+
+template <int NPT, bool NT>
+__global__ __launch_bounds__(kSpmvBS) void spmv_xcd_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ vals, GatherX<double> x, const TileEntry* __restrict__ sched,
+    const int32_t* __restrict__ xcd_tiles, XcdGroups g, EpiY<double> epi)
+{
+    __shared__ double lv[kSpmvBS * NPT];
+    __shared__ int32_t lc[kSpmvBS * NPT];
+    const int k = blockIdx.x % kNumXcd;
+    const int q = blockIdx.x / kNumXcd;
+    if (q >= g.off[k + 1] - g.off[k]) return;
+    const int tile = xcd_tiles[g.off[k] + q];
+    x.prepare();
+    double ca, cb;
+    spmv_tile_body<double, kSpmvBS, NPT, NT, EpiY<double>, GatherX<double>, true>(
+        tile, rowptr, colind, vals, x, sched, epi, lv, lc, ca, cb);
+}
+
+__global__ void xcd_reduce_kernel(int32_t n, const int32_t* __restrict__ row_seg_ptr,
+                                  const int32_t* __restrict__ row_seg_idx,
+                                  const double* __restrict__ part, double* __restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double acc = 0.0;
+    for (int i = row_seg_ptr[r]; i < row_seg_ptr[r + 1]; ++i) acc += part[row_seg_idx[i]];
+    __builtin_nontemporal_store(acc, y + r);
+}
+
+template <int NPT>
+static void launch_xcd_npt(const num_csr* A, const double* x, double* part, hipStream_t st)
+{
+    const num_csr* S = A->xcd_seg;
+    XcdGroups g;
+    int32_t most = 0;
+    for (int k = 0; k <= kNumXcd; ++k) g.off[k] = A->xcd_off[k];
+    for (int k = 0; k < kNumXcd; ++k) most = std::max(most, g.off[k + 1] - g.off[k]);
+    if (most == 0) return;
+    const TileEntry* tiles = static_cast<const TileEntry*>(S->sched.d_tiles);
+    if (S->sched.nt64)
+        spmv_xcd_kernel<NPT, true><<<most * kNumXcd, kSpmvBS, 0, st>>>(
+            S->rowptr, S->colind, S->vals, GatherX<double>{x}, tiles, A->xcd_tiles, g,
+            EpiY<double>{part});
+    else
+        spmv_xcd_kernel<NPT, false><<<most * kNumXcd, kSpmvBS, 0, st>>>(
+            S->rowptr, S->colind, S->vals, GatherX<double>{x}, tiles, A->xcd_tiles, g,
+            EpiY<double>{part});
+}
+
 // One row of the Dirichlet stencil on an nx x ny x nz grid (x fastest):
 // entries in column order -z, -y, -x, self, +x, +y, +z, off-diagonals -1,
 // diagonal 2*ndim + shift.
@@ -369,6 +436,30 @@ int num_spmv(const num_csr* A, const double* d_x, double* d_y, void* stream)
     int rc = check_device(A);
     if (rc) return rc;
     return launch_spmv<double>(A, A->sched, A->vals, d_x, d_y, (hipStream_t)stream);
+}
+
+int num_spmv_xcd(const num_csr* A, const double* d_x, double* d_y, double* d_work, void* stream)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    if (!A->xcd_seg) return fail(NUM_EINVAL, "no column-partitioned layout attached");
+    if (!d_x || !d_y || (A->xcd_seg->nrows > 0 && !d_work))
+        return fail(NUM_EINVAL, "x, y or work is NULL");
+    int rc = check_device(A);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    switch (A->xcd_seg->sched.npt) {
+    case 5: launch_xcd_npt<5>(A, d_x, d_work, st); break;
+    case 7: launch_xcd_npt<7>(A, d_x, d_work, st); break;
+    case 12: launch_xcd_npt<12>(A, d_x, d_work, st); break;
+    default: launch_xcd_npt<16>(A, d_x, d_work, st); break;
+    }
+    NUM_LAUNCH_CHECK();
+    if (A->nrows > 0) {
+        xcd_reduce_kernel<<<(A->nrows + 255) / 256, 256, 0, st>>>(
+            A->nrows, A->xcd_row_seg_ptr, A->xcd_row_seg_idx, d_work, d_y);
+        NUM_LAUNCH_CHECK();
+    }
+    return NUM_OK;
 }
 
 int num_spmv_rows(const num_csr* A, int32_t row_begin, int32_t row_end, const double* d_x,

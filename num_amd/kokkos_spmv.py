@@ -131,8 +131,8 @@ class CrsMatrix:
     """Device CSR matrix (src/crs_matrix.hpp:37-73).  Owns its device arrays
     (torch tensors) and the native handle with its row-block schedule."""
 
-    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_rel16", "_d16", "_nrows",
-                 "_ncols", "_nnz", "_h_rowptr", "__weakref__")
+    __slots__ = ("_h", "_rowptr", "_colind", "_values", "_values32", "_rel16", "_d16", "_xcd",
+                 "_nrows", "_ncols", "_nnz", "_h_rowptr", "__weakref__")
 
     def __init__(self):
         raise TypeError("CrsMatrix objects are created by upload_csr()")
@@ -147,6 +147,7 @@ class CrsMatrix:
         self._values32 = None
         self._rel16 = None
         self._d16 = None
+        self._xcd = None
         self._h_rowptr = h_rowptr
         h = C.c_void_p()
         call("num_csr_create", h_rowptr.ctypes.data_as(C.c_void_p), _ptr(rowptr), _ptr(colind),
@@ -164,6 +165,12 @@ class CrsMatrix:
             call("num_csr_attach_d16", h, _ptr(codes), _stream(), C.byref(ok))
             self._d16 = codes if ok.value else None
         return self
+
+    @property
+    def layout(self) -> str:
+        """'xcd' when spmv() runs the column-partitioned layout (see
+        upload_csr), else 'rows'."""
+        return "xcd" if self._xcd is not None else "rows"
 
     @property
     def index_format(self) -> str:
@@ -234,8 +241,72 @@ class CrsMatrix:
             self._h = None
 
 
+# x blocks of at most 4 MiB (one XCD's L2), at least one per XCD: on the Zipf
+# 4M matrix 4 / 2 / 1 MiB blocks gave 289 / 315 / 359 us (fewer segments)
+_XCD_BLOCK_BYTES = 4 << 20
+_NUM_XCD = 8
+
+
+def _wants_xcd(rp: np.ndarray, ci: np.ndarray, nrows: int, ncols: int) -> bool:
+    """The column-partitioned layout pays when x is far larger than one
+    XCD's 4 MB L2 and most gathers land far from their row (random
+    columns): then nearly every gather fills a 128-byte L2 line."""
+    nnz = int(rp[-1])
+    if ncols * 8 <= 2 * _XCD_BLOCK_BYTES or nnz < (1 << 20):
+        return False
+    rows = np.repeat(np.arange(nrows, dtype=np.int64), np.diff(rp))
+    far = np.abs(ci.astype(np.int64) - rows) > (_XCD_BLOCK_BYTES // 8)
+    return bool(far.mean() > 0.5)
+
+
+def _xcd_plan(rp: np.ndarray, ci: np.ndarray, vv: np.ndarray, nrows: int, ncols: int):
+    """Segments of the column-partitioned layout: columns cut into nb equal
+    blocks (nb a multiple of 8, each block's x at most 4 MiB); a segment is
+    one row's entries in one block, in their original order; segments are
+    block-major, rows ascending inside a block."""
+    per = _NUM_XCD * _XCD_BLOCK_BYTES // 8
+    nb = _NUM_XCD * max(1, -(-ncols // per))
+    bs = -(-ncols // nb)
+    lens = np.diff(rp)
+    rows = np.repeat(np.arange(nrows, dtype=np.int64), lens)
+    key = (ci.astype(np.int64) // bs) * nrows + rows
+    order = np.argsort(key, kind="stable")
+    kk = key[order]
+    first = np.ones(kk.shape[0], dtype=bool)
+    first[1:] = kk[1:] != kk[:-1]
+    seg_start = np.flatnonzero(first)
+    seg_key = kk[seg_start]
+    seg_blk = seg_key // nrows
+    seg_row = seg_key - seg_blk * nrows
+    nseg = seg_start.shape[0]
+    seg_ptr = np.empty(nseg + 1, dtype=np.int32)
+    seg_ptr[:-1] = seg_start
+    seg_ptr[-1] = kk.shape[0]
+    block_start = np.searchsorted(seg_blk, np.arange(nb + 1)).astype(np.int32)
+    by_row = np.lexsort((seg_blk, seg_row)).astype(np.int32)  # row, then block
+    row_seg_ptr = np.zeros(nrows + 1, dtype=np.int32)
+    np.cumsum(np.bincount(seg_row, minlength=nrows), out=row_seg_ptr[1:])
+    return (nb, seg_ptr, ci[order].astype(np.int32), vv[order], block_start, row_seg_ptr,
+            by_row)
+
+
+def _attach_xcd(A: CrsMatrix, rp, ci, vv) -> None:
+    nb, seg_ptr, cols, vals, block_start, row_seg_ptr, row_seg_idx = _xcd_plan(
+        rp, ci, vv, A.nrows, A.ncols)
+    dev = A._values.device
+    S = CrsMatrix._from_device(seg_ptr, torch.from_numpy(seg_ptr).to(dev),
+                               torch.from_numpy(cols).to(dev), torch.from_numpy(vals).to(dev),
+                               seg_ptr.shape[0] - 1, A.ncols, index="int32")
+    d_rsp = torch.from_numpy(row_seg_ptr).to(dev)
+    d_rsi = torch.from_numpy(row_seg_idx).to(dev) if row_seg_idx.size else \
+        torch.zeros(1, dtype=torch.int32, device=dev)
+    call("num_csr_attach_xcd", A._h, S._h, int(nb), block_start.ctypes.data_as(C.c_void_p),
+         _ptr(d_rsp), _ptr(d_rsi))
+    A._xcd = (S, d_rsp, d_rsi, int(seg_ptr.shape[0] - 1))
+
+
 def upload_csr(rowptr, colind, values, nrows: int, ncols: int, *,
-               index: str = "auto") -> CrsMatrix:
+               index: str = "auto", layout: str = "auto") -> CrsMatrix:
     """Copy a host CSR matrix to the GPU (bindings.cpp:97-123).
 
     rowptr int32[nrows+1], colind int32[nnz], values float64[nnz] (host).
@@ -246,9 +317,18 @@ def upload_csr(rowptr, colind, values, nrows: int, ncols: int, *,
     where every row holds <= 32 entries in ascending column order;
     index="int32" keeps the int32 columns only.  Results are the same
     either way.
+    layout="auto" also builds, for a matrix whose x (more than 8 MiB) is
+    gathered mostly far from each row (random columns), a column-partitioned
+    copy that spmv() runs instead (each XCD gathers from its own x blocks of
+    up to 4 MiB; 12 bytes per entry and about 12 per row-block pair more on the
+    device); layout="xcd" forces it, layout="rows" never builds it.  Rows
+    whose entries lie in one block give the same bits either way, the others
+    agree to rounding.
     """
     if index not in ("auto", "int32"):
         raise ValueError("index must be 'auto' or 'int32'")
+    if layout not in ("auto", "rows", "xcd"):
+        raise ValueError("layout must be 'auto', 'rows' or 'xcd'")
     _ensure_init()
     nrows, ncols = int(nrows), int(ncols)
     rp = _host_array(rowptr, "rowptr", np.int32)
@@ -271,7 +351,10 @@ def upload_csr(rowptr, colind, values, nrows: int, ncols: int, *,
     d_rp = torch.from_numpy(rp).to(dev)
     d_ci = torch.from_numpy(ci).to(dev) if nnz else torch.zeros(1, dtype=torch.int32, device=dev)
     d_v = torch.from_numpy(vv).to(dev) if nnz else torch.zeros(1, dtype=torch.float64, device=dev)
-    return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, nrows, ncols, index=index)
+    A = CrsMatrix._from_device(rp, d_rp, d_ci, d_v, nrows, ncols, index=index)
+    if nnz and (layout == "xcd" or (layout == "auto" and _wants_xcd(rp, ci, nrows, ncols))):
+        _attach_xcd(A, rp, ci, vv)
+    return A
 
 
 def upload_stencil(dims, shift: float = 0.0) -> CrsMatrix:
@@ -308,7 +391,13 @@ def spmv(A: CrsMatrix, x, y, kokkoskernels: bool = True, *, sync: bool = True) -
         raise ValueError("spmv: x length != A.ncols")
     if y.shape[0] != A.nrows:
         raise ValueError("spmv: y length != A.nrows")
-    call("num_spmv", A._h, _ptr(x), _ptr(y), _stream())
+    if A._xcd is not None:
+        # per-call partials from torch's stream-ordered allocator (no hipMalloc
+        # after the first call), so calls on different streams never share them
+        work = torch.empty(max(1, A._xcd[3]), dtype=torch.float64, device=y.device)
+        call("num_spmv_xcd", A._h, _ptr(x), _ptr(y), _ptr(work), _stream())
+    else:
+        call("num_spmv", A._h, _ptr(x), _ptr(y), _stream())
     if sync:
         _sync()
 
