@@ -2,7 +2,7 @@
 
 Known-answer tests follow tests/test_cg.cpp [1]-[7], [9] of the reference;
 parity against the oracle (the reference's CG sequence restated in C) is:
-same convergence flag, iteration count within +-2%, both
+same convergence flag, iteration count within +-2% (the parity bar for CG), both
 recursive residuals below tol, and solutions within what the conditioning
 allows.
 """
