@@ -321,6 +321,13 @@ def test_library_rejects_bad_arguments_without_gpu():
     rp = np.array([0, 1], np.int32)
     assert L.num_csr_create_host(rp.ctypes.data_as(C.c_void_p), None, None, 0, 1,
                                  C.byref(out)) == _lib.NUM_EINVAL
+    flag = C.c_int32(7)
+    # the derived-layout entry points check their arguments before any device call
+    assert L.num_csr_attach_d16(None, None, None, C.byref(flag)) == _lib.NUM_EINVAL
+    assert L.num_csr_index_info(None, C.byref(flag)) == _lib.NUM_EINVAL
+    assert L.num_csr_attach_xcd(None, None, 8, None, None, None) == _lib.NUM_EINVAL
+    assert L.num_spmv_xcd(None, None, None, None, None) == _lib.NUM_EINVAL
+    assert "NULL" in _lib.last_error()
     with pytest.raises(ValueError):
         _lib.check(_lib.NUM_EINVAL, "x")
     with pytest.raises(RuntimeError):
