@@ -254,8 +254,10 @@ def _wants_xcd(rp: np.ndarray, ci: np.ndarray, nrows: int, ncols: int) -> bool:
     nnz = int(rp[-1])
     if ncols * 8 <= 2 * _XCD_BLOCK_BYTES or nnz < (1 << 20):
         return False
-    rows = np.repeat(np.arange(nrows, dtype=np.int64), np.diff(rp))
-    far = np.abs(ci.astype(np.int64) - rows) > (_XCD_BLOCK_BYTES // 8)
+    # a strided sample of at most 1M entries decides
+    idx = np.arange(0, nnz, max(1, nnz // (1 << 20)), dtype=np.int64)
+    rows = np.searchsorted(rp, idx, side="right") - 1
+    far = np.abs(ci[idx].astype(np.int64) - rows) > (_XCD_BLOCK_BYTES // 8)
     return bool(far.mean() > 0.5)
 
 
