@@ -83,15 +83,17 @@ int num_csr_attach_f32(num_csr* A, const float* d_vals32);
  * in int16 (stencils, banded matrices); otherwise the FP32 path keeps the
  * int32 columns.  Synchronises the stream. */
 int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* attached);
-/* Build a delta-coded copy of the column ids into d_codes (uint16, 4-byte
- * aligned, nnz + 2 + ceil(nrows / 2) entries: the nnz codes, read in 4-byte
- * pairs, then one length byte per row that replaces rowptr in the kernels) for
+/* Build a delta-coded copy of the column ids into d_codes (uint16, 16-byte
+ * aligned, nnz + 8 + ceil(nrows / 2) entries: the nnz codes and 8 padding
+ * codes, read in 16-byte units, then one length byte per row that replaces
+ * rowptr in the kernels) for
  * the FP64 row-tile kernels (no reference counterpart: a data format of this
  * build; the reference reads int32 colind, crs_matrix.hpp:22-34).  Entry j
  * of a row holds its column minus the previous entry's column; a row's first
  * entry holds its column minus its tile's base column.  Attached
  * (*attached = 1) only when every row holds <= 32 entries in ascending
- * column order and every code fits in 16 bits; then the FP64 SpMV and CG on
+ * column order, every code fits in 16 bits and the values are 16-byte
+ * aligned; then the FP64 SpMV and CG on
  * the whole-matrix schedule read 2 index bytes per entry instead of 4 and 1
  * byte per row instead of two 4-byte row bounds, with the same columns in the
  * same order (bit-identical results).  The buffer

@@ -62,15 +62,16 @@ inline int chunk_grid(int ntiles, int chunk)
     return ((nchunks + kNumXcd - 1) / kNumXcd) * kNumXcd * chunk;
 }
 
-// Row lengths (one byte per row) stored after the nnz + 2 delta codes in the
-// buffer of num_csr_attach_d16.
+// Row lengths (one byte per row) stored after the nnz delta codes and their
+// 8 padding codes in the buffer of num_csr_attach_d16.
+constexpr int kD16Pad = 8;
 __host__ __device__ inline uint8_t* d16_lens(uint16_t* codes, int64_t nnz)
 {
-    return reinterpret_cast<uint8_t*>(codes + nnz + 2);
+    return reinterpret_cast<uint8_t*>(codes + nnz + kD16Pad);
 }
 __host__ __device__ inline const uint8_t* d16_lens(const uint16_t* codes, int64_t nnz)
 {
-    return reinterpret_cast<const uint8_t*>(codes + nnz + 2);
+    return reinterpret_cast<const uint8_t*>(codes + nnz + kD16Pad);
 }
 
 // Schedule entry of one SpMV row tile; the entry after the last tile holds

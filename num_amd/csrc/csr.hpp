@@ -42,7 +42,7 @@ struct num_csr {
     const int16_t* rel16 = nullptr;     // column - row as int16 (FP32 path), if attached
     // Delta-coded column ids (FP64 path on the whole-matrix schedule), if
     // attached: see num_csr_attach_d16.  d16_buf is the caller's buffer
-    // (nnz + 2 codes, then one length byte per row: d16_lens), kept so a
+    // (nnz + 8 codes, then one length byte per row: d16_lens), kept so a
     // new schedule can re-code into it.
     const uint16_t* d16 = nullptr;
     uint16_t* d16_buf = nullptr;

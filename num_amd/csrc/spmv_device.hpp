@@ -349,35 +349,47 @@ __device__ __forceinline__ void spmv_rel16_run(
 // later code its column minus the previous entry's, as uint16.  Half the
 // index bytes of int32 (192^3 Laplacian: 98.6 of its 733 MB).  Every row of
 // the matrix holds <= kShortRow entries in ascending column order, so every
-// tile is walked one thread per row: the stage of spmv_tile_run with 2-byte
-// codes, then the running column is rebuilt by integer adds in the walk,
-// entries summed in index order from 0.0 as in spmv_tile_run (bit-identical).
+// tile is walked one thread per row, entries summed in index order from 0.0
+// as in spmv_tile_run (bit-identical); the running column is rebuilt by
+// integer adds.  Stage: the values and codes go straight from memory to LDS
+// in 16-byte units (global_load_lds_dwordx4: no VGPR round trip, no LDS
+// store instructions; 105.6 -> 104.7 us on 192^3), values from the even
+// entry jb <= j0, codes from the entry jc <= j0 that is a multiple of 8
+// (the code array is padded to nnz + 8, and 16-byte aligned like the
+// values); lanes past the last whole unit are masked off, and an odd last
+// value is loaded by one thread, so no value past the tile is read.  Row
+// starts come from one length byte per row: a workgroup scan after the
+// stage barrier.
 template <int BS, int NPT>
-constexpr int d16_pairs() { return (NPT + 2) / 2; }  // 2*BS*pairs >= BS*NPT + 1 codes
+constexpr int d16_pairs() { return (NPT + 2) / 2; }  // 16-byte value units per thread
+template <int BS, int NPT>
+constexpr int d16_code_rounds() { return ((BS * NPT + 14) / 8 + BS - 1) / BS; }
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glob_void_t;
 
 template <int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_d16_run(
     const TileEntry e0, const TileEntry e1, const uint8_t* __restrict__ lens,
     const uint16_t* __restrict__ code, const double* __restrict__ vals, const Gather& x,
-    const Epi& epi, double* __restrict__ lv, uint32_t* __restrict__ lc2, double& ca, double& cb)
+    const Epi& epi, double* __restrict__ lvu, uint16_t* __restrict__ lcu,
+    double& ca, double& cb)
 {
     constexpr int NPP = d16_pairs<BS, NPT>();
+    constexpr int NPC = d16_code_rounds<BS, NPT>();
     constexpr int NW = BS / kWave;
     __shared__ int wtot[NW];
     const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int w = t / kWave;
     const int r0 = e0.row, j0 = e0.nz0;
     const int r1 = e1.row, j1 = e1.nz0;
     const int nz = j1 - j0, nr = r1 - r0;
     ca = 0.0;
     cb = 0.0;
-    // row bounds from one length byte per row (instead of two int32 bounds):
-    // a wave-level inclusive scan now, the waves' totals combined after the
-    // stage barrier; the length load is in flight with the stage loads
     const bool active = t < nr;
-    const uint8_t len_raw = lens[r0 + min(t, nr - 1)];  // branch-free (nr >= 1)
-    const int lane = t & (kWave - 1);
+    const uint8_t len_raw = lens[r0 + min(t, nr - 1)];
     int len = 0, incl = 0;
-    // after the stage loads are issued: waits for the length byte only
     auto scan = [&]() {
         len = active ? (int)len_raw : 0;
         incl = len;
@@ -386,52 +398,51 @@ __device__ __forceinline__ void spmv_d16_run(
             const int y = __shfl_up(incl, d, kWave);
             if (lane >= d) incl += y;
         }
-        if (lane == kWave - 1) wtot[t / kWave] = incl;
+        if (lane == kWave - 1) wtot[w] = incl;
     };
-    // codes staged as 4-byte pairs from the even entry jb <= j0 (the code
-    // array holds nnz + 2 entries, so the last pair stays inside it): code
-    // of entry j at lc[j - jb], value at lv[j - j0]
-    const int jb = j0 & ~1;
-    const int off = j0 - jb;
+    const int jb = j0 & ~1, off = j0 - jb;    // values: 16-byte units from jb
+    const int jc = j0 & ~7, offc = j0 - jc;   // codes: 16-byte units (8 codes) from jc
+    constexpr unsigned kPol = NT ? 2u : 0u;
     if (nz > 0) {
-        const int last = j1 - 1;
-        const int lastp = (j1 - 1 - jb) >> 1;
-        const uint32_t* cp = reinterpret_cast<const uint32_t*>(code + jb);
-        double v[NPT];
-        uint32_t c[NPP];
-#pragma unroll
-        for (int s = 0; s < NPT; ++s) v[s] = ld_mat<NT>(vals + min(j0 + t + s * BS, last));
-#pragma unroll
-        for (int s = 0; s < NPP; ++s) c[s] = ld_mat<NT>(cp + min(t + s * BS, lastp));
-        if constexpr (has_hook<Gather>::value) x.issue();
-        scan();
-#pragma unroll
-        for (int s = 0; s < NPT; ++s) {
-            const int k = t + s * BS;
-            if (k < nz) lv[k] = v[s];
-        }
+        // whole 16-byte value units only (lanes past them are masked off, so
+        // they write no LDS); an odd last value goes through one thread, so
+        // nothing past the tile's last value is read
+        const int nu = (j1 - jb) >> 1;
+        const int nc = ((j1 - 1 - jc) >> 3) + 1;  // code units (may read past j1: padding)
 #pragma unroll
         for (int s = 0; s < NPP; ++s) {
-            const int k = t + s * BS;
-            if (k <= lastp) lc2[k] = c[s];
+            const int ub = s * BS + w * kWave;  // wave-uniform
+            if (ub + lane < nu)
+                __builtin_amdgcn_global_load_lds((glob_void_t*)(vals + jb + 2 * (ub + lane)),
+                                                 (lds_void_t*)(lvu + 2 * ub), 16, 0, kPol);
         }
-    } else {
-        if constexpr (has_hook<Gather>::value) x.issue();
-        scan();
+#pragma unroll
+        for (int s = 0; s < NPC; ++s) {
+            const int ub = s * BS + w * kWave;
+            if (ub + lane < nc)
+                __builtin_amdgcn_global_load_lds((glob_void_t*)(code + jc + 8 * (ub + lane)),
+                                                 (lds_void_t*)(lcu + 8 * ub), 16, 0, kPol);
+        }
+        if (((j1 - jb) & 1) && t == 0) lvu[2 * nu] = ld_mat<NT>(vals + j1 - 1);
     }
+    if constexpr (has_hook<Gather>::value) x.issue();
+    // the stage barrier waits for every load; the row-length scan follows it
+    __syncthreads();
+    scan();
     __syncthreads();
     if constexpr (has_hook<Gather>::value) {
         if (!x.resolve()) return;
     }
-    const uint16_t* __restrict__ lc = reinterpret_cast<const uint16_t*>(lc2) + off;
+    const uint16_t* __restrict__ lc = lcu + offc;
+    const double* __restrict__ lv = lvu + off;
     if (!active) return;
     int before = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w)
-        if (w < t / kWave) before += wtot[w];
+    for (int v = 0; v < NW; ++v)
+        if (v < w) before += wtot[v];
     const int rb = before + incl - len, re = rb + len;
     double acc = 0.0;
-    int col = e0.base;  // a valid column: the smallest first column of the tile (or 0)
+    int col = e0.base;
     int k = rb;
     for (; k + 7 < re; k += 8) {
         double vv[8], xv[8];
@@ -449,9 +460,9 @@ __device__ __forceinline__ void spmv_d16_run(
     for (int u = 0; u < 7; ++u) {
         const int kk = k + u;
         const bool ok = kk < re;
-        const int ks = ok ? kk : 0;  // slot 0 is always inside the LDS array
+        const int ks = ok ? kk : 0;
         vv[u] = lv[ks];
-        col += ok ? (int)lc[ks] : 0;  // past the row: the last column again
+        col += ok ? (int)lc[ks] : 0;
         xv[u] = x(col);
     }
 #pragma unroll
@@ -507,16 +518,16 @@ __global__ __launch_bounds__(BS) void spmv_d16_kernel(
     const double* __restrict__ vals, Gather x, const TileEntry* __restrict__ sched, int ntiles,
     int chunk, Epi epi)
 {
-    __shared__ double lv[BS * NPT];
-    __shared__ uint32_t lc2[BS * d16_pairs<BS, NPT>()];
+    __shared__ __attribute__((aligned(16))) double lv[2 * BS * d16_pairs<BS, NPT>() + 2];
+    __shared__ __attribute__((aligned(16))) uint16_t lc[8 * BS * d16_code_rounds<BS, NPT>()];
     __shared__ double red[2 * BS / kWave];
     const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
     if (tile < 0) return;
     if (epi.skip()) return;
     x.prepare();
     double ca, cb;
-    spmv_d16_run<BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], lens, code, vals, x,
-                                           epi, lv, lc2, ca, cb);
+    spmv_d16_run<BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], lens, code, vals, x, epi,
+                                           lv, lc, ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
         if (threadIdx.x == 0) epi.partial_a[tile] = s;

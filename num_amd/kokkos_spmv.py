@@ -158,8 +158,8 @@ class CrsMatrix:
         self._nrows, self._ncols, self._nnz = r.value, c.value, z.value
         if index == "auto" and self._nnz > 0 and int(np.diff(h_rowptr).max()) <= 32:
             # delta-coded column ids for the FP64 kernels where they fit
-            # nnz + 2 codes, then one row-length byte per row
-            codes = torch.empty(self._nnz + 2 + (self._nrows + 1) // 2, dtype=torch.int16,
+            # nnz + 8 codes (8 of padding), then one row-length byte per row
+            codes = torch.empty(self._nnz + 8 + (self._nrows + 1) // 2, dtype=torch.int16,
                                 device=values.device)
             ok = C.c_int32()
             call("num_csr_attach_d16", h, _ptr(codes), _stream(), C.byref(ok))
