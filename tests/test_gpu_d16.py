@@ -137,3 +137,32 @@ def test_cg_d16_bit_identical(ks, name):
     assert r0["converged"] and r0["iters"] == r1["iters"]
     assert r0["residual"] == r1["residual"]
     assert np.array_equal(x0, x1)
+
+
+def _banded(n, w, lead, seed):
+    """n rows of exactly w ascending columns (min(r, n - w) + k), except row 0
+    with `lead` entries: every later tile then holds exactly its entry
+    capacity and starts at a column-id offset that moves with `lead`."""
+    lens = np.full(n, w, np.int64)
+    lens[0] = lead
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(lens, out=rp[1:])
+    ci = np.concatenate([min(r, n - w) + np.arange(lens[r]) for r in range(n)]).astype(np.int32)
+    va = np.random.default_rng(seed).uniform(-1, 1, ci.size)
+    return rp, ci, va
+
+
+@pytest.mark.parametrize("w,npb", [(5, 1280), (7, 0), (12, 3072), (32, 4096)])
+def test_d16_full_tiles_every_offset(ks, orc, w, npb):
+    # Tiles filled to the staging capacity (256 threads x 5/7/12/16 entries)
+    # at start offsets that include the worst ones, an odd value offset and a
+    # code offset of 7 mod 8 (all 8 code offsets for w = 7, 12, 32): the LDS
+    # stage is sized to exactly what such a tile needs.
+    n = 256 * 12
+    x = np.random.default_rng(13).uniform(-1, 1, n)
+    for lead in range(8):
+        rp, ci, va = _banded(n, w, lead, seed=lead)
+        dA = ks.upload_csr(rp, ci, va, n, n)
+        dA.set_schedule(rows_per_block=0, nnz_per_block=npb, nt64=1)
+        assert dA.index_format == "delta16"
+        assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp, ci, va, x)), lead
