@@ -76,6 +76,9 @@ static int32_t auto_nt(const num_csr* A, int value_bytes)
     return bytes > kMallResident ? 1 : 0;
 }
 
+// Smallest plane-aligned XCD chunk (in tiles) the schedule takes.
+constexpr int32_t kMinPlaneChunk = 3;
+
 int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
                    int32_t nnz_per_block, int32_t chunk, int32_t nt64, int32_t nt32,
                    Schedule& s)
@@ -127,8 +130,13 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
     } else {
         const int32_t pc =
             ns.nblocks > 0 ? plane_chunk(A, (double)(r1 - r0) / ns.nblocks) : 0;
-        // 192^3 Laplacian: 18-tile chunks 107.0 us against 110.4 us for 64
-        ns.chunk = pc > 0 && ns.nblocks >= 8 * pc ? pc : tile_chunk(ns.nblocks);
+        // Plane-aligned chunks of at least kMinPlaneChunk tiles: 192^3
+        // Laplacian 18 tiles 103.3 us against 106.3 us for 64, 128^3 8 tiles
+        // 30.4 against 34.8 us, 80^3 3 tiles 9.9 against 10.1 us.  Below that
+        // the size-based chunk is faster: 2-D 3000^2 (1 tile) 112.0 against
+        // 101.6 us, 40x40x4000 (1 tile) 106 against 93 us, 64x64x2000
+        // (2 tiles) 123.5 against 119.7 us.
+        ns.chunk = pc >= kMinPlaneChunk && ns.nblocks >= 8 * pc ? pc : tile_chunk(ns.nblocks);
     }
     ns.nt64 = nt64 >= 0 ? (nt64 != 0) : auto_nt(A, 8);
     ns.nt32 = nt32 >= 0 ? (nt32 != 0) : auto_nt(A, 4);

@@ -139,13 +139,15 @@ def test_schedules_agree(ks, orc, sched):
 
 
 def test_plane_aligned_chunks(ks, orc):
-    # 64^3: the z-plane offset is 4096 rows, so the automatic XCD chunk is
-    # 4096 / 8 = 512 rows = 2 tiles of 256 rows; a 2-D 200^2 grid (offset
-    # 200 rows, under one tile per XCD) keeps the size-based chunk.  The
+    # 80^3: the z-plane offset is 6400 rows, so the automatic XCD chunk is
+    # 6400 / 8 = 800 rows = 3 tiles of 256 rows.  64^3 (4096 / 8 = 512 rows
+    # = 2 tiles) and a 2-D 200^2 grid (offset 200 rows) are under the 3-tile
+    # minimum and keep the size-based chunk (1 tile at these sizes).  The
     # order of the workgroups never changes a result.
-    A = g.laplacian_3d(64)
+    assert _up(ks, g.laplacian_3d(64)).schedule_info()["chunk"] == 1
+    A = g.laplacian_3d(80)
     dA = _up(ks, A)
-    assert dA.schedule_info()["chunk"] == 2
+    assert dA.schedule_info()["chunk"] == 3
     x = np.random.default_rng(6).uniform(-1, 1, A.ncols)
     y = _spmv(ks, dA, x)
     assert np.array_equal(y, orc.spmv(A.rowptr, A.colind, A.values, x))
