@@ -284,16 +284,20 @@ static void launch_batch_wave(const num_csr* A, const double* X, double* Y, int 
                                                            Y, A->nrows, k, nblocks, chunk);
 }
 
-// k >= kBatchWaveK: a wave per row (16-byte X loads when k > 64 is even and
+// k >= kBatchWaveK, or k > kBatchWaveK1 with 8-byte X loads (odd k or
+// unaligned X, Y): a wave per row (16-byte X loads when k > 64 is even and
 // X, Y are 16-byte aligned).  Smaller k: G lanes per row, G the smallest
-// power of two with G*V >= k (V = 2 when k is even and aligned).
+// power of two with G*V >= k (V = 2 when k is even and aligned).  100^3:
+// k = 33 589 -> 358 us and k = 63 657 -> 418 us on the wave kernel, while
+// even k = 40-56 stay 3-4 % faster on 32 lanes of 16-byte loads.
 constexpr int kBatchWaveK = 64;
+constexpr int kBatchWaveK1 = 32;
 constexpr int kBatchWaveRows = 4;
 static void dispatch_batch(const num_csr* A, const double* X, double* Y, int k,
                            hipStream_t st)
 {
     const bool v2 = (k % 2 == 0) && ((uintptr_t)X % 16 == 0) && ((uintptr_t)Y % 16 == 0);
-    if (k >= kBatchWaveK) {
+    if (k >= kBatchWaveK || (!v2 && k > kBatchWaveK1)) {
         if (v2 && k > kWave) launch_batch_wave<2, kBatchWaveRows>(A, X, Y, k, st);
         else launch_batch_wave<1, kBatchWaveRows>(A, X, Y, k, st);
         return;
@@ -311,8 +315,7 @@ static void dispatch_batch(const num_csr* A, const double* X, double* Y, int k,
         else if (w <= 4) launch_batch<4, 1>(A, X, Y, k, st);
         else if (w <= 8) launch_batch<8, 1>(A, X, Y, k, st);
         else if (w <= 16) launch_batch<16, 1>(A, X, Y, k, st);
-        else if (w <= 32) launch_batch<32, 1>(A, X, Y, k, st);
-        else launch_batch<64, 1>(A, X, Y, k, st);
+        else launch_batch<32, 1>(A, X, Y, k, st);  // w = k <= kBatchWaveK1
     }
 }
 

@@ -280,11 +280,12 @@ def test_batch_spmv_wide(ks, orc, k):
     assert np.array_equal(Y.cpu().numpy(), orc.batch_spmv(rp, ci, va, X))
 
 
-def test_batch_spmv_8byte_aligned(ks, orc):
-    # X and Y one element into their buffers: the 8-byte-load form for even k.
+@pytest.mark.parametrize("k", [6, 40])
+def test_batch_spmv_8byte_aligned(ks, orc, k):
+    # X and Y one element into their buffers: the 8-byte-load form for even k
+    # (k = 40: the wave-per-row kernel, which 8-byte loads take above k = 32).
     A = g.laplacian_2d(30)
     dA = _up(ks, A)
-    k = 6
     X = np.random.default_rng(5).standard_normal((A.ncols, k))
     xb = torch.zeros(A.ncols * k + 1, dtype=torch.float64, device="cuda")
     yb = torch.zeros(A.nrows * k + 1, dtype=torch.float64, device="cuda")
