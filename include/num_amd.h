@@ -65,7 +65,9 @@ int32_t num_abi_version(void);
 /* Borrow device CSR arrays owned by the caller (they must outlive the handle)
  * and build the row-block schedule from the host copy of rowptr.
  * Rejects nrows<=0 or ncols<=0 (crs_matrix.hpp:52-53) and, unlike the
- * reference, a non-monotone rowptr or rowptr[0] != 0. */
+ * reference (no range validation, crs_matrix.hpp:50-68), a non-monotone
+ * rowptr, rowptr[0] != 0 and a column index outside [0, ncols) (checked on
+ * the device, with NUM_EINVAL). */
 int num_csr_create(const int32_t* h_rowptr, const int32_t* d_rowptr,
                    const int32_t* d_colind, const double* d_vals, int32_t nrows,
                    int32_t ncols, num_csr** out);
@@ -124,6 +126,8 @@ int num_spmv_xcd(const num_csr* A, const double* d_x, double* d_y, double* d_wor
                  void* stream);
 /* *delta16 = 1 if a delta-coded index is attached (num_csr_attach_d16). */
 int num_csr_index_info(const num_csr* A, int32_t* delta16);
+/* Fails with NUM_ERUNTIME (and frees nothing) while an engine system admitted
+ * with A (num_engine_add) has not been removed. */
 int num_csr_destroy(num_csr* A);
 /* nrows / ncols / nnz read-only attributes (src/bindings.cpp:79-88). */
 int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz);
@@ -132,7 +136,9 @@ int num_csr_shape(const num_csr* A, int32_t* nrows, int32_t* ncols, int64_t* nnz
  * their own); tiles dealt to the 8 XCDs in chunks of `chunk` tiles; nt64 /
  * nt32 = 1 streams the FP64 / FP32 values non-temporally.  Values <= 0 (for
  * the sizes) or < 0 (chunk, nt64, nt32) pick the defaults.  Rebuilds the
- * schedule; used by the autotuner. */
+ * schedule; used by the autotuner.  Fails with NUM_ERUNTIME while an engine
+ * system admitted with A (num_engine_add) has not been removed: the engine
+ * reads A's tiles whenever it rebuilds its launch lists. */
 int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_block,
                          int32_t chunk, int32_t nt64, int32_t nt32);
 int num_csr_schedule_info(const num_csr* A, int32_t* nblocks, int32_t* rows_per_block,

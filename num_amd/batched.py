@@ -93,8 +93,13 @@ class CGEngine:
         x = _dev_tensor(x, "x", 1)
         if A.nrows != A.ncols or b.shape[0] != A.nrows or x.shape[0] != A.nrows:
             raise ValueError("engine.add: A must be square and b, x of length A.nrows")
-        if A.schedule_info()["nnz_per_block"] > ENGINE_MAX_NNZ_PER_TILE:
-            A.set_schedule(nnz_per_block=ENGINE_MAX_NNZ_PER_TILE)
+        info = A.schedule_info()
+        if info["nnz_per_block"] > ENGINE_MAX_NNZ_PER_TILE:
+            # only the tile capacity changes: rows per tile, XCD chunk and
+            # load kinds stay what the caller (or the autotuner) chose
+            A.set_schedule(rows_per_block=info["rows_per_block"],
+                           nnz_per_block=ENGINE_MAX_NNZ_PER_TILE, chunk=info["chunk"],
+                           nt64=info["nt64"], nt32=info["nt32"])
         if mixed:
             A.ensure_f32()
         nbytes = self.work_bytes(A.nrows, mixed)

@@ -224,7 +224,11 @@ __global__ __launch_bounds__(kFin) void cg_kc(int64_t n, const double* __restric
     }
     const double pAp = sum_partials_block<kFin, 32>(pa, npa, red);
     if (t == 0) {
-        const int brk = !(fabs(pAp) >= 1e-300);  // src/cg_solver.hpp:84-85 (NaN included)
+        // the guard of src/cg_solver.hpp:84-85 (std::abs(pAp) < 1e-300): a NaN
+        // p.Ap is no breakdown there, so it runs on to a non-converged result
+        // with a NaN residual, as the unguarded cg_solve_local
+        // (cg_solver.hpp:165-167) returns
+        const int brk = fabs(pAp) < 1e-300;
         s_brk = brk;
         s_alpha = brk ? 0.0 : rr / pAp;
     }

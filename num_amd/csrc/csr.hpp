@@ -1,5 +1,6 @@
 // csr.hpp -- the CSR handle behind num_csr* and its row-block schedule.
 #pragma once
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -63,6 +64,10 @@ struct num_csr {
     int32_t* xcd_tiles = nullptr;                // owned: [tiles], grouped by XCD
     int32_t xcd_off[numamd::kNumXcd + 1] = {};   // group bounds in xcd_tiles
     numamd::Schedule sched;             // whole-matrix schedule
+    // Engine systems admitted with this matrix and not yet removed: their
+    // launch lists are expanded from sched.d_tiles, so the schedule must not
+    // be rebuilt (and the handle not destroyed) while this is non-zero.
+    mutable std::atomic<int32_t> engine_holds{0};
     mutable std::mutex mu;              // guards range_sched
     mutable std::map<std::pair<int32_t, int32_t>, numamd::Schedule> range_sched;
 };

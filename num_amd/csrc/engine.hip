@@ -826,6 +826,9 @@ struct num_engine {
     std::vector<int32_t> live;  // in the launch lists (not yet known finished)
     std::vector<int32_t> done;  // finished and retired: res is valid
     std::vector<int32_t> gen;   // admission counter, tags device records
+    // the CSR handle a num_engine_add system was admitted with (its hold is
+    // released at remove / destroy), else nullptr
+    std::vector<const num_csr*> held;
     std::vector<EngRec> res;
     std::vector<int32_t> order;  // admission order of live systems
     bool dirty = true;
@@ -1120,6 +1123,7 @@ int num_engine_create(int32_t max_systems, int32_t nt64, int32_t nt32, num_engin
     E->live.assign(c, 0);
     E->done.assign(c, 0);
     E->gen.assign(c, 0);
+    E->held.assign(c, nullptr);
     E->res.assign(c, EngRec{});
     const size_t lists = (3 * c + 2) * sizeof(int32_t);
     hipError_t e = hipGetDevice(&E->device);
@@ -1164,6 +1168,8 @@ int num_engine_destroy(num_engine* E)
     if (E->d_state) (void)hipFree(E->d_state);
     if (E->d_lists) (void)hipFree(E->d_lists);
     if (E->d_rec) (void)hipFree(E->d_rec);
+    for (const num_csr* A : E->held)
+        if (A) A->engine_holds.fetch_sub(1);
     for (int i = 0; i < 2; ++i) {
         if (E->stage_ev[i]) (void)hipEventDestroy(E->stage_ev[i]);
         if (E->rec_ev[i]) (void)hipEventDestroy(E->rec_ev[i]);
@@ -1232,6 +1238,10 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     d.crows = chunk_rows(n);
     rc = eng_admit(E, s, d, h, (int32_t)((n + d.crows - 1) / d.crows), st);
     if (rc) return rc;
+    // the launch lists are expanded from A's tiles while the system is in
+    // the engine: hold the schedule until the slot is removed
+    A->engine_holds.fetch_add(1);
+    E->held[s] = A;
     *slot_out = s;
     return NUM_OK;
 }
@@ -1427,6 +1437,10 @@ int num_engine_remove(num_engine* E, int32_t slot)
     }
     E->used[slot] = 0;
     E->done[slot] = 0;
+    if (E->held[slot]) {
+        E->held[slot]->engine_holds.fetch_sub(1);
+        E->held[slot] = nullptr;
+    }
     return NUM_OK;
 }
 

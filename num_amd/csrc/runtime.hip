@@ -187,10 +187,13 @@ static int validate_rowptr(const int32_t* h_rowptr, int32_t nrows, int32_t ncols
 // |col - row| over all entries by power-of-two band (band b: offsets in
 // [2^b, 2^(b+1)); zero offsets skipped), with each band's largest offset.
 // Workgroup-local counts in LDS, one global atomic per band per workgroup.
-__global__ void offset_hist_kernel(int32_t nrows, const int32_t* __restrict__ rowptr,
+// Also flags (*bad = 1) a column index outside [0, ncols): every kernel
+// gathers x[col] unchecked, so such a matrix is refused at create.
+__global__ void offset_hist_kernel(int32_t nrows, int32_t ncols,
+                                   const int32_t* __restrict__ rowptr,
                                    const int32_t* __restrict__ colind,
                                    unsigned long long* __restrict__ count,
-                                   int32_t* __restrict__ maxo)
+                                   int32_t* __restrict__ maxo, int32_t* __restrict__ bad)
 {
     __shared__ unsigned int lcount[32];
     __shared__ int lmax[32];
@@ -199,9 +202,15 @@ __global__ void offset_hist_kernel(int32_t nrows, const int32_t* __restrict__ ro
         lmax[threadIdx.x] = 0;
     }
     __syncthreads();
+    int out_of_range = 0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += gridDim.x * blockDim.x) {
         for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            const int d = colind[j] - r;
+            const int c = colind[j];
+            if (c < 0 || c >= ncols) {
+                out_of_range = 1;
+                continue;
+            }
+            const int d = c - r;
             const int o = d < 0 ? -d : d;
             if (o > 0) {
                 const int b = 31 - __clz(o);
@@ -210,6 +219,7 @@ __global__ void offset_hist_kernel(int32_t nrows, const int32_t* __restrict__ ro
             }
         }
     }
+    if (out_of_range) atomicOr(bad, 1);
     __syncthreads();
     if (threadIdx.x < 32 && lcount[threadIdx.x]) {
         atomicAdd(&count[threadIdx.x], (unsigned long long)lcount[threadIdx.x]);
@@ -217,29 +227,35 @@ __global__ void offset_hist_kernel(int32_t nrows, const int32_t* __restrict__ ro
     }
 }
 
-// A->offset_d from the band histogram (synchronous).
+// A->offset_d from the band histogram (synchronous); NUM_EINVAL if a
+// column index lies outside [0, ncols).
 static int find_offset(num_csr* A)
 {
     A->offset_d = 0;
     if (A->nnz == 0) return NUM_OK;
     // the arrays may have been filled on any stream (set-up only, once)
     NUM_HIP(hipDeviceSynchronize());
+    constexpr size_t kBytes = 32 * sizeof(unsigned long long) + 33 * sizeof(int32_t);
     void* d = nullptr;
-    NUM_HIP(hipMalloc(&d, 32 * sizeof(unsigned long long) + 32 * sizeof(int32_t)));
+    NUM_HIP(hipMalloc(&d, kBytes));
     auto* d_count = static_cast<unsigned long long*>(d);
     auto* d_max = reinterpret_cast<int32_t*>(d_count + 32);
+    int32_t* d_bad = d_max + 32;
     unsigned long long h_count[32];
-    int32_t h_max[32];
-    hipError_t e = hipMemset(d, 0, 32 * sizeof(unsigned long long) + 32 * sizeof(int32_t));
+    int32_t h_max[33];
+    hipError_t e = hipMemset(d, 0, kBytes);
     if (e == hipSuccess) {
         const int grid = std::min(2048, (A->nrows + 255) / 256);
-        offset_hist_kernel<<<grid, 256>>>(A->nrows, A->rowptr, A->colind, d_count, d_max);
+        offset_hist_kernel<<<grid, 256>>>(A->nrows, A->ncols, A->rowptr, A->colind, d_count,
+                                          d_max, d_bad);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(h_count, d_count, sizeof(h_count), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(h_max, d_max, sizeof(h_max), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(NUM_EHIP, "offset histogram failed: %s", hipGetErrorString(e));
+    if (h_max[32] != 0)
+        return fail(NUM_EINVAL, "column index outside [0, %d)", A->ncols);
     unsigned long long cum = 0;
     for (int b = 31; b >= 0; --b) {
         cum += h_count[b];
@@ -583,6 +599,9 @@ int num_csr_attach_xcd(num_csr* A, const num_csr* S, int32_t nblocks,
 int num_csr_destroy(num_csr* A)
 {
     if (!A) return NUM_OK;
+    if (A->engine_holds.load() != 0)
+        return fail(NUM_ERUNTIME, "matrix is held by %d engine system(s); remove them first",
+                    (int)A->engine_holds.load());
     if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
     free_schedule(A->sched);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
@@ -608,6 +627,10 @@ int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_blo
                          int32_t chunk, int32_t nt64, int32_t nt32)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    // a live engine system expands its launch lists from the current tiles
+    if (A->engine_holds.load() != 0)
+        return fail(NUM_ERUNTIME, "matrix is held by %d engine system(s); its schedule cannot "
+                    "change until they are removed", (int)A->engine_holds.load());
     if (rows_per_block <= 0) rows_per_block = kSpmvBS;
     if (nnz_per_block <= 0) nnz_per_block = default_nnz_per_block(A);
     int rc = build_schedule(A, 0, A->nrows, rows_per_block, nnz_per_block, chunk, nt64, nt32,

@@ -435,7 +435,12 @@ __device__ __forceinline__ void spmv_d16_run(
         if (((j1 - jb) & 1) && t == 0) lvu[2 * nu] = ld_mat<NT>(vals + j1 - 1);
     }
     if constexpr (has_hook<Gather>::value) x.issue();
-    // the stage barrier waits for every load; the row-length scan follows it
+    // The direct-to-LDS loads are counted by vmcnt, and s_barrier does not
+    // wait for them: each wave must drain its own loads before the barrier,
+    // or another wave may read LDS that is not filled yet.  Explicit, so it
+    // does not depend on where the compiler places its own waits.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // the stage barrier then covers every load; the row-length scan follows it
     __syncthreads();
     scan();
     __syncthreads();

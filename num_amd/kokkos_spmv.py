@@ -723,6 +723,76 @@ def halo_pack(A_dist: DistCrsMatrix, x_local, send_buf=None, *, sync: bool = Tru
     return send_buf
 
 
+def send_ranges(A_dist: DistCrsMatrix):
+    """[(neighbour rank, offset, count)] of each neighbour's part of the packed
+    send buffer, neighbours ascending (halo_exchange.hpp:154-191)."""
+    out, off = [], 0
+    for r, ids in A_dist.send:
+        out.append((int(r), off, int(ids.shape[0])))
+        off += int(ids.shape[0])
+    return out
+
+
+def halo_scatter(A_dist: DistCrsMatrix, recv_buf, x_ghost, src_rank: int, *,
+                 sync: bool = True) -> None:
+    """x_ghost[offset + i] = recv_buf[i] for the ghost slots owned by
+    ``src_rank`` (halo_scatter, halo_exchange.hpp:287-300): the values that
+    rank sent, in its send order, which is this rank's ghost order for that
+    owner (ascending global id)."""
+    recv_buf = _dev_tensor(recv_buf, "recv_buf", 1)
+    x_ghost = _dev_tensor(x_ghost, "x_ghost", 1)
+    if x_ghost.shape[0] != A_dist.n_ghost:
+        raise ValueError(f"halo_scatter: x_ghost must have {A_dist.n_ghost} entries")
+    for r, off, cnt in A_dist.recv:
+        if int(r) == int(src_rank):
+            break
+    else:
+        raise ValueError(f"halo_scatter: rank {src_rank} owns no ghost of this share")
+    if recv_buf.shape[0] < cnt:
+        raise ValueError(f"halo_scatter: recv_buf holds {recv_buf.shape[0]} values, "
+                         f"rank {src_rank} sends {cnt}")
+    call("num_halo_scatter", _ptr(recv_buf), int(cnt), _ptr(x_ghost), int(off), _stream())
+    if sync:
+        _sync()
+
+
+# ------------------------------------------------- host round-trip helpers --
+
+def spmv_host(rowptr, colind, values, nrows: int, ncols: int, x) -> np.ndarray:
+    """Upload, y = A x, download (spmv_host_roundtrip, spmv_kernel.hpp:186-217):
+    host arrays in, host array out, through num_spmv_host."""
+    _ensure_init()
+    rp = _host_array(rowptr, "rowptr", np.int32)
+    ci = _host_array(colind, "colind", np.int32)
+    vv = _host_array(values, "values", np.float64)
+    xx = _host_array(x, "x", np.float64)
+    if rp.shape[0] != int(nrows) + 1 or xx.shape[0] != int(ncols):
+        raise ValueError("spmv_host: rowptr must hold nrows+1 entries and x ncols")
+    y = np.empty(int(nrows), np.float64)
+    call("num_spmv_host", rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+         vv.ctypes.data_as(C.c_void_p), int(nrows), int(ncols), xx.ctypes.data_as(C.c_void_p),
+         y.ctypes.data_as(C.c_void_p))
+    return y
+
+
+def batch_spmv_host(rowptr, colind, values, nrows: int, ncols: int, X) -> np.ndarray:
+    """batch_spmv_host_roundtrip (batch_spmv_kernel.hpp:179-209): X (ncols, k)
+    row-major on the host in, Y (nrows, k) out, through num_batch_spmv_host."""
+    _ensure_init()
+    rp = _host_array(rowptr, "rowptr", np.int32)
+    ci = _host_array(colind, "colind", np.int32)
+    vv = _host_array(values, "values", np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    if X.ndim != 2 or X.shape[0] != int(ncols) or rp.shape[0] != int(nrows) + 1:
+        raise ValueError("batch_spmv_host: X must be (ncols, k), rowptr nrows+1 entries")
+    k = X.shape[1]
+    Y = np.empty((int(nrows), k), np.float64)
+    call("num_batch_spmv_host", rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+         vv.ctypes.data_as(C.c_void_p), int(nrows), int(ncols), int(k),
+         X.ctypes.data_as(C.c_void_p), Y.ctypes.data_as(C.c_void_p))
+    return Y
+
+
 def cg_solve(A_dist: DistCrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000) -> dict:
     """Distributed CG (bindings.cpp:239-270).  On one process (size 1) this is
     cg_solve_local on the whole matrix; a share of a multi-rank partition needs

@@ -151,6 +151,24 @@ def norm2(x):
     return float(lib().orc_norm2(len(x), x))
 
 
+def copy(src, dst):
+    """dst[:] = src (orc_copy, vec_ops.hpp:74-79); returns dst."""
+    src = np.ascontiguousarray(src, np.float64)
+    if not (isinstance(dst, np.ndarray) and dst.dtype == np.float64 and dst.flags.c_contiguous
+            and dst.shape == src.shape):
+        raise ValueError("copy: dst must be a contiguous float64 array shaped like src")
+    lib().orc_copy(len(src), src, dst)
+    return dst
+
+
+def fill(x, val):
+    """x[:] = val in place (orc_fill, vec_ops.hpp:84-88); returns x."""
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+        raise ValueError("fill: x must be a contiguous float64 array")
+    lib().orc_fill(len(x), x, float(val))
+    return x
+
+
 def scale(x, alpha):
     x = np.array(x, np.float64, copy=True)
     lib().orc_scale(len(x), x, alpha)

@@ -37,6 +37,8 @@
 /*   5 dot skips the last element                                            */
 /*   6 axpy2 swaps its two coefficients                                      */
 /*   7 batch spmv reads X column-major instead of row-major                   */
+/*   8 fill leaves the last element untouched                                 */
+/*   9 copy copies all but the first element                                  */
 /* ------------------------------------------------------------------------ */
 static int g_mutant = 0;
 static int g_threads = 0; /* 0 = OpenMP default */
@@ -161,14 +163,22 @@ double orc_dot(int64_t n, const double* x, const double* y)
 /* norm2 = sqrt(dot(x,x))  (vec_ops.hpp:64-69). */
 double orc_norm2(int64_t n, const double* x) { return sqrt(orc_dot(n, x, x)); }
 
-/* copy / fill / scale  (vec_ops.hpp:74-100). */
+/* copy / fill / scale  (vec_ops.hpp:74-100).  Pinned by elementwise
+ * identities over every element (a copy equal to its source, a fill of a
+ * value equal to np.full, a scale equal to the product), including the first
+ * and last element; mutants 8 and 9 are caught. */
 void orc_copy(int64_t n, const double* src, double* dst)
 {
+    if (g_mutant == 9 && n > 0) {
+        memcpy(dst + 1, src + 1, sizeof(double) * (size_t)(n - 1));
+        return;
+    }
     memcpy(dst, src, sizeof(double) * (size_t)n);
 }
 void orc_fill(int64_t n, double* x, double val)
 {
-    for (int64_t i = 0; i < n; ++i) x[i] = val;
+    const int64_t m = (g_mutant == 8 && n > 0) ? n - 1 : n;
+    for (int64_t i = 0; i < m; ++i) x[i] = val;
 }
 void orc_scale(int64_t n, double* x, double alpha)
 {

@@ -194,3 +194,37 @@ def test_large_system_wide_chunks(ks, mixed):
     b = g.hash_uniform(A.nrows, big[2])
     true = np.linalg.norm(b - A.to_scipy() @ x_alone.cpu().numpy()) / np.linalg.norm(b)
     assert true < (1e-10 if mixed else 2e-10)
+
+
+def test_schedule_held_while_in_engine(ks):
+    """A live engine system expands its launch lists from its matrix's tiles:
+    rescheduling that matrix is refused until the system is removed, and the
+    solve is unaffected by the refused attempt."""
+    from num_amd.batched import CGEngine
+    A, dA = _upload(ks, (40, 30), 0.05)
+    b = np.random.default_rng(2).uniform(-1, 1, A.nrows)
+    x_alone = _dev(np.zeros(A.nrows))
+    ref = CGEngine(max_systems=1)
+    s0 = ref.add(dA, _dev(b), x_alone, tol=1e-10, max_iter=2000)
+    ref.run(4)
+    r_alone = ref.result(s0)
+    ref.remove(s0)
+    eng = CGEngine(max_systems=2)
+    x = _dev(np.zeros(A.nrows))
+    s = eng.add(dA, _dev(b), x, tol=1e-10, max_iter=2000)
+    eng.step(3)  # mid-solve
+    before = dA.schedule_info()
+    with pytest.raises(RuntimeError, match="engine"):
+        dA.set_schedule(rows_per_block=64)
+    with pytest.raises(RuntimeError, match="engine"):
+        import ctypes as C
+        from num_amd._lib import call
+        call("num_csr_destroy", dA.handle)
+    assert dA.schedule_info() == before
+    eng.run(4)
+    r = eng.result(s)
+    assert r["converged"] and r["iters"] == r_alone["iters"]
+    assert torch.equal(x, x_alone)
+    eng.remove(s)
+    dA.set_schedule(rows_per_block=64)  # released
+    assert dA.schedule_info()["rows_per_block"] == 64

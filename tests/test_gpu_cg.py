@@ -119,6 +119,21 @@ def test_breakdown_guard(ks):
         ks.cg_solve_local(dA, _dev(np.array([1.0, 1.0])), _dev(np.zeros(2)), 1e-10, 10)
 
 
+def test_nan_runs_to_max_iter_like_reference(ks, orc):
+    # cg_solve_local has no breakdown guard (cg_solver.hpp:165-167) and the
+    # distributed one, |pAp| < 1e-300, is false for NaN: a NaN right-hand side
+    # runs max_iter iterations and returns a non-converged NaN residual
+    A = g.laplacian_1d(8)
+    b = np.ones(8)
+    b[3] = np.nan
+    dA = _up(ks, A)
+    res = ks.cg_solve_local(dA, _dev(b), _dev(np.zeros(8)), 1e-10, 5)
+    _, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, np.zeros(8), 1e-10, 5)
+    assert res["iters"] == ro["iters"] == 5
+    assert not res["converged"] and not ro["converged"]
+    assert np.isnan(res["residual"]) and np.isnan(ro["residual"])
+
+
 def test_warm_start(ks, orc):
     A = g.laplacian_2d(32)
     b = np.random.default_rng(0).uniform(-1, 1, A.nrows)

@@ -123,3 +123,42 @@ def test_single_rank_share_is_whole_matrix(ks, orc):
     res = ks.cg_solve(D, _dev(b), x, tol=1e-8, max_iter=2000)
     xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, np.zeros(A.nrows), 1e-8, 2000)
     assert res["converged"] and abs(res["iters"] - ro["iters"]) <= max(1, 0.02 * ro["iters"])
+
+
+def test_halo_round_trip_every_rank(ks, orc):
+    """pack on every neighbour -> scatter into each rank's ghost block ->
+    dist_spmv equals the global rows (halo_exchange.hpp:209-303 on one GPU:
+    the transport between pack and scatter is the only part left out)."""
+    A = g.laplacian_3d(10)
+    size = 4
+    D = [ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=r, size=size)
+         for r in range(size)]
+    x = np.random.default_rng(9).uniform(-1, 1, A.ncols)
+    y_ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+    bufs = [ks.halo_pack(D[r], _dev(x[D[r].local_row_start:D[r].local_row_end]))
+            for r in range(size)]
+    for q in range(size):
+        xg = torch.full((D[q].n_ghost,), float("nan"), dtype=torch.float64, device="cuda")
+        for src, _, cnt in D[q].recv:
+            part = [(off, c) for dst, off, c in ks.send_ranges(D[src]) if dst == q]
+            assert len(part) == 1 and part[0][1] == cnt
+            off = part[0][0]
+            ks.halo_scatter(D[q], bufs[src][off:off + cnt].contiguous(), xg, src_rank=src)
+        assert np.array_equal(xg.cpu().numpy(), x[D[q].ghost_ids])
+        s, e = D[q].local_row_start, D[q].local_row_end
+        yl = torch.empty(e - s, dtype=torch.float64, device="cuda")
+        ks.dist_spmv(D[q], _dev(x[s:e]), yl, xg)
+        assert np.array_equal(yl.cpu().numpy(), y_ref[s:e])
+
+
+def test_halo_scatter_checks(ks):
+    A = g.laplacian_2d(8)
+    D = ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=1, size=3)
+    xg = torch.zeros(D.n_ghost, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):  # rank 1 owns none of its own ghosts
+        ks.halo_scatter(D, torch.zeros(8, dtype=torch.float64, device="cuda"), xg, src_rank=1)
+    with pytest.raises(ValueError):  # too few values from rank 0
+        ks.halo_scatter(D, torch.zeros(1, dtype=torch.float64, device="cuda"), xg, src_rank=0)
+    with pytest.raises(ValueError):  # wrong ghost block length
+        ks.halo_scatter(D, torch.zeros(8, dtype=torch.float64, device="cuda"),
+                        torch.zeros(3, dtype=torch.float64, device="cuda"), src_rank=0)

@@ -1,0 +1,270 @@
+"""GPU parity at the full sizes of the benchmarked workloads.
+
+Every workload the benchmark measures is checked here at its real shape,
+against the oracle (the reference's CPU path restated in C) or, where a
+size-independent property is the stronger check, against that property:
+
+* 3-D 7-point Laplacian 192^3 (n = 7,077,888, nnz = 49,324,032): the
+  delta-coded SpMV bit-identical to the oracle's row loop; plain FP64 CG to
+  1e-10 against the oracle's CG (iterations within +-2 %, FP64 true residual
+  below 1e-10, solutions within the conditioning bound).
+* Zipf(2.0) matrix, n = 4M, rows clipped to [1, 4096], seed 2: both layouts
+  (row tiles and column-partitioned) against the oracle, each row within
+  1e-12 of sum_j |a_ij x_j| (the rounding bound of a reordered sum of up to
+  4096 terms), rows of one column block bit-identical.
+* Rank 3 of the 8-slab partition of 192^3 (884,736 rows, 73,728 ghosts):
+  its ghost block delivered by the halo pack of ranks 2 and 4 and the halo
+  scatter into rank 3 (halo_exchange.hpp:209-303 without the transport),
+  then dist_spmv bit-identical to the oracle's global rows.
+* 128^3 lognormal diffusion (sigma 1, seed 3): FP64 and mixed-precision CG
+  both reach an FP64 true residual below 1e-10, and both solutions agree
+  with the oracle's FP64 solve.
+* The 4096-system batch (seed 4), FP64 and mixed: every system converged
+  with an FP64 true residual below 1e-10 (closed-form stencil, computed
+  without the library); 32 evenly spaced systems against the oracle's CG.
+* A served trace from the served stream's size distribution (n log-uniform
+  in [1k, 2M], 2-D p = 0.7, seed 5): every request's FP64 true residual
+  below 1e-10; two requests against the oracle's solution.
+
+Each test prints its case before the GPU work starts.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from num_amd import generators as g
+from _helpers import stencil_residual
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _say(*a):
+    print("fullsize:", *a, flush=True)
+
+
+def _conditioning(dims, shift):
+    """lambda_max / lambda_min bound of the shifted Dirichlet stencil:
+    Gershgorin gives lambda_max < 4 ndim + shift, and the Laplacian part is
+    positive definite, so lambda_min > shift."""
+    return (4.0 * len(dims) + shift) / shift
+
+
+@pytest.fixture(scope="module")
+def lap192(orc):
+    _say("building 3-D Laplacian 192^3 and its oracle SpMV")
+    A = g.laplacian_3d(192)
+    x = np.random.default_rng(1).uniform(-1, 1, A.ncols)
+    return A, x, orc.spmv(A.rowptr, A.colind, A.values, x)
+
+
+def test_spmv_lap3d_192_bitexact(ks, lap192):
+    A, x, y_ref = lap192
+    assert (A.nrows, A.nnz) == (7_077_888, 49_324_032)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    assert dA.index_format == "delta16"
+    _say("spmv 192^3 (delta16)")
+    y = torch.full((A.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    ks.spmv(dA, _dev(x), y)
+    assert np.array_equal(y.cpu().numpy(), y_ref)
+    # the int32-column kernel gives the same bits
+    dI = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, index="int32")
+    _say("spmv 192^3 (int32)")
+    y.fill_(float("nan"))
+    ks.spmv(dI, _dev(x), y)
+    assert np.array_equal(y.cpu().numpy(), y_ref)
+
+
+def test_cg_lap3d_192(ks, orc, lap192):
+    A = lap192[0]
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    db = _dev(b)
+    x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+    _say("cg 192^3 tol 1e-10")
+    res = ks.cg_solve_local(dA, db, x, tol=1e-10, max_iter=5000)
+    y = torch.empty_like(x)
+    ks.spmv(dA, x, y)
+    true = float(torch.linalg.norm(db - y) / torch.linalg.norm(db))
+    _say(f"gpu iters {res['iters']} residual {res['residual']:.3e} true {true:.3e}; oracle cg")
+    x_ref, r_ref = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=5000)
+    assert res["converged"] and r_ref["converged"]
+    assert abs(res["iters"] - r_ref["iters"]) <= max(1, 0.02 * r_ref["iters"]), (res, r_ref)
+    assert res["residual"] < 1e-10 and true < 1e-10
+    # lambda_max / lambda_min of the 192^3 Laplacian: (1 + cos(pi/193)) / (1 - cos(pi/193))
+    kappa = (1 + math.cos(math.pi / 193)) / (1 - math.cos(math.pi / 193))
+    err = float(np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref))
+    assert err <= kappa * 2e-10, err
+
+
+def _zipf_check(y, y_ref, y_abs, single_block=None):
+    assert np.all(np.abs(y - y_ref) <= 1e-12 * y_abs)
+    if single_block is not None:
+        assert np.array_equal(y[single_block], y_ref[single_block])
+
+
+def test_spmv_zipf_4m_both_layouts(ks, orc):
+    _say("building Zipf n=4M")
+    Z = g.zipf_matrix(4_000_000, 2.0, 4096, seed=2)
+    lens = np.diff(Z.rowptr)
+    assert Z.nrows == 4_000_000 and lens.min() >= 1 and lens.max() == 4096
+    x = np.random.default_rng(1).uniform(-1, 1, Z.ncols)
+    y_ref = orc.spmv(Z.rowptr, Z.colind, Z.values, x)
+    y_abs = orc.spmv(Z.rowptr, Z.colind, np.abs(Z.values), np.abs(x))
+    dx = _dev(x)
+    y = torch.empty(Z.nrows, dtype=torch.float64, device="cuda")
+    dR = ks.upload_csr(Z.rowptr, Z.colind, Z.values, Z.nrows, Z.ncols, layout="rows")
+    assert dR.layout == "rows"
+    _say("zipf spmv, row layout")
+    ks.spmv(dR, dx, y)
+    yr = y.cpu().numpy()
+    # rows of <= 32 entries are summed by one lane in entry order
+    _zipf_check(yr, y_ref, y_abs, single_block=lens <= 32)
+    del dR
+    dZ = ks.upload_csr(Z.rowptr, Z.colind, Z.values, Z.nrows, Z.ncols)
+    assert dZ.layout == "xcd"
+    _say("zipf spmv, column-partitioned layout")
+    y.fill_(float("nan"))
+    ks.spmv(dZ, dx, y)
+    yz = y.cpu().numpy()
+    # rows whose entries all lie in one column block keep the row loop's order
+    per = ks._NUM_XCD * ks._XCD_BLOCK_BYTES // 8
+    nb = ks._NUM_XCD * max(1, -(-Z.ncols // per))
+    bs = -(-Z.ncols // nb)
+    blk = Z.colind.astype(np.int64) // bs
+    first = np.repeat(blk[Z.rowptr[:-1]], lens)
+    one_block = np.ones(Z.nrows, dtype=bool)
+    rows = np.repeat(np.arange(Z.nrows), lens)
+    one_block[rows[blk != first]] = False
+    _zipf_check(yz, y_ref, y_abs, single_block=one_block & (lens <= 32))
+
+
+def test_slab_192_rank3_of_8_halo_round_trip(ks, orc, lap192):
+    A, x, y_ref = lap192
+    _say("distribute_csr 192^3 ranks 2, 3, 4 of 8")
+    D = {r: ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=r, size=8)
+         for r in (2, 3, 4)}
+    me = D[3]
+    plane = 192 * 192
+    assert (me.local_row_start, me.local_row_end) == (72 * plane, 96 * plane)
+    assert me.local.nrows == 884_736 and me.n_ghost == 73_728 and me.local.nnz == 6_174_720
+    assert len(me.interior_rows) == 811_008 and len(me.boundary_rows) == 73_728
+    # the exchange: ranks 2 and 4 pack what rank 3 needs, rank 3 scatters it
+    x_ghost = torch.full((me.n_ghost,), float("nan"), dtype=torch.float64, device="cuda")
+    for src in (2, 4):
+        s, e = D[src].local_row_start, D[src].local_row_end
+        _say(f"halo pack on rank {src}, scatter into rank 3")
+        buf = ks.halo_pack(D[src], _dev(x[s:e]))
+        part = [(off, cnt) for q, off, cnt in ks.send_ranges(D[src]) if q == 3]
+        assert len(part) == 1
+        off, cnt = part[0]
+        ks.halo_scatter(me, buf[off:off + cnt].contiguous(), x_ghost, src_rank=src)
+    assert np.array_equal(x_ghost.cpu().numpy(), x[me.ghost_ids])
+    s, e = me.local_row_start, me.local_row_end
+    xl = _dev(x[s:e])
+    yl = torch.full((e - s,), float("nan"), dtype=torch.float64, device="cuda")
+    _say("dist_spmv rank 3, one launch and interior-then-boundary")
+    ks.dist_spmv(me, xl, yl, x_ghost)
+    assert np.array_equal(yl.cpu().numpy(), y_ref[s:e])
+    yl.fill_(float("nan"))
+    ks.dist_spmv(me, xl, yl, x_ghost, overlap=True)
+    assert np.array_equal(yl.cpu().numpy(), y_ref[s:e])
+
+
+def test_mixed_vs_fp64_diffusion_128(ks, orc):
+    _say("building 128^3 lognormal diffusion")
+    A = g.diffusion_3d(128, sigma=1.0, seed=3)
+    assert (A.nrows, A.nnz) == (2_097_152, 14_581_760)
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    db = _dev(b)
+    y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+
+    def true_res(x):
+        ks.spmv(dA, x, y)
+        return float(torch.linalg.norm(db - y) / torch.linalg.norm(db))
+
+    x64 = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+    _say("fp64 cg 128^3")
+    r64 = ks.cg_solve_local(dA, db, x64, tol=1e-10, max_iter=20000)
+    x32 = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+    _say("mixed cg 128^3")
+    r32 = ks.cg_solve_mixed(dA, db, x32, tol=1e-10)
+    t64, t32 = true_res(x64), true_res(x32)
+    _say(f"fp64 {r64['iters']} iters true {t64:.3e}; mixed {r32['inner_total']} steps "
+         f"{r32['outer']} refinements true {t32:.3e}; oracle cg")
+    assert r64["converged"] and r32["converged"]
+    assert t64 < 1e-10 and t32 < 1e-10 and r32["residual"] < 1e-10
+    x_ref, r_ref = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10,
+                                      max_iter=20000)
+    assert r_ref["converged"]
+    assert abs(r64["iters"] - r_ref["iters"]) <= 0.02 * r_ref["iters"], (r64, r_ref)
+    xr = np.linalg.norm(x_ref)
+    # both reach 1e-10 on the same system: the oracle's two solvers agree to
+    # about 1e-9 here, so 1e-7 leaves room for rounding, not for a wrong solve
+    assert np.linalg.norm(x64.cpu().numpy() - x_ref) / xr < 1e-7
+    assert np.linalg.norm(x32.cpu().numpy() - x_ref) / xr < 1e-7
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+def test_batched_4096(ks, orc, mixed):
+    from num_amd.batched import CGEngine
+    specs = g.batched_specs(count=4096, seed=4)
+    eng = CGEngine(max_systems=4096)
+    _say(f"batched 4096 systems, mixed={mixed}")
+    slots = [eng.add_stencil(d, sh, sd, tol=1e-10, max_iter=20000, mixed=mixed)
+             for d, sh, sd in specs]
+    eng.run(16)
+    res = [eng.result(s) for s in slots]
+    assert all(r["converged"] and r["residual"] < 1e-10 for r in res)
+    _say("true residuals (closed-form stencil)")
+    worst = 0.0
+    for (d, sh, sd), s in zip(specs, slots):
+        worst = max(worst, stencil_residual(eng.solution(s), d, sh, sd))
+    assert worst < 1e-10, worst
+    _say(f"max true residual {worst:.3e}; oracle sample")
+    for k in range(0, 4096, 128):  # 32 evenly spaced systems
+        d, sh, sd = specs[k]
+        A = g.stencil(d, sh)
+        b = g.hash_uniform(A.nrows, sd)
+        x_ref, r_ref = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10,
+                                          max_iter=20000)
+        xs = eng.solution(slots[k]).cpu().numpy()
+        if not mixed:
+            it = res[k]["iters"]
+            assert abs(it - r_ref["iters"]) <= max(1, 0.02 * r_ref["iters"]), (k, it, r_ref)
+        bound = _conditioning(d, sh) * (1e-10 + r_ref["residual"])
+        assert np.linalg.norm(xs - x_ref) / np.linalg.norm(x_ref) <= bound, k
+
+
+def test_served_trace(ks, orc):
+    from num_amd.serve import CGServer, requests_from_trace
+    trace = g.served_requests(rate=1000.0, duration=10.0, seed=5, max_requests=40)
+    reqs = requests_from_trace(trace)
+    ns = [math.prod(r.dims) for r in reqs]
+    assert min(ns) >= 500 and max(ns) <= 2_100_000
+    srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True)
+    _say(f"served trace: {len(reqs)} requests, n from {min(ns)} to {max(ns)}")
+    summ = srv.run_open_loop(reqs, keep_solutions=True)
+    assert summ["completed"] == len(reqs) and summ["converged"] == len(reqs)
+    _say("per-request true residuals")
+    for r in srv.done:
+        tr = stencil_residual(r.solution, r.dims, r.shift, r.seed)
+        assert tr < 1e-10, (r.rid, r.dims, tr)
+    small = [r for r in sorted(srv.done, key=lambda q: q.rid) if r.n <= 300_000][:2]
+    assert len(small) == 2
+    for r in small:
+        _say(f"oracle solve of request {r.rid} dims {r.dims}")
+        A = g.stencil(r.dims, r.shift)
+        b = g.hash_uniform(A.nrows, r.seed)
+        x_ref, r_ref = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10,
+                                          max_iter=20000)
+        assert r_ref["converged"]
+        bound = _conditioning(r.dims, r.shift) * (1e-10 + r_ref["residual"])
+        err = np.linalg.norm(r.solution.cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+        assert err <= bound, (r.rid, err, bound)

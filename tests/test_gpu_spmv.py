@@ -368,3 +368,59 @@ def test_autotuner_keeps_results(ks, orc, tmp_path):
         assert np.array_equal(y[short], y_ref[short]) and rel(y, y_ref) < 1e-12
         again = tune_spmv(_up(ks, A), cache_file=cache)
         assert again["cached"] and again["schedule"] == t["schedule"]
+
+
+# -- host round-trip helpers (spmv_host_roundtrip / batch_spmv_host_roundtrip)
+
+def test_spmv_host_roundtrip_kats(ks, orc):
+    # spmv_kernel.hpp:186-217 through num_spmv_host: the 4x4 tridiagonal KAT
+    rp = np.array([0, 2, 5, 8, 10], np.int32)
+    ci = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3], np.int32)
+    v = np.array([2, -1, -1, 2, -1, -1, 2, -1, -1, 2], np.float64)
+    y = ks.spmv_host(rp, ci, v, 4, 4, np.array([1.0, 2, 3, 4]))
+    assert y.tolist() == [0.0, 0.0, 0.0, 5.0]
+    # the LCG KAT (unsorted, repeated columns) bit-exact against the oracle
+    A, x = g.lcg_matrix()
+    y = ks.spmv_host(A.rowptr, A.colind, A.values, A.nrows, A.ncols, x)
+    assert np.array_equal(y, orc.spmv(A.rowptr, A.colind, A.values, x))
+
+
+def test_batch_spmv_host_roundtrip_kats(ks, orc):
+    # batch_spmv_kernel.hpp:179-209 through num_batch_spmv_host: X = I -> Y = A
+    rp = np.array([0, 2, 5, 8, 10], np.int32)
+    ci = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3], np.int32)
+    v = np.array([2, -1, -1, 2, -1, -1, 2, -1, -1, 2], np.float64)
+    Y = ks.batch_spmv_host(rp, ci, v, 4, 4, np.eye(4))
+    dense = np.zeros((4, 4))
+    for r in range(4):
+        dense[r, ci[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
+    assert np.array_equal(Y, dense)
+    A, _ = g.lcg_matrix()
+    X = np.random.default_rng(4).uniform(-1, 1, (A.ncols, 5))
+    Y = ks.batch_spmv_host(A.rowptr, A.colind, A.values, A.nrows, A.ncols, X)
+    assert np.array_equal(Y, orc.batch_spmv(A.rowptr, A.colind, A.values, X))
+
+
+def test_host_roundtrip_errors(ks):
+    rp = np.array([0, 1, 2], np.int32)
+    with pytest.raises(ValueError):  # column out of range, refused before upload
+        ks.spmv_host(rp, np.array([0, 5], np.int32), np.ones(2), 2, 2, np.ones(2))
+    with pytest.raises(ValueError):
+        ks.spmv_host(rp, np.array([0, 1], np.int32), np.ones(2), 2, 2, np.ones(3))
+    with pytest.raises(ValueError):
+        ks.batch_spmv_host(rp, np.array([0, 1], np.int32), np.ones(2), 2, 2, np.ones((3, 2)))
+
+
+def test_device_create_rejects_bad_columns(ks):
+    # num_csr_create over device arrays checks column ids on the device
+    import ctypes as C
+    from num_amd._lib import call
+    rp = np.array([0, 2, 3], np.int32)
+    for bad in ([0, 2, 1], [0, -1, 1]):
+        d_rp = torch.from_numpy(rp).cuda()
+        d_ci = torch.tensor(bad, dtype=torch.int32, device="cuda")
+        d_v = torch.ones(3, dtype=torch.float64, device="cuda")
+        h = C.c_void_p()
+        with pytest.raises(ValueError, match="column index"):
+            call("num_csr_create", rp.ctypes.data_as(C.c_void_p), C.c_void_p(d_rp.data_ptr()),
+                 C.c_void_p(d_ci.data_ptr()), C.c_void_p(d_v.data_ptr()), 2, 2, C.byref(h))

@@ -195,7 +195,19 @@ def blas_axpy2_elementwise():
     return np.array_equal(oracle.axpy2(2.0, x, -0.5, y), 2.0 * x + (-0.5) * y)
 
 
-BLAS_CHECKS = [blas_exact_integer_dot, blas_fsum_dot, blas_axpy2_elementwise]
+def blas_fill_every_element():
+    x = np.random.default_rng(5).uniform(-1, 1, 1001)
+    return np.array_equal(oracle.fill(x, -2.5), np.full(1001, -2.5))
+
+
+def blas_copy_every_element():
+    src = np.random.default_rng(6).uniform(-1, 1, 1001)
+    dst = np.full(1001, np.nan)
+    return np.array_equal(oracle.copy(src, dst), src)
+
+
+BLAS_CHECKS = [blas_exact_integer_dot, blas_fsum_dot, blas_axpy2_elementwise,
+               blas_fill_every_element, blas_copy_every_element]
 
 
 @pytest.mark.parametrize("check", BLAS_CHECKS, ids=lambda f: f.__name__)
@@ -203,7 +215,7 @@ def test_blas_pinned(check):
     assert check()
 
 
-@pytest.mark.parametrize("m", [5, 6])
+@pytest.mark.parametrize("m", [5, 6, 8, 9])
 def test_blas_mutants_caught(m):
     with oracle.mutant(m):
         assert not all(c() for c in BLAS_CHECKS)
