@@ -81,6 +81,64 @@ def cpu_baseline(A, x, target_s=10.0):
                       "oracle.c row loop, OpenMP static"}, y
 
 
+def cg_config1(ks, reps=5):
+    """Plain FP64 CG on the CPU-path configuration: 2-D 5-point Dirichlet
+    Laplacian 256^2, b = U(-1,1) seed 0, x0 = 0, tol 1e-8, at most 2000
+    iterations; whole-solve wall time, median of ``reps`` after one warm-up
+    (python/benchmark_cg.py:152-182).  Timed three ways: the oracle (the
+    reference's CG sequence in C, OpenMP on every host core), SciPy's CG on
+    one thread (benchmark_cg.py:79-96) and cg_solve_local on the GPU."""
+    import torch
+    import oracle
+    import scipy.sparse.linalg as sla
+    from threadpoolctl import threadpool_limits
+    from num_amd import generators as g
+    A = g.laplacian_2d(256)
+    b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
+    threads = len(os.sched_getaffinity(0))
+    oracle.set_threads(threads)
+
+    def med(fn):
+        fn()
+        ts, out = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out
+    t_orc, (x_orc, r_orc) = med(lambda: oracle.cg_solve_local(
+        A.rowptr, A.colind, A.values, b, tol=1e-8, max_iter=2000))
+    S = A.to_scipy()
+    it_sp = [0]
+
+    def scipy_solve():
+        it_sp[0] = 0
+
+        def cb(_):
+            it_sp[0] += 1
+        return sla.cg(S, b, rtol=1e-8, maxiter=2000, callback=cb)
+    with threadpool_limits(limits=1):
+        t_sp, (x_sp, info_sp) = med(scipy_solve)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    db = torch.from_numpy(b).cuda()
+    x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+
+    def gpu_solve():
+        x.zero_()
+        torch.cuda.synchronize()
+        return ks.cg_solve_local(dA, db, x, tol=1e-8, max_iter=2000)
+    t_gpu, r_gpu = med(gpu_solve)
+    return {"matrix": "2-D 5-point Laplacian 256^2", "n": A.nrows, "nnz": A.nnz, "tol": 1e-8,
+            "oracle": {"ms": round(t_orc * 1e3, 3), "iters": r_orc["iters"],
+                       "residual": r_orc["residual"], "cores": threads, "kind": "port"},
+            "scipy": {"ms": round(t_sp * 1e3, 3), "iters": it_sp[0], "info": int(info_sp),
+                      "cores": 1},
+            "gpu": {"ms": round(t_gpu * 1e3, 3), "iters": r_gpu["iters"],
+                    "residual": r_gpu["residual"]},
+            "gpu_speedup_vs_oracle": round(t_orc / t_gpu, 2),
+            "timing": f"whole solve, median of {reps} after 1 warm-up"}
+
+
 def cg_bench(ks, A, dA, solves, tol=1e-10):
     import torch
     b = np.random.default_rng(0).uniform(-1, 1, A.nrows)
@@ -286,7 +344,7 @@ def batched_bench(ks, count=4096, sample=256):
     return out
 
 
-def served_bench(seconds=(3.0, 3.0, 1.0), base_requests=200):
+def served_bench(seconds=(60.0, 60.0, 10.0), base_requests=200):
     """Open-loop Poisson streams (lambda 100, 1k, 10k req/s; seed 5) of 2-D /
     3-D shifted Laplacians, n log-uniform in [1k, 2M], tol 1e-10, mixed
     refined solver, 200 GB admission budget; p50 / p99 latency and solves/s,
@@ -300,8 +358,12 @@ def served_bench(seconds=(3.0, 3.0, 1.0), base_requests=200):
         reqs = requests_from_trace(g.served_requests(rate=lam, duration=dur, seed=5))
         srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True)
         s = srv.run_open_loop(reqs, max_wall=dur + 600, progress_every=20.0)
+        offered = s["submitted"] / dur
         out[f"lambda_{int(lam)}"] = {
             "duration_s": dur, "submitted": s["submitted"], "completed": s["completed"],
+            # arrivals faster than the GPU serves: the queue grows for the whole
+            # stream, so p50 / p99 there measure the stream's length
+            "saturated": bool(s["solves_per_s"] < 0.95 * offered),
             "converged": s["converged"], "solves_per_s": round(s["solves_per_s"], 1),
             "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
             "max_residual": s["max_residual"], "peak_device_GB": round(srv.peak_in_use / 1e9, 2)}
@@ -322,7 +384,9 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "spmv_lap3d192_pmc.txt")
 def measured_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading for
-    a plain 1 GiB copy), or None when the summary is absent."""
+    a plain 1 GiB copy), or None when the summary is absent.  The figure is
+    read from that file, not measured in this run: the summary names the
+    kernel build it was taken on."""
     try:
         for line in open(PMC_SUMMARY):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -357,11 +421,12 @@ def main():
                     help="skip the mixed-precision, batched and served CG measurements")
     ap.add_argument("--solvers", default="mixed,batched,served",
                     help="which solver forms to measure (comma list)")
-    ap.add_argument("--served-seconds", type=float, default=3.0,
-                    help="duration of the lambda = 100 and 1k served streams (60 = full)")
-    ap.add_argument("--served-10k-seconds", type=float, default=1.0,
-                    help="duration of the lambda = 10k served stream (60 = full; it then "
-                         "drains for a few minutes)")
+    ap.add_argument("--served-seconds", type=float, default=60.0,
+                    help="duration of the lambda = 100 and 1k served streams")
+    ap.add_argument("--served-10k-seconds", type=float, default=10.0,
+                    help="duration of the lambda = 10k served stream (arrivals exceed what "
+                         "the GPU serves, so the queue grows for as long as it runs and then "
+                         "drains; 60 = the full configuration, about 4 minutes)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the slab-share, Zipf and multi-RHS SpMV measurements")
     ap.add_argument("--extras", default="slab,zipf,batch",
@@ -409,7 +474,10 @@ def main():
                    "bytes_per_step": B},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(), "kernel_us": round(avg * 1e6, 2),
+                     "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) + " (rocprofv3 --pmc "
+                                       "FETCH_SIZE / WRITE_SIZE, not measured in this run)",
+                     "kernel_us": round(avg * 1e6, 2),
                      "kernel_min_bytes": Bk,
                      "kernel_min_frac": round(Bk / avg / 1e9 / HBM_PEAK_GBS, 4)},
     }
@@ -417,6 +485,8 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         cb, y_ref = cpu_baseline(A, x)
+        log("bench: CG on the CPU-path configuration (2-D 256^2): oracle, SciPy, GPU")
+        cb["cg_lap2d256"] = cg_config1(ks)
         out["cpu_baseline"] = cb
         out["parity_max_abs"] = float(np.max(np.abs(y_gpu - y_ref)))
     extras = set() if args.no_extra else set(args.extras.split(","))

@@ -289,6 +289,12 @@ int num_engine_info(const num_engine* E, int32_t* n_live, int32_t* n_tiles, int3
 int num_stencil_fill(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, double shift,
                      const int32_t* d_rowptr, int32_t* d_colind, double* d_vals, void* stream);
 
+/* d_out[i] = 2u - 1 for i < n, u the top 53 bits of SplitMix64 of
+ * seed * 0xD1B54A32D192ED03 + (i + 1) * 0x9E3779B97F4A7C15: the right-hand
+ * side num_engine_add_stencil assembles, for callers that build the same
+ * system themselves (no reference counterpart: a workload generator). */
+int num_hash_uniform(int64_t n, uint64_t seed, double* d_out, void* stream);
+
 /* ---- halo pack / scatter of one rank's share (src/halo_exchange.hpp) --- */
 /* buf[i] = x[idx[i]] for i < count (halo_pack, halo_exchange.hpp:222-235). */
 int num_halo_pack(const int32_t* d_idx, int32_t count, const double* d_x, double* d_buf,

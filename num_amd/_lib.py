@@ -85,6 +85,7 @@ SIGNATURES = {
     "num_engine_remove": (C.c_int, [vp, i32]),
     "num_engine_info": (C.c_int, [vp, pi32, pi32, pi32]),
     "num_stencil_fill": (C.c_int, [i32, i32, i32, i32, f64, vp, vp, vp, vp]),
+    "num_hash_uniform": (C.c_int, [i64, C.c_uint64, vp, vp]),
     "num_halo_pack": (C.c_int, [vp, i32, vp, vp, vp]),
     "num_halo_scatter": (C.c_int, [vp, i32, vp, i32, vp]),
 }

@@ -679,6 +679,13 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, int64_t i)
     return 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
 }
 
+__global__ void hash_uniform_kernel(int64_t n, uint64_t seed, double* __restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = hash_uniform(seed, i);
+}
+
 // rowptr[i] of the Dirichlet stencil on an nx x ny x nz grid (x fastest):
 // i diagonals plus, per axis of stride s and extent L (block B = s*L), the
 // rows before i that have a lower neighbour, (i/B)(B-s) + max(0, i%B - s),
@@ -1243,6 +1250,17 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     A->engine_holds.fetch_add(1);
     E->held[s] = A;
     *slot_out = s;
+    return NUM_OK;
+}
+
+int num_hash_uniform(int64_t n, uint64_t seed, double* d_out, void* stream)
+{
+    if (n < 0) return fail(NUM_EINVAL, "n must be >= 0");
+    if (n == 0) return NUM_OK;
+    if (!d_out) return fail(NUM_EINVAL, "d_out is NULL");
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hash_uniform_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, seed, d_out);
+    NUM_LAUNCH_CHECK();
     return NUM_OK;
 }
 

@@ -382,6 +382,16 @@ def upload_stencil(dims, shift: float = 0.0) -> CrsMatrix:
     return CrsMatrix._from_device(rp, d_rp, d_ci, d_v, n, n)
 
 
+def hash_uniform(n: int, seed: int) -> torch.Tensor:
+    """generators.hash_uniform(n, seed) computed on the GPU (the right-hand
+    side the engine assembles for a served request)."""
+    _ensure_init()
+    out = torch.empty(max(1, int(n)), dtype=torch.float64,
+                      device=torch.device("cuda", torch.cuda.current_device()))[: int(n)]
+    call("num_hash_uniform", int(n), int(seed) % 2**64, _ptr(out), _stream())
+    return out
+
+
 # ------------------------------------------------------------------ SpMV --
 
 def spmv(A: CrsMatrix, x, y, kokkoskernels: bool = True, *, sync: bool = True) -> None:

@@ -225,17 +225,18 @@ def requests_from_trace(trace, tol: float = 1e-10):
 
 def one_at_a_time(requests, mixed: bool = False, max_iter: int = 20000) -> dict:
     """Baseline: each request built and solved by itself, one after another,
-    with plain FP64 cg_solve_local (or cg_solve_mixed).  ``solves_per_s``
-    counts the solve calls only (matrix and b set-up excluded, which favours
-    the baseline); ``wall_solves_per_s`` includes the set-up."""
-    from .generators import hash_uniform
+    with plain FP64 cg_solve_local (or cg_solve_mixed).  The set-up is built
+    on the device as the engine builds it (the stencil assembled by
+    upload_stencil, b by the same counter-based draws, x = 0).
+    ``solves_per_s`` counts the solve calls only (set-up excluded, which
+    favours the baseline); ``wall_solves_per_s`` includes the set-up."""
     ks.init(0)
     t0 = time.perf_counter()
     t_solve = 0.0
     iters = []
     for req in requests:
         A = ks.upload_stencil(req.dims, req.shift)
-        b = torch.from_numpy(hash_uniform(A.nrows, req.seed)).cuda()
+        b = ks.hash_uniform(A.nrows, req.seed)
         x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t1 = time.perf_counter()

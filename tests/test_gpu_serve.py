@@ -152,3 +152,10 @@ def test_pipelined_launch_equals_step(ks, mixed):
         out.append([(eng.result(s)["iters"], eng.solution(s).clone()) for s in slots])
     for (i0, x0), (i1, x1) in zip(*out):
         assert i0 == i1 and torch.equal(x0, x1)
+
+
+@pytest.mark.parametrize("n,seed", [(0, 1), (1, 7), (1000, 12345), (300_001, 2**40 + 3)])
+def test_device_hash_uniform_equals_host(ks, n, seed):
+    # the device draws of num_hash_uniform are the host generator's, bit for bit
+    got = ks.hash_uniform(n, seed).cpu().numpy()
+    assert np.array_equal(got, g.hash_uniform(n, seed))
