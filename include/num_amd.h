@@ -190,6 +190,18 @@ int64_t num_dot_workspace_bytes(int64_t n);
  * device (vec_local_dot, vec_ops.hpp:47-59, minus its host read-back). */
 int num_dot(int64_t n, const double* d_x, const double* d_y, double* d_out, void* d_work,
             void* stream);
+/* num_dot without a workspace argument and without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The same two
+ * kernels as num_dot, bit-identical results, with the stage-1 partials in a
+ * static 16 KiB device array (2048 doubles, enough for any n: the partial
+ * count is capped) that each GPU holds until the process ends, its address
+ * resolved by num_init (else by the first call on a device).  Calls no
+ * hipMalloc / hipFree.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (different streams, or threads) race and must be
+ * serialised by the caller. */
+int num_dot_noalloc(int64_t n, const double* d_x, const double* d_y, double* d_out,
+                    void* stream);
 /* *d_out = sqrt(sum x_i^2) (vec_local_norm2, vec_ops.hpp:64-69). */
 int num_norm2(int64_t n, const double* d_x, double* d_out, void* d_work, void* stream);
 

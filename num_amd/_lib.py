@@ -63,6 +63,7 @@ SIGNATURES = {
     "num_copy": (C.c_int, [i64, vp, vp, vp]),
     "num_dot_workspace_bytes": (i64, [i64]),
     "num_dot": (C.c_int, [i64, vp, vp, vp, vp, vp]),
+    "num_dot_noalloc": (C.c_int, [i64, vp, vp, vp, vp]),
     "num_norm2": (C.c_int, [i64, vp, vp, vp, vp]),
     "num_cg_workspace_bytes": (i64, [vp]),
     "num_cg_solve": (C.c_int, [vp, vp, vp, f64, i32, vp, i64, C.POINTER(CgResult), vp]),

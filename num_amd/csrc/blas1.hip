@@ -1,8 +1,59 @@
 // blas1.hip -- vector ops of src/vec_ops.hpp on the device: axpy, axpy2,
 // scale, fill, copy, and a deterministic two-stage dot / norm2.
+#include <mutex>
+
 #include "blas1.hpp"
 
 namespace numamd {
+
+// Scratch of num_dot_noalloc: the stage-1 partial sums of the two-stage dot,
+// one double per stage-1 workgroup (dot_grid caps them at kMaxRedBlocks, so
+// any n fits: 16 KiB).  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` only names the symbol:
+// kernels get the per-device address resolved by noalloc_setup.
+constexpr int kNoAllocPartials = kMaxRedBlocks;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int noalloc_setup()
+{
+    int dev = -1;
+    NUM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(NUM_EINVAL, "device %d outside the %d devices the static scratch serves", dev,
+                    kMaxScratchDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return NUM_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(NUM_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                    hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+    return NUM_OK;
+}
+
+// The current device's scratch address (resolved on its first use there).
+static int noalloc_scratch(double** out)
+{
+    int dev = -1;
+    NUM_HIP(hipGetDevice(&dev));
+    if (dev >= 0 && dev < kMaxScratchDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (g_scratch_ptr[dev]) {
+            *out = g_scratch_ptr[dev];
+            return NUM_OK;
+        }
+    }
+    const int rc = noalloc_setup();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    *out = g_scratch_ptr[dev];
+    return NUM_OK;
+}
 
 __global__ __launch_bounds__(kRedBlock) void axpy_kernel(int64_t n, double a,
                                                          const double* __restrict__ x,
@@ -160,6 +211,24 @@ int num_dot(int64_t n, const double* d_x, const double* d_y, double* d_out, void
     if (!d_out || !d_work) return fail(NUM_EINVAL, "out or workspace is NULL");
     if (n > 0 && (!d_x || !d_y)) return fail(NUM_EINVAL, "NULL vector");
     return launch_dot(n, d_x, d_y, d_out, (double*)d_work, 0, (hipStream_t)stream);
+}
+
+int num_dot_noalloc(int64_t n, const double* d_x, const double* d_y, double* d_out,
+                    void* stream)
+{
+    if (n < 0) return fail(NUM_EINVAL, "n must be >= 0");
+    if (!d_out) return fail(NUM_EINVAL, "out is NULL");
+    if (n > 0 && (!d_x || !d_y)) return fail(NUM_EINVAL, "NULL vector");
+    const int g = dot_grid(n);
+    if (g > kNoAllocPartials)
+        return fail(NUM_EINVAL, "dot of length %lld needs %d partials; the static scratch "
+                    "holds %d", (long long)n, g, kNoAllocPartials);
+    double* partial = nullptr;
+    const int rc = noalloc_scratch(&partial);
+    if (rc) return rc;
+    // the same two kernels as num_dot: stage 1 writes partial[0, g) before
+    // stage 2 reads them, in stream order, so nothing needs clearing
+    return launch_dot(n, d_x, d_y, d_out, partial, 0, (hipStream_t)stream);
 }
 
 int num_norm2(int64_t n, const double* d_x, double* d_out, void* d_work, void* stream)

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <new>
 
+#include "blas1.hpp"
 #include "csr.hpp"
 
 namespace numamd {
@@ -384,7 +385,8 @@ int num_init(int32_t device_id)
     if (device_id < 0 || device_id >= count)
         return fail(NUM_EINVAL, "device_id %d out of range (%d devices)", device_id, count);
     NUM_HIP(hipSetDevice(device_id));
-    return NUM_OK;
+    // the static scratch of num_dot_noalloc, resolved once per device
+    return noalloc_setup();
 }
 
 int num_finalize(void) { return NUM_OK; }
