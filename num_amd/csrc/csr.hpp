@@ -16,6 +16,13 @@ constexpr int kSpmvBS = 256;  // threads per row-block workgroup
 constexpr int kNptChoices[] = {5, 7, 12, 16};
 constexpr int kNptMax = 16;
 
+// Pass-1 tiles of the column-partitioned SpMV: up to kSegRpt segments and
+// kSegNpt entries per thread.
+constexpr int kSegNpt = 8;
+constexpr int kSegRpt = 4;
+constexpr int kSegRows = kSegRpt * kSpmvBS;
+constexpr int kSegNnz = kSegNpt * kSpmvBS;
+
 // Row-tile schedule over rows [row_begin, row_end): consecutive rows packed
 // greedily into tiles of at most rows_per_block rows and nnz_per_block
 // non-zeros; a row with more than nnz_per_block non-zeros is a tile alone.
@@ -61,6 +68,7 @@ struct num_csr {
     const num_csr* xcd_seg = nullptr;            // the caller's; outlives A
     const int32_t* xcd_row_seg_ptr = nullptr;    // [nrows + 1]
     const int32_t* xcd_row_seg_idx = nullptr;    // [nseg]
+    numamd::Schedule xcd_sched;                  // owned: the segment tiles of pass 1
     int32_t* xcd_tiles = nullptr;                // owned: [tiles], grouped by XCD
     int32_t xcd_off[numamd::kNumXcd + 1] = {};   // group bounds in xcd_tiles
     numamd::Schedule sched;             // whole-matrix schedule
@@ -75,9 +83,12 @@ struct num_csr {
 namespace numamd {
 
 // Builds `s` for rows [r0, r1) of A and uploads it.  Returns NUM_OK or error.
+// rows_per_block may reach max_rows (the row-tile kernels walk one row per
+// thread: kSpmvBS; the column-partitioned pass 1 walks kSegRows); a row
+// longer than long_row entries gets a tile of its own.
 int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
                    int32_t nnz_per_block, int32_t chunk, int32_t nt64, int32_t nt32,
-                   Schedule& s);
+                   Schedule& s, int32_t max_rows = kSpmvBS, int32_t long_row = kLongRow);
 // Default tile capacity for A: the smallest kNptChoices entry covering the
 // mean row length, times the block size.
 int32_t default_nnz_per_block(const num_csr* A);

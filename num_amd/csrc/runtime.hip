@@ -82,10 +82,10 @@ constexpr int32_t kMinPlaneChunk = 3;
 
 int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_block,
                    int32_t nnz_per_block, int32_t chunk, int32_t nt64, int32_t nt32,
-                   Schedule& s)
+                   Schedule& s, int32_t max_rows, int32_t long_row)
 {
-    if (rows_per_block < 1 || rows_per_block > kSpmvBS)
-        return fail(NUM_EINVAL, "rows_per_block must be in [1, %d], got %d", kSpmvBS,
+    if (rows_per_block < 1 || rows_per_block > max_rows)
+        return fail(NUM_EINVAL, "rows_per_block must be in [1, %d], got %d", max_rows,
                     rows_per_block);
     if (nnz_per_block < 1 || nnz_per_block > kNptMax * kSpmvBS)
         return fail(NUM_EINVAL, "nnz_per_block must be in [1, %d], got %d", kNptMax * kSpmvBS,
@@ -100,7 +100,7 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
     while (r < r1) {
         const int32_t start = r;
         const int32_t len = rp[r + 1] - rp[r];
-        if (len > nnz_per_block || len > kLongRow) {
+        if (len > nnz_per_block || len > long_row) {
             ++r;
             ++longs;
             tiles.push_back({start, kWalkLong, rp[start], 0});
@@ -110,7 +110,7 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
         int32_t maxlen = 0;
         while (r < r1 && r - start < rows_per_block) {
             const int32_t l = rp[r + 1] - rp[r];
-            if (nz + l > nnz_per_block || l > kLongRow) break;
+            if (nz + l > nnz_per_block || l > long_row) break;
             nz += l;
             maxlen = std::max(maxlen, l);
             ++r;
@@ -559,12 +559,23 @@ int num_csr_attach_xcd(num_csr* A, const num_csr* S, int32_t nblocks,
         return fail(NUM_EINVAL, "row -> segment map is NULL");
     int rc = check_device(A);
     if (rc) return rc;
-    // tiles of the segment matrix by the XCD of their first segment's block
-    const int32_t nt = S->sched.nblocks;
+    // the segment tiles of pass 1: up to kSegRows segments and kSegNnz
+    // entries each (the dense-gather walk keeps every lane busy whatever the
+    // segment lengths, and a wave sums a long segment's products from LDS),
+    // a segment longer than a tile holds alone
+    Schedule seg;
+    rc = build_schedule(S, 0, S->nrows, kSegRows, kSegNnz, 1, 1, 1, seg, kSegRows, kSegNnz);
+    if (rc) return rc;
+    const int32_t nt = seg.nblocks;
     std::vector<TileEntry> tiles((size_t)nt + 1);
-    if (nt > 0)
-        NUM_HIP(hipMemcpy(tiles.data(), S->sched.d_tiles, tiles.size() * sizeof(TileEntry),
-                          hipMemcpyDeviceToHost));
+    if (nt > 0) {
+        const hipError_t e = hipMemcpy(tiles.data(), seg.d_tiles, tiles.size() * sizeof(TileEntry),
+                                       hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            free_schedule(seg);
+            return fail(NUM_EHIP, "segment schedule read-back failed: %s", hipGetErrorString(e));
+        }
+    }
     std::vector<int32_t> group[kNumXcd];
     int32_t j = 0;
     for (int32_t t = 0; t < nt; ++t) {
@@ -581,16 +592,20 @@ int num_csr_attach_xcd(num_csr* A, const num_csr* S, int32_t nblocks,
     }
     int32_t* d_tiles = nullptr;
     if (nt > 0) {
-        NUM_HIP(hipMalloc(&d_tiles, flat.size() * sizeof(int32_t)));
-        hipError_t e = hipMemcpy(d_tiles, flat.data(), flat.size() * sizeof(int32_t),
-                                 hipMemcpyHostToDevice);
+        hipError_t e = hipMalloc(&d_tiles, flat.size() * sizeof(int32_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(d_tiles, flat.data(), flat.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            (void)hipFree(d_tiles);
+            if (d_tiles) (void)hipFree(d_tiles);
+            free_schedule(seg);
             return fail(NUM_EHIP, "XCD tile list upload failed: %s", hipGetErrorString(e));
         }
     }
     if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
     A->xcd_tiles = d_tiles;
+    free_schedule(A->xcd_sched);
+    A->xcd_sched = seg;
     std::copy(off, off + kNumXcd + 1, A->xcd_off);
     A->xcd_row_seg_ptr = d_row_seg_ptr;
     A->xcd_row_seg_idx = d_row_seg_idx;
@@ -605,6 +620,7 @@ int num_csr_destroy(num_csr* A)
         return fail(NUM_ERUNTIME, "matrix is held by %d engine system(s); remove them first",
                     (int)A->engine_holds.load());
     if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
+    free_schedule(A->xcd_sched);
     free_schedule(A->sched);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
     if (A->owns) {

@@ -322,52 +322,181 @@ static void dispatch_batch(const num_csr* A, const double* X, double* Y, int k,
 // ---------------------------------------------------------------------------
 // Column-partitioned SpMV (num_csr_attach_xcd), for matrices whose x gathers
 // are spread far wider than an XCD's 4 MB L2 (random columns: the Zipf
-// power-law rows).  x is cut into column blocks of up to 4 MB, and block j is served
-// by XCD j % 8 only, so each XCD's L2 holds the one x block it gathers from
-// instead of filling a 128-byte line per random 8-byte gather.  Pass 1 runs
-// the row-tile kernel (trimmed short-row walk) over the segment matrix (a
-// row per (block, row) pair, block-major) with each workgroup's tile taken
-// from its own XCD's group
-// (workgroup b runs on XCD b % 8), writing one partial sum per segment.
-// Pass 2 sums each row's partials in block order.  A row whose entries all
-// fall in one block keeps the row loop's sum bit for bit (0.0 + s = s);
-// other rows are the sum of their blocks' partials.
+// power-law rows).  x is cut into column blocks of up to 4 MB, and block j is
+// served by XCD j % 8 only, so each XCD's L2 holds the one x block it gathers
+// from instead of filling a 128-byte line per random 8-byte gather.  Pass 1
+// (xcd_seg_kernel) walks the segment matrix (a row per (block, row) pair,
+// block-major) in tiles of up to kSegRows segments and kSegNnz entries taken
+// from its own XCD's group (workgroup b runs on XCD b % 8), writing one
+// partial sum per segment.  Its gathers are dense: thread t gathers x for the
+// entries it stages, whatever the segment lengths (3 entries on average, 62 %
+// single; a row-per-thread walk left most lanes idle), and the segments are
+// summed from the products in LDS.  Pass 2 sums each row's partials in block
+// order.  A row whose entries all fall in one block keeps the row loop's sum
+// bit for bit (0.0 + s = s); other rows are the sum of their blocks' partials.
+// Zipf n = 4M: 290 us with the row-tile kernel as pass 1, 225 us now (pass 1
+// 186 us, bound by L2 hit latency on ~24M random gathers: the L1 waits on L2
+// data 72 % of its cycles; pass 2 39 us).
 // ---------------------------------------------------------------------------
 struct XcdGroups {
     int32_t off[kNumXcd + 1];
 };
 
-template <int NPT, bool NT>
-__global__ __launch_bounds__(kSpmvBS) void spmv_xcd_kernel(
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-    const double* __restrict__ vals, GatherX<double> x, const TileEntry* __restrict__ sched,
-    const int32_t* __restrict__ xcd_tiles, XcdGroups g, EpiY<double> epi)
+// Pass 1 over one tile of segments [s0, s1) holding entries [j0, j1):
+// thread t stages entries j0 + t + u*BS (u < kSegNpt) and gathers their x at
+// once (every lane busy, however short the segments), products to LDS; then
+// thread t sums segments t + r*BS (r < kSegRpt) of <= kShortRow entries in
+// index order from 0.0 (bit-identical to the row loop), and a longer segment
+// of the tile goes to a wave, whose 64 lanes stride over its products (4
+// groups of 64 in order, then the tail) before a fixed butterfly.  A
+// segment longer than a tile holds (kSegNnz entries) is a tile of its own,
+// summed by the whole workgroup straight from memory.
+template <bool NT>
+__global__ __launch_bounds__(kSpmvBS) void xcd_seg_kernel(
+    const int32_t* __restrict__ segptr, const int32_t* __restrict__ colind,
+    const double* __restrict__ vals, const double* __restrict__ x,
+    const TileEntry* __restrict__ sched, const int32_t* __restrict__ xcd_tiles, XcdGroups g,
+    double* __restrict__ part)
 {
-    __shared__ double lv[kSpmvBS * NPT];
-    __shared__ int32_t lc[kSpmvBS * NPT];
+    constexpr int BS = kSpmvBS, NPT = kSegNpt, RPT = kSegRpt, NW = BS / kWave;
+    __shared__ double lp[BS * NPT];
+    __shared__ unsigned long long wmask[RPT * NW];
     const int k = blockIdx.x % kNumXcd;
     const int q = blockIdx.x / kNumXcd;
     if (q >= g.off[k + 1] - g.off[k]) return;
     const int tile = xcd_tiles[g.off[k] + q];
-    x.prepare();
-    double ca, cb;
-    spmv_tile_body<double, kSpmvBS, NPT, NT, EpiY<double>, GatherX<double>, true>(
-        tile, rowptr, colind, vals, x, sched, epi, lv, lc, ca, cb);
+    const TileEntry e0 = sched[tile], e1 = sched[tile + 1];
+    const int t = threadIdx.x, w = t / kWave, l = t & (kWave - 1);
+    const int s0 = e0.row, j0 = e0.nz0;
+    const int ns = e1.row - s0, nz = e1.nz0 - j0;
+    if (e0.wide == kWalkLong) {
+        // one long segment: 8 loads in flight per thread, fixed tree
+        double acc = 0.0;
+        const int j1 = j0 + nz, last = j1 - 1;
+        for (int j = j0 + t; j < j1; j += 8 * BS) {
+            double v[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int jj = min(j + u * BS, last);
+                v[u] = ld_mat<NT>(vals + jj);
+                xv[u] = x[ld_mat<NT>(colind + jj)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j + u * BS < j1) acc += v[u] * xv[u];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (l == 0) lp[w] = acc;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) sum += lp[i];
+            __builtin_nontemporal_store(sum, part + s0);
+        }
+        return;
+    }
+    // segment bounds (branch-free: a thread past the tile reads the end twice)
+    int sb[RPT], se[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int i = t + r * BS;
+        const int li = min(i, ns);
+        sb[r] = segptr[s0 + li];
+        se[r] = segptr[s0 + li + (i < ns ? 1 : 0)];
+    }
+    if (nz > 0) {
+        const int last = j0 + nz - 1;
+        double v[NPT], xv[NPT];
+        int c[NPT];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) {
+            const int j = min(j0 + t + u * BS, last);
+            v[u] = ld_mat<NT>(vals + j);
+            c[u] = ld_mat<NT>(colind + j);
+        }
+        // clamped slots repeat the tile's last entry: one line, coalesced
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) {
+            const int kk = t + u * BS;
+            if (kk < nz) lp[kk] = v[u] * xv[u];
+        }
+    }
+    __syncthreads();
+    const bool wide = e0.wide != kWalkShort;  // uniform over the workgroup
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int i = t + r * BS;
+        const int b = sb[r] - j0, e = se[r] - j0;
+        if (i < ns && e - b <= kShortRow) {
+            double acc = 0.0;
+            for (int kk = b; kk < e; ++kk) acc += lp[kk];
+            __builtin_nontemporal_store(acc, part + s0 + i);
+        }
+        if (wide) {
+            const unsigned long long bal = __ballot(i < ns && e - b > kShortRow);
+            if (l == 0) wmask[r * NW + w] = bal;
+        }
+    }
+    if (!wide) return;
+    __syncthreads();
+    // the m-th longer segment (in segment order) goes to wave m % NW
+    int m = 0;
+    for (int i = 0; i < RPT * NW; ++i) {
+        unsigned long long bits = wmask[i];
+        while (bits) {
+            const int bpos = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            if ((m++ % NW) != w) continue;
+            // wmask[r * NW + v] covers segments r*BS + v*64 + bit
+            const int seg = (i / NW) * BS + (i % NW) * kWave + bpos;
+            const int mb = segptr[s0 + seg] - j0, me = segptr[s0 + seg + 1] - j0;
+            double acc = 0.0;
+            int kk = mb + l;
+            for (; kk + 3 * kWave < me; kk += 4 * kWave) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc += lp[kk + u * kWave];
+            }
+            for (; kk < me; kk += kWave) acc += lp[kk];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (l == 0) __builtin_nontemporal_store(acc, part + s0 + seg);
+        }
+    }
 }
 
+// Pass 2: y[r] = the row's segment partials summed in block order from 0.0.
+// A row has at most one segment per column block; up to 8 segment ids, then
+// their partials, are loaded at once (lanes past the row's segments masked
+// off) instead of one dependent pair per segment.
 __global__ void xcd_reduce_kernel(int32_t n, const int32_t* __restrict__ row_seg_ptr,
                                   const int32_t* __restrict__ row_seg_idx,
                                   const double* __restrict__ part, double* __restrict__ y)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
+    const int b = row_seg_ptr[r], e = row_seg_ptr[r + 1];
     double acc = 0.0;
-    for (int i = row_seg_ptr[r]; i < row_seg_ptr[r + 1]; ++i) acc += part[row_seg_idx[i]];
+    for (int i = b; i < e; i += 8) {
+        int idx[8];
+        double pv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i + u < e) idx[u] = row_seg_idx[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i + u < e) pv[u] = part[idx[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i + u < e) acc += pv[u];
+    }
     __builtin_nontemporal_store(acc, y + r);
 }
 
-template <int NPT>
-static void launch_xcd_npt(const num_csr* A, const double* x, double* part, hipStream_t st)
+static void launch_xcd(const num_csr* A, const double* x, double* part, hipStream_t st)
 {
     const num_csr* S = A->xcd_seg;
     XcdGroups g;
@@ -375,15 +504,9 @@ static void launch_xcd_npt(const num_csr* A, const double* x, double* part, hipS
     for (int k = 0; k <= kNumXcd; ++k) g.off[k] = A->xcd_off[k];
     for (int k = 0; k < kNumXcd; ++k) most = std::max(most, g.off[k + 1] - g.off[k]);
     if (most == 0) return;
-    const TileEntry* tiles = static_cast<const TileEntry*>(S->sched.d_tiles);
-    if (S->sched.nt64)
-        spmv_xcd_kernel<NPT, true><<<most * kNumXcd, kSpmvBS, 0, st>>>(
-            S->rowptr, S->colind, S->vals, GatherX<double>{x}, tiles, A->xcd_tiles, g,
-            EpiY<double>{part});
-    else
-        spmv_xcd_kernel<NPT, false><<<most * kNumXcd, kSpmvBS, 0, st>>>(
-            S->rowptr, S->colind, S->vals, GatherX<double>{x}, tiles, A->xcd_tiles, g,
-            EpiY<double>{part});
+    const TileEntry* tiles = static_cast<const TileEntry*>(A->xcd_sched.d_tiles);
+    xcd_seg_kernel<true><<<most * kNumXcd, kSpmvBS, 0, st>>>(S->rowptr, S->colind, S->vals, x,
+                                                             tiles, A->xcd_tiles, g, part);
 }
 
 // One row of the Dirichlet stencil on an nx x ny x nz grid (x fastest):
@@ -450,12 +573,7 @@ int num_spmv_xcd(const num_csr* A, const double* d_x, double* d_y, double* d_wor
     int rc = check_device(A);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    switch (A->xcd_seg->sched.npt) {
-    case 5: launch_xcd_npt<5>(A, d_x, d_work, st); break;
-    case 7: launch_xcd_npt<7>(A, d_x, d_work, st); break;
-    case 12: launch_xcd_npt<12>(A, d_x, d_work, st); break;
-    default: launch_xcd_npt<16>(A, d_x, d_work, st); break;
-    }
+    launch_xcd(A, d_x, d_work, st);
     NUM_LAUNCH_CHECK();
     if (A->nrows > 0) {
         xcd_reduce_kernel<<<(A->nrows + 255) / 256, 256, 0, st>>>(

@@ -82,10 +82,7 @@ struct GatherSplit {
 // Body for one tile given its schedule entry and the next one.  Leaves this
 // thread's epilogue contributions in ca/cb (zero for threads that finish no
 // row).
-// TRIM: the last (up to 7) slots of a short-row walk gather only as far as
-// the longest row of the wave needs (one ballot per slot) instead of always
-// 7 (clamped) - for rows of 1-3 entries (the column-partitioned segments).
-template <typename T, int BS, int NPT, bool NT, class Epi, class Gather, bool TRIM = false>
+template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_tile_run(
     const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ colind, const T* __restrict__ vals, const Gather& x,
@@ -186,16 +183,8 @@ __device__ __forceinline__ void spmv_tile_run(
             for (int u = 0; u < 8; ++u) acc += vv[u] * xv[u];
         }
         T vv[7], xv[7];
-        int lim = 7;
-        if constexpr (TRIM) {
-            lim = 0;
-#pragma unroll
-            for (int u = 0; u < 7; ++u)
-                if (__ballot(k + u < re) != 0) lim = u + 1;
-        }
 #pragma unroll
         for (int u = 0; u < 7; ++u) {
-            if (TRIM && u >= lim) break;
             const int kk = k + u;
             const bool ok = kk < re;
             const int ks = ok ? kk : 0;  // slot 0 is always inside the LDS array
@@ -485,14 +474,14 @@ __device__ __forceinline__ void spmv_d16_run(
     epi(r0 + t, acc, ca, cb, x);
 }
 
-template <typename T, int BS, int NPT, bool NT, class Epi, class Gather, bool TRIM = false>
+template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_tile_body(
     int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const T* __restrict__ vals, const Gather& x, const TileEntry* __restrict__ sched,
     const Epi& epi, T* __restrict__ lv, int32_t* __restrict__ lc, double& ca, double& cb)
 {
-    spmv_tile_run<T, BS, NPT, NT, Epi, Gather, TRIM>(sched[tile], sched[tile + 1], rowptr,
-                                                     colind, vals, x, epi, lv, lc, ca, cb);
+    spmv_tile_run<T, BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], rowptr, colind,
+                                               vals, x, epi, lv, lc, ca, cb);
 }
 
 // Generic kernel: per-tile partial sums (if Epi::kRed) go to
