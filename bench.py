@@ -305,18 +305,23 @@ def batched_bench(ks, count=4096, sample=256):
     solve calls only)."""
     import torch
     from num_amd import generators as g
+    from num_amd.autotune import tune_engine
     from num_amd.batched import CGEngine
     specs = g.batched_specs(count=count, seed=4)
     out = {"systems": count, "sum_n": int(sum(d[0] * d[1] for d, _, _ in specs))}
     for mixed in (False, True):
+        # the engine schedule picked by the autotuner on 512 of the systems
+        tuned = tune_engine(specs, mixed=mixed, sample=512)
+        sch = tuned["schedule"]
         best = None
         for _ in range(2):  # the second run reuses warm allocations
-            eng = CGEngine(max_systems=count)
+            eng = CGEngine(max_systems=count, nt32=sch["nt32"])
+            eng.set_schedule(sch["ka_chunk"])
             slots = [eng.add_stencil(d, sh, sd, tol=1e-10, max_iter=20000, mixed=mixed)
                      for d, sh, sd in specs]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            eng.run(16)
+            eng.run(sch["iters_per_step"])
             el = time.perf_counter() - t0
             res = [eng.result(s) for s in slots]
             best = el if best is None else min(best, el)
@@ -324,7 +329,12 @@ def batched_bench(ks, count=4096, sample=256):
         out[key] = {"seconds": round(best, 4), "solves_per_s": round(count / best, 1),
                     "all_converged": all(r["converged"] for r in res),
                     "max_residual": max(r["residual"] for r in res),
-                    "max_iters": max(r["iters"] for r in res)}
+                    "max_iters": max(r["iters"] for r in res),
+                    "tuned_schedule": sch,
+                    "tuning": {"sample": tuned["sample"], "candidates": tuned["timed"],
+                               "sample_solves_per_s": round(tuned["solves_per_s"], 1),
+                               "sample_default_solves_per_s":
+                                   round(tuned["default_solves_per_s"], 1)}}
     idx = list(range(0, count, max(1, count // sample)))
     t_solve = 0.0
     for j, k in enumerate(idx[:2] + idx):  # the first two are warm-up solves
@@ -350,13 +360,26 @@ def served_bench(seconds=(60.0, 60.0, 10.0), base_requests=200):
     refined solver, 200 GB admission budget; p50 / p99 latency and solves/s,
     against the same requests solved one at a time with plain FP64 CG."""
     from num_amd import generators as g
+    from num_amd.autotune import tune_engine
     from num_amd.serve import CGServer, one_at_a_time, requests_from_trace
     CGServer().run_open_loop(requests_from_trace(
         g.served_requests(rate=1000, duration=1.0, seed=99, nmax=20000, max_requests=8)))
-    out = {}
+    # the engine schedule for this request mix, tuned on requests of another
+    # seed drawn from the same distribution
+    probe = g.served_requests(rate=10000, duration=1.0, seed=55, max_requests=256)
+    # (8 iterations per step: arrivals are admitted between steps)
+    tuned = tune_engine([(q["dims"], q["shift"], q["seed"]) for q in probe], mixed=True,
+                        sample=256, iters=(8,))
+    sch = tuned["schedule"]
+    out = {"tuned_schedule": sch,
+           "tuning": {"sample": tuned["sample"], "candidates": tuned["timed"],
+                      "sample_solves_per_s": round(tuned["solves_per_s"], 1),
+                      "sample_default_solves_per_s": round(tuned["default_solves_per_s"], 1)}}
     for lam, dur in zip((100.0, 1000.0, 10000.0), seconds):
         reqs = requests_from_trace(g.served_requests(rate=lam, duration=dur, seed=5))
-        srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True)
+        srv = CGServer(budget_bytes=200e9, max_systems=4096, mixed=True,
+                       iters_per_step=sch["iters_per_step"], nt32=sch["nt32"],
+                       ka_chunk=sch["ka_chunk"])
         s = srv.run_open_loop(reqs, max_wall=dur + 600, progress_every=20.0)
         offered = s["submitted"] / dur
         out[f"lambda_{int(lam)}"] = {

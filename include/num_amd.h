@@ -273,6 +273,11 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
  * and replay it on the caller's stream; the graph is re-captured when the
  * live set or the iteration count changes.  Results are identical. */
 int num_engine_set_graph(num_engine* E, int32_t on);
+/* Tile deal of the engine's SpMV launch: ka_chunk consecutive tiles per XCD
+ * turn (0 = chosen from the tile count: min(8, tiles / 64)).  Only the
+ * tile-to-workgroup mapping changes, never a result (every system's partial
+ * sums keep their order).  Set by the autotuner (no reference counterpart). */
+int num_engine_set_schedule(num_engine* E, int32_t ka_chunk);
 /* Queue `iters` iterations of every live system and return; at most two
  * steps are in flight (with two queued, the older is waited for first). */
 int num_engine_launch(num_engine* E, int32_t iters, void* stream);

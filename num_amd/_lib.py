@@ -78,6 +78,7 @@ SIGNATURES = {
     "num_engine_add_stencil": (C.c_int, [vp, i32, i32, i32, i32, f64, C.c_uint64, f64, i32, i32,
                                          f64, vp, i64, vp, pi32]),
     "num_engine_set_graph": (C.c_int, [vp, i32]),
+    "num_engine_set_schedule": (C.c_int, [vp, i32]),
     "num_engine_launch": (C.c_int, [vp, i32, vp]),
     "num_engine_poll": (C.c_int, [vp, i32, pi32, pi32]),
     "num_engine_step": (C.c_int, [vp, i32, pi32, vp]),

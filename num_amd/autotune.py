@@ -1,4 +1,4 @@
-"""Schedule autotuner for the row-tile SpMV.
+"""Schedule autotuners: the row-tile SpMV and the batched / served engine.
 
 The row-tile kernel has four knobs (include/num_amd.h,
 num_csr_set_schedule): entries per tile (which also fixes the LDS staging
@@ -10,7 +10,10 @@ keeps the fastest, and remembers the choice per matrix signature (shape,
 nnz, row-length profile) so a re-uploaded matrix of the same structure is
 not re-timed.  Every candidate computes the same y (the kernel sums rows of
 <= 32 entries in index order whatever the tiling), so tuning never changes
-results beyond the 1e-12 bar of longer rows.
+results beyond the 1e-12 bar of longer rows.  ``tune_engine`` picks the
+engine's tile deal, iterations per step and FP32 load kind for a workload of
+systems (below).  Both replace the reference's fixed heuristic
+(src/spmv_kernel.hpp:102-109, rows per team from the mean row length).
 """
 from __future__ import annotations
 
@@ -18,6 +21,7 @@ import hashlib
 import itertools
 import json
 import os
+import time
 
 import numpy as np
 import torch
@@ -36,11 +40,13 @@ def signature(A: ks.CrsMatrix) -> str:
     return f"{A.nrows}x{A.ncols}:{A.nnz}:{h[:16]}"
 
 
-def candidates(A: ks.CrsMatrix):
+def candidates(A: ks.CrsMatrix, max_npt: int = 16):
     """Schedules worth timing for A: staging depths around the mean row
-    length, rows per tile 256 / 128, chunks 1 / 16 / 64, both load kinds."""
+    length (at most max_npt entries per thread), rows per tile 256 / 128,
+    chunks 1 / 16 / 64, both load kinds."""
     mean = A.nnz / max(1, A.nrows)
-    depths = [d for d in (5, 7, 12, 16) if d >= min(16, mean * 0.8)][:2] or [16]
+    allowed = [d for d in (5, 7, 12, 16) if d <= max_npt]
+    depths = [d for d in allowed if d >= min(allowed[-1], mean * 0.8)][:2] or [allowed[-1]]
     for npt, rows, chunk, nt in itertools.product(depths, (256, 128), (1, 16, 64), (1, 0)):
         yield {"rows_per_block": rows, "nnz_per_block": npt * 256, "chunk": chunk, "nt64": nt}
 
@@ -61,10 +67,12 @@ def _time(A, x, y, reps: int) -> float:
 
 
 def tune_spmv(A: ks.CrsMatrix, reps: int = 20, cache_file: str | None = None,
-              retune: bool = False) -> dict:
+              retune: bool = False, max_npt: int = 16) -> dict:
     """Pick and apply the fastest schedule for A; returns
-    {'schedule': ..., 'us': ..., 'default_us': ..., 'timed': n}."""
-    key = signature(A)
+    {'schedule': ..., 'us': ..., 'default_us': ..., 'timed': n}.  max_npt
+    bounds the staging depth (the batched engine's tile kernel holds 7
+    entries per thread)."""
+    key = signature(A) + (f":npt{max_npt}" if max_npt < 16 else "")
     cache = dict(_CACHE)
     if cache_file and os.path.exists(cache_file):
         with open(cache_file) as f:
@@ -76,10 +84,12 @@ def tune_spmv(A: ks.CrsMatrix, reps: int = 20, cache_file: str | None = None,
     x = torch.rand(A.ncols, dtype=torch.float64, device="cuda")
     y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
     A.set_schedule()  # defaults
+    if A.schedule_info()["nnz_per_block"] > max_npt * 256:  # within the depth bound
+        A.set_schedule(nnz_per_block=max_npt * 256)
     default_us = _time(A, x, y, reps)
     best = {"schedule": {}, "us": default_us}
     timed = 1
-    for cand in candidates(A):
+    for cand in candidates(A, max_npt):
         A.set_schedule(**cand)
         us = _time(A, x, y, reps)
         timed += 1
@@ -91,6 +101,93 @@ def tune_spmv(A: ks.CrsMatrix, reps: int = 20, cache_file: str | None = None,
     _CACHE[key] = {"schedule": out["schedule"], "us": out["us"], "default_us": default_us}
     if cache_file:
         cache[key] = _CACHE[key]
+        with open(cache_file, "w") as f:
+            json.dump(cache, f, indent=1)
+    return dict(out, cached=False)
+
+
+# ---------------------------------------------------------------- engine ---
+# The batched / served engine (batched.CGEngine) advances many systems with
+# one tile launch per iteration.  Its knobs: the tile deal to the XCDs
+# (ka_chunk: tiles per XCD turn), the iterations queued per step and whether
+# the FP32 matrix copies stream non-temporally.  None of them changes a
+# result (each system's partial sums keep their order), so tuning only moves
+# time.  tune_engine times a batch drawn evenly from the workload under each
+# candidate and remembers the fastest per workload class.
+
+ENGINE_KA_CHUNKS = (0, 1, 2, 4, 8, 16, 32)
+ENGINE_ITERS = (8, 16, 32)
+
+_ENGINE_CACHE: dict = {}
+
+
+def engine_signature(specs, mixed: bool) -> str:
+    """Workload class of a list of stencil systems (dims, shift, seed): the
+    mode, the number of systems and a histogram of log2 n and grid rank."""
+    n = np.array([int(np.prod(d)) for d, _, _ in specs], dtype=np.float64)
+    rank = np.array([len(d) for d, _, _ in specs])
+    hist = np.histogram(np.log2(np.maximum(n, 1)), bins=24, range=(0, 24))[0]
+    h = hashlib.sha1(hist.tobytes() + np.bincount(rank, minlength=4).tobytes()).hexdigest()
+    return f"{'mixed' if mixed else 'fp64'}:{len(specs)}:{h[:16]}"
+
+
+def _time_engine(sample, mixed, ka_chunk, iters, nt32, tol, max_iter, delta):
+    from .batched import CGEngine
+    best = None
+    for _ in range(2):  # the second run reuses warm allocations
+        eng = CGEngine(max_systems=len(sample), nt32=nt32)
+        eng.set_schedule(ka_chunk)
+        for d, sh, sd in sample:
+            eng.add_stencil(d, sh, sd, tol=tol, max_iter=max_iter, mixed=mixed, delta=delta)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run(iters)
+        el = time.perf_counter() - t0
+        best = el if best is None else min(best, el)
+    return len(sample) / best
+
+
+def tune_engine(specs, mixed: bool = False, sample: int = 512, tol: float = 1e-10,
+                max_iter: int = 20000, delta: float = 0.003, cache_file: str | None = None,
+                retune: bool = False, iters=ENGINE_ITERS) -> dict:
+    """Pick the engine schedule for a workload of stencil systems (dims,
+    shift, seed): every candidate (ka_chunk x iterations per step x FP32
+    load kind) solves a batch of ``sample`` systems drawn evenly from
+    ``specs``; the fastest wins.  Returns {'schedule': {'ka_chunk',
+    'iters_per_step', 'nt32'}, 'solves_per_s', 'default_solves_per_s',
+    'timed', 'cached'}; the choice is remembered per workload class.
+    ``iters``: the iterations-per-step candidates (a server that must admit
+    arrivals between steps keeps them short: pass (8,))."""
+    iters = tuple(int(i) for i in iters)
+    key = engine_signature(specs, mixed) + f":it{'-'.join(map(str, iters))}"
+    cache = dict(_ENGINE_CACHE)
+    if cache_file and os.path.exists(cache_file):
+        with open(cache_file) as f:
+            cache.update(json.load(f))
+    if key in cache and not retune:
+        return dict(cache[key], cached=True)
+    step = max(1, len(specs) // sample)
+    batch = list(specs[::step])[:sample]
+    default = {"ka_chunk": 0, "iters_per_step": 16 if 16 in iters else iters[0], "nt32": True}
+    args = (tol, max_iter, delta)
+    default_rate = _time_engine(batch, mixed, default["ka_chunk"], default["iters_per_step"],
+                                default["nt32"], *args)
+    best = dict(default)
+    best_rate = default_rate
+    timed = 1
+    for ka, it, nt in itertools.product(ENGINE_KA_CHUNKS, iters, (True, False)):
+        cand = {"ka_chunk": ka, "iters_per_step": it, "nt32": nt}
+        if cand == default:
+            continue
+        rate = _time_engine(batch, mixed, ka, it, nt, *args)
+        timed += 1
+        if rate > best_rate:
+            best, best_rate = cand, rate
+    out = {"schedule": best, "solves_per_s": best_rate, "default_solves_per_s": default_rate,
+           "sample": len(batch), "timed": timed}
+    _ENGINE_CACHE[key] = out
+    if cache_file:
+        cache[key] = out
         with open(cache_file, "w") as f:
             json.dump(cache, f, indent=1)
     return dict(out, cached=False)

@@ -81,18 +81,31 @@ class CGEngine:
             call("num_engine_set_graph", h, 1)
         self._sys = {}
         self.max_systems = int(max_systems)
+        self.ka_chunk = 0
+
+    def set_schedule(self, ka_chunk: int = 0) -> None:
+        """Tiles per XCD turn in the engine's SpMV launch (0 = by tile count);
+        the autotuner's knob.  Never changes a result."""
+        call("num_engine_set_schedule", self._h, int(ka_chunk))
+        self.ka_chunk = int(ka_chunk)
 
     @staticmethod
     def work_bytes(n: int, mixed: bool = False) -> int:
         return int(_lib.load().num_engine_work_bytes(int(n), 1 if mixed else 0))
 
     def add(self, A: CrsMatrix, b, x, tol: float = 1e-10, max_iter: int = 1000,
-            mixed: bool = False, delta: float = 0.003, work=None) -> int:
-        """Admit A x = b (x holds x0, receives the solution).  Returns the slot."""
+            mixed: bool = False, delta: float = 0.003, work=None, tune: bool = False) -> int:
+        """Admit A x = b (x holds x0, receives the solution).  Returns the slot.
+        ``tune``: first let the autotuner pick A's row-tile schedule among
+        those the engine's tile kernel runs (<= 7 entries per thread),
+        remembered per matrix structure."""
         b = _dev_tensor(b, "b", 1)
         x = _dev_tensor(x, "x", 1)
         if A.nrows != A.ncols or b.shape[0] != A.nrows or x.shape[0] != A.nrows:
             raise ValueError("engine.add: A must be square and b, x of length A.nrows")
+        if tune:
+            from .autotune import tune_spmv
+            tune_spmv(A, max_npt=ENGINE_MAX_NNZ_PER_TILE // 256)
         info = A.schedule_info()
         if info["nnz_per_block"] > ENGINE_MAX_NNZ_PER_TILE:
             # only the tile capacity changes: rows per tile, XCD chunk and

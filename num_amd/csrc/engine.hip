@@ -858,6 +858,8 @@ struct num_engine {
     std::deque<Flight> flights;
     // optional hipGraph of one step (captured on a private stream, replayed
     // on the caller's; re-captured when the launch lists or iters change)
+    // XCD chunk of the tile launch (tiles per XCD deal); 0: by tile count
+    int32_t ka_chunk = 0;
     bool use_graph = false;
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1006,7 +1008,7 @@ static int eng_enqueue_step(num_engine* E, int32_t iters, hipStream_t st)
     // plane offset applies (csr.hpp: plane_chunk); small chunks keep the
     // XCDs on neighbouring tiles (served lambda = 10k: 3257 -> 3288 solves/s,
     // batched FP64 52.4k -> 53.2k against 64-tile chunks)
-    const int chunk = std::min(8, std::max(1, E->nbt / 64));
+    const int chunk = E->ka_chunk > 0 ? E->ka_chunk : std::min(8, std::max(1, E->nbt / 64));
     const int grid_a = chunk_grid(E->nbt, chunk);
     for (int32_t k = 0; k < iters; ++k) {
         if (E->nt64) {
@@ -1160,6 +1162,18 @@ int num_engine_set_graph(num_engine* E, int32_t on)
         NUM_HIP(hipStreamCreateWithFlags(&E->cap_stream, hipStreamNonBlocking));
     E->use_graph = on != 0;
     E->g_valid = false;
+    return NUM_OK;
+}
+
+int num_engine_set_schedule(num_engine* E, int32_t ka_chunk)
+{
+    if (!E) return fail(NUM_EINVAL, "engine is NULL");
+    if (ka_chunk < 0 || ka_chunk > 4096)
+        return fail(NUM_EINVAL, "ka_chunk must be in [0, 4096], got %d", ka_chunk);
+    if (ka_chunk != E->ka_chunk) {
+        E->ka_chunk = ka_chunk;
+        E->g_valid = false;  // the captured step's grid changes
+    }
     return NUM_OK;
 }
 

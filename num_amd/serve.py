@@ -74,12 +74,14 @@ class CGServer:
     the served stream: 3.0k solves/s at lambda = 10k against 2.7k with 0.1,
     as each refinement costs an FP64 iteration of its system); ``nt32``:
     stream the FP32 matrix copies non-temporally (measured slower on the
-    served stream: 2.68k against 2.74k solves/s)."""
+    served stream: 2.68k against 2.74k solves/s).  ``ka_chunk``,
+    ``iters_per_step`` and ``nt32`` are what autotune.tune_engine picks."""
 
     def __init__(self, budget_bytes: float = 200e9, max_systems: int = 4096,
                  mixed: bool = True, iters_per_step: int = 8, max_iter: int = 20000,
                  delta: float = 0.003, nt64: bool = True, nt32: bool = False,
-                 timeout_s: float | None = None, admit_per_pump: int = 32):
+                 timeout_s: float | None = None, admit_per_pump: int = 32,
+                 ka_chunk: int = 0):
         self.budget = float(budget_bytes)
         self.in_use = 0.0
         self.peak_in_use = 0.0
@@ -92,6 +94,7 @@ class CGServer:
         # never leaves the device without queued work for long
         self.admit_per_pump = int(admit_per_pump)
         self.engine = CGEngine(max_systems=max_systems, nt64=nt64, nt32=nt32)
+        self.engine.set_schedule(ka_chunk)
         self.max_systems = int(max_systems)
         self.queue: collections.deque = collections.deque()
         self.running: dict = {}    # slot -> (request, bytes)

@@ -228,3 +228,50 @@ def test_schedule_held_while_in_engine(ks):
     eng.remove(s)
     dA.set_schedule(rows_per_block=64)  # released
     assert dA.schedule_info()["rows_per_block"] == 64
+
+
+def test_engine_autotune_keeps_results(ks):
+    """The engine tuner picks a schedule among its candidates, remembers it per
+    workload class, and no candidate changes a result: the tuned batch equals
+    the default-schedule batch bit for bit."""
+    from num_amd import autotune
+    from num_amd.batched import CGEngine
+    specs = g.batched_specs(count=48, nmin=64, nmax=4000, seed=9)
+    for mixed in (False, True):
+        t = autotune.tune_engine(specs, mixed=mixed, sample=16)
+        sch = t["schedule"]
+        assert sch["ka_chunk"] in autotune.ENGINE_KA_CHUNKS
+        assert sch["iters_per_step"] in autotune.ENGINE_ITERS
+        assert t["timed"] > 1 and not t["cached"]
+        assert autotune.tune_engine(specs, mixed=mixed, sample=16)["cached"]
+        sols = []
+        for ka, it, nt in ((0, 16, True), (sch["ka_chunk"], sch["iters_per_step"], sch["nt32"]),
+                           (1, 8, False)):
+            eng = CGEngine(max_systems=len(specs), nt32=nt)
+            eng.set_schedule(ka)
+            slots = [eng.add_stencil(d, sh, sd, tol=1e-10, max_iter=5000, mixed=mixed)
+                     for d, sh, sd in specs]
+            eng.run(it)
+            assert all(eng.result(s)["converged"] for s in slots)
+            sols.append([eng.solution(s).clone() for s in slots])
+        for other in sols[1:]:
+            assert all(torch.equal(a, b) for a, b in zip(sols[0], other))
+
+
+def test_engine_add_tune_bounded_depth(ks):
+    """add(..., tune=True) lets the SpMV tuner pick the matrix schedule within
+    the engine's 7 entries per thread; the solve matches the untuned one."""
+    from num_amd.batched import CGEngine
+    A, dA = _upload(ks, (70, 50), 0.02)
+    b = np.random.default_rng(5).uniform(-1, 1, A.nrows)
+    xs = []
+    for tune in (False, True):
+        eng = CGEngine(max_systems=1)
+        x = _dev(np.zeros(A.nrows))
+        s = eng.add(dA, _dev(b), x, tol=1e-10, max_iter=3000, tune=tune)
+        assert dA.schedule_info()["nnz_per_block"] <= 7 * 256
+        eng.run(8)
+        assert eng.result(s)["converged"]
+        eng.remove(s)
+        xs.append(x.cpu().numpy())
+    assert np.linalg.norm(xs[0] - xs[1]) <= 1e-8 * np.linalg.norm(xs[0])
