@@ -292,13 +292,20 @@ static void launch_batch_wave(const num_csr* A, const double* X, double* Y, int 
 // even k = 40-56 stay 3-4 % faster on 32 lanes of 16-byte loads.
 constexpr int kBatchWaveK = 64;
 constexpr int kBatchWaveK1 = 32;
+// Rows per wave: 2 when the wave's X loads cover whole 128-column passes
+// (k a multiple of 128 with 16-byte loads: 100^3, k = 128 647 -> 617 us,
+// k = 256 1349 -> 1313 us), else 4 (k = 64 403 against 414 us with 2, k = 96
+// 498 against 529 us).
 constexpr int kBatchWaveRows = 4;
+constexpr int kBatchWaveRowsFull = 2;
 static void dispatch_batch(const num_csr* A, const double* X, double* Y, int k,
                            hipStream_t st)
 {
     const bool v2 = (k % 2 == 0) && ((uintptr_t)X % 16 == 0) && ((uintptr_t)Y % 16 == 0);
     if (k >= kBatchWaveK || (!v2 && k > kBatchWaveK1)) {
-        if (v2 && k > kWave) launch_batch_wave<2, kBatchWaveRows>(A, X, Y, k, st);
+        if (v2 && k > kWave && k % (2 * kWave) == 0)
+            launch_batch_wave<2, kBatchWaveRowsFull>(A, X, Y, k, st);
+        else if (v2 && k > kWave) launch_batch_wave<2, kBatchWaveRows>(A, X, Y, k, st);
         else launch_batch_wave<1, kBatchWaveRows>(A, X, Y, k, st);
         return;
     }
