@@ -257,3 +257,65 @@ def test_full_width_update_groups(ks, orc, mixed):
     assert rel(xs[0], xo) < 1e-7
     if not mixed:
         assert abs(r["iters"] - ro["iters"]) <= max(1, int(0.02 * ro["iters"]))
+
+
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 4, 5, 6, 7])
+def test_iterate_after_each_iteration_count(ks, orc, max_iter):
+    # the x update runs every other iteration (two at once); a solve that
+    # stops after an odd or an even number of iterations must return the
+    # iterate of exactly that many iterations (the oracle's x after max_iter)
+    A = g.laplacian_2d(24)
+    b = np.random.default_rng(7).uniform(-1, 1, A.nrows)
+    x, r = _solve(ks, A, b, 1e-14, max_iter)
+    xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, np.zeros(A.nrows), 1e-14,
+                                max_iter)
+    assert r["iters"] == ro["iters"] == max_iter and not r["converged"]
+    assert rel(x, xo) < 1e-13, rel(x, xo)
+
+
+@pytest.mark.parametrize("L", [5, 6, 9])
+def test_converged_iterate_odd_and_even_counts(ks, orc, L):
+    # n distinct eigenvalues: CG converges in a few iterations, odd or even
+    A = g.laplacian_1d(L)
+    b = np.random.default_rng(L).uniform(-1, 1, L)
+    x, r = _solve(ks, A, b, 1e-12, 100)
+    xo, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, np.zeros(L), 1e-12, 100)
+    assert r["converged"] and r["iters"] == ro["iters"]
+    assert rel(x, xo) < 1e-12
+    dense = np.zeros((L, L))
+    for i in range(L):
+        dense[i, A.colind[A.rowptr[i]:A.rowptr[i + 1]]] = A.values[A.rowptr[i]:A.rowptr[i + 1]]
+    assert np.linalg.norm(dense @ x - b) <= 1e-11 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_breakdown_keeps_last_iterate(ks, mixed):
+    # A = diag(1, 0), b = [1, 1]: the first step gives x = [2, 2]; the second
+    # direction [0, 2] has p.Ap = 0 (breakdown).  The returned iterate must
+    # hold the first step's update: the FP64 solver runs its x update every
+    # other iteration, so that update is still pending when the solve stops.
+    A = _csr([[(0, 1.0)], [(1, 0.0)]])
+    dA = _up(ks, A)
+    x = _dev(np.zeros(2))
+    with pytest.raises(RuntimeError):
+        if mixed:
+            ks.cg_solve_mixed(dA, _dev(np.array([1.0, 1.0])), x, tol=1e-10, max_iter=10)
+        else:
+            ks.cg_solve_local(dA, _dev(np.array([1.0, 1.0])), x, 1e-10, 10)
+    assert x.cpu().numpy().tolist() == [2.0, 2.0]
+
+
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 4, 5, 8])
+def test_mixed_stop_after_each_step_count(ks, max_iter):
+    # the mixed solver ends with an FP64 refinement whose true residual it
+    # reports: the returned x must be the iterate that residual belongs to,
+    # whatever the parity of the last step (a pending x32 update is folded in)
+    A = g.laplacian_2d(20)
+    b = np.random.default_rng(3).uniform(-1, 1, A.nrows)
+    dA = _up(ks, A)
+    x = _dev(np.zeros(A.nrows))
+    r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-14, max_iter=max_iter)
+    assert not r["converged"] and r["inner_total"] == max_iter
+    xh = x.cpu().numpy()
+    true = np.linalg.norm(b - A.to_scipy() @ xh) / np.linalg.norm(b)
+    assert abs(true - r["residual"]) <= 1e-12 * r["residual"], (true, r["residual"])
