@@ -480,10 +480,10 @@ def main():
     log("bench: timing SpMV")
     wall, avg = time_spmv(ks, dA, dx, dy, args.steps, args.warmup)
     B = spmv_bytes(A.nrows, A.ncols, A.nnz)
-    # bytes the kernel itself must move: with delta-coded column ids it reads
-    # 2 index bytes per entry instead of the formula's 4, and one length byte
-    # per row instead of the 4(n+1) bytes of rowptr
-    Bk = B - (2 * A.nnz + 3 * A.nrows + 4 if dA.index_format == "delta16" else 0)
+    # bytes the kernel itself must move: with window slots (or delta-coded
+    # column ids) it reads 2 index bytes per entry instead of the formula's
+    # 4, and one length byte per row instead of the 4(n+1) bytes of rowptr
+    Bk = B - (2 * A.nnz + 3 * A.nrows + 4 if dA.index_format in ("delta16", "window16") else 0)
     value = B * args.steps / wall / 1e9
     achieved = B / avg / 1e9
     out = {

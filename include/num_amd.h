@@ -126,6 +126,20 @@ int num_spmv_xcd(const num_csr* A, const double* d_x, double* d_y, double* d_wor
                  void* stream);
 /* *delta16 = 1 if a delta-coded index is attached (num_csr_attach_d16). */
 int num_csr_index_info(const num_csr* A, int32_t* delta16);
+/* Column windows over the delta-coded index (no reference counterpart: an
+ * index layout of this build, used by the FP64 SpMV and CG kernels in place
+ * of the delta codes).  With enable = 1 (the default at num_csr_attach_d16)
+ * every tile of the schedule must read its x from at most 4 contiguous
+ * column ranges of at most 1536 columns in all -- a banded or stencil
+ * matrix: the ranges are fetched once per tile into LDS with coalesced
+ * loads, and the buffer of num_csr_attach_d16 is re-coded with each entry's
+ * slot in them (same entries, same order: bit-identical results).  When any
+ * tile does not fit the delta codes stay.  enable = 0 restores the delta
+ * codes.  num_csr_set_schedule rebuilds the windows for the new tiles.
+ * Synchronises the stream. */
+int num_csr_set_windows(num_csr* A, int32_t enable, void* stream);
+/* *windowed = 1 if the column windows are in use. */
+int num_csr_window_info(const num_csr* A, int32_t* windowed);
 /* Fails with NUM_ERUNTIME (and frees nothing) while an engine system admitted
  * with A (num_engine_add) has not been removed. */
 int num_csr_destroy(num_csr* A);

@@ -138,10 +138,14 @@ struct EpiPAp {
     double* partial_a;
     double* partial_b;
     __device__ __forceinline__ bool skip() const { return st->done != 0; }
-    __device__ __forceinline__ void operator()(int row, double v, double& ca, double&,
+    __device__ __forceinline__ void operator()(int row, double v, double& ca, double& cb,
                                                const GatherPNew<double>& g) const
     {
-        const double pn = g(row);
+        own(row, v, ca, cb, g(row));
+    }
+    // pn = p_new[row], computed by the caller (the windowed walk has it in LDS)
+    __device__ __forceinline__ void own(int row, double v, double& ca, double&, double pn) const
+    {
         __builtin_nontemporal_store(pn, (st->psel ? p0 : p1) + row);
         __builtin_nontemporal_store(v, Ap + row);
         ca = pn * v;

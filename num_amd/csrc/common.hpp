@@ -74,6 +74,17 @@ __host__ __device__ inline const uint8_t* d16_lens(const uint16_t* codes, int64_
     return reinterpret_cast<const uint8_t*>(codes + nnz + kD16Pad);
 }
 
+// Windowed sliced layout (spmv_sell_run).  Column windows of a tile: at
+// most kXwWin contiguous column ranges holding every column the tile's
+// entries read, kXwSlots columns in all.  Per tile the record holds the
+// windows' first columns (lo[kXwWin]), their cumulative ends in LDS slots
+// (end[kXwWin]; an unused window has lo 0 and the previous end), the tile's
+// first 256-entry slice and its width (longest row, <= kSellW entries).
+constexpr int kXwWin = 4;
+constexpr int kXwSlots = 1152;
+constexpr int kSellW = 8;
+constexpr int kXwRec = 2 * kXwWin + 4;
+
 // Schedule entry of one SpMV row tile; the entry after the last tile holds
 // {r_end, 0, rowptr[r_end], 0}.
 struct TileEntry {

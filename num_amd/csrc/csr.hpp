@@ -54,6 +54,15 @@ struct num_csr {
     // new schedule can re-code into it.
     const uint16_t* d16 = nullptr;
     uint16_t* d16_buf = nullptr;
+    // Windowed sliced layout (num_csr_set_windows), owned, built over
+    // A->sched when every tile fits: per-tile records (kXwRec int32: window
+    // first columns, cumulative slot ends, slice base, width), the FP64
+    // values and window slots in sliced order, one length byte per row.
+    int32_t* xw = nullptr;
+    double* sell_vals = nullptr;
+    uint16_t* sell_codes = nullptr;
+    uint8_t* sell_lens = nullptr;
+    bool xw_enabled = true;
     bool owns = false;                  // rowptr/colind/vals allocated here
     std::vector<int32_t> h_rowptr;      // host copy, drives scheduling
     int32_t max_row_nnz = 0;
@@ -93,6 +102,8 @@ int build_schedule(const num_csr* A, int32_t r0, int32_t r1, int32_t rows_per_bl
 // mean row length, times the block size.
 int32_t default_nnz_per_block(const num_csr* A);
 void free_schedule(Schedule& s);
+// Frees the windowed sliced layout of A (if any).
+void free_windows(num_csr* A);
 // XCD chunk (in units of rows_per_unit rows) that deals the rows offset_d
 // apart to the same XCD: offset_d / 8 rows, so rows offset_d apart sit
 // exactly 8 chunks apart and the x entries one chunk gathers across the

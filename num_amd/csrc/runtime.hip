@@ -368,6 +368,198 @@ static int build_d16(num_csr* A, hipStream_t st)
     return NUM_OK;
 }
 
+// Windows of one tile (rows [r0, r1)): the columns its entries read plus
+// its own rows as columns (so the CG kernel finds p_new of its own rows
+// there), as ranges sorted by first column and merged where they overlap or
+// lie at most kXwGap columns apart (the columns in between lie inside
+// [0, ncols) too).  The ranges come from the exact column range of each
+// distinct column offset (column - row) -- a handful for a stencil -- or,
+// for a tile with more than kMaxOff offsets, from its sorted columns.
+// Fills rec (lo[kXwWin], end[kXwWin]) and returns false when the tile needs
+// more than kXwWin windows or kXwSlots slots.
+static bool tile_windows(const int32_t* rp, const int32_t* col, int32_t r0, int32_t r1,
+                         int32_t ncols, int32_t* rec)
+{
+    constexpr int kMaxOff = 32;
+    constexpr int kXwGap = 16;
+    std::vector<std::pair<int32_t, int32_t>> rng;  // [lo, hi) column ranges
+    if (r0 < std::min(r1, ncols)) rng.push_back({r0, std::min(r1, ncols)});  // own rows
+    int32_t off[kMaxOff], lo[kMaxOff], hi[kMaxOff];
+    int nd = 0, k = 0;
+    bool many = false;
+    for (int32_t r = r0; r < r1 && !many; ++r) {
+        for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+            const int32_t c = col[j], d = c - r;
+            // consecutive rows repeat one offset sequence: try the next first
+            k = (nd > 0 && k + 1 < nd) ? k + 1 : 0;
+            if (nd == 0 || off[k] != d) {
+                k = 0;
+                while (k < nd && off[k] != d) ++k;
+                if (k == nd) {
+                    if (nd == kMaxOff) {
+                        many = true;
+                        break;
+                    }
+                    off[nd] = d;
+                    lo[nd] = c;
+                    hi[nd] = c + 1;
+                    ++nd;
+                    continue;
+                }
+            }
+            lo[k] = std::min(lo[k], c);
+            hi[k] = std::max(hi[k], c + 1);
+        }
+    }
+    if (!many) {
+        for (int a = 0; a < nd; ++a) rng.push_back({lo[a], hi[a]});
+    } else {
+        std::vector<int32_t> cs(col + rp[r0], col + rp[r1]);
+        std::sort(cs.begin(), cs.end());
+        const size_t first = rng.size();  // column ranges follow the own rows
+        for (const int32_t c : cs) {
+            if (rng.size() > first && c <= rng.back().second + kXwGap)
+                rng.back().second = std::max(rng.back().second, c + 1);
+            else
+                rng.push_back({c, c + 1});
+        }
+    }
+    std::sort(rng.begin(), rng.end());
+    int nw = 0, total = 0;
+    int32_t wl[kXwWin], wh[kXwWin];
+    for (const auto& q : rng) {
+        if (nw > 0 && q.first <= wh[nw - 1] + kXwGap) {
+            wh[nw - 1] = std::max(wh[nw - 1], q.second);
+            continue;
+        }
+        if (nw == kXwWin) return false;
+        wl[nw] = q.first;
+        wh[nw] = q.second;
+        ++nw;
+    }
+    for (int w = 0; w < kXwWin; ++w) {
+        if (w < nw) total += wh[w] - wl[w];
+        rec[w] = w < nw ? wl[w] : 0;
+        rec[kXwWin + w] = total;
+    }
+    return total <= kXwSlots;
+}
+
+// Scatter of the FP64 values into the sliced layout: tile b's entry k of its
+// row i at base_b + k*256 + i, zeros in the padding (rows past the tile's
+// last and entries past a row's length, up to the tile's width).  One
+// workgroup per tile, one thread per row slot.
+__global__ void sell_scatter_kernel(const int32_t* __restrict__ rowptr,
+                                    const double* __restrict__ vals,
+                                    const TileEntry* __restrict__ tiles,
+                                    const int32_t* __restrict__ rec, double* __restrict__ svals)
+{
+    const int b = blockIdx.x, i = threadIdx.x;
+    const int r0 = tiles[b].row, r1 = tiles[b + 1].row;
+    const int64_t base = (int64_t)rec[(size_t)b * kXwRec + 2 * kXwWin] * kSpmvBS;
+    const int width = rec[(size_t)b * kXwRec + 2 * kXwWin + 1];
+    const int js = r0 + i < r1 ? rowptr[r0 + i] : 0;
+    const int len = r0 + i < r1 ? rowptr[r0 + i + 1] - js : 0;
+    for (int k = 0; k < width; ++k)
+        svals[base + (int64_t)k * kSpmvBS + i] = k < len ? vals[js + k] : 0.0;
+}
+
+void free_windows(num_csr* A)
+{
+    if (A->xw) (void)hipFree(A->xw);
+    if (A->sell_vals) (void)hipFree(A->sell_vals);
+    if (A->sell_codes) (void)hipFree(A->sell_codes);
+    if (A->sell_lens) (void)hipFree(A->sell_lens);
+    A->xw = nullptr;
+    A->sell_vals = nullptr;
+    A->sell_codes = nullptr;
+    A->sell_lens = nullptr;
+}
+
+// (Re)builds the windowed layout of A->sched (spmv_sell_run): every tile's
+// windows (tile_windows), its width (longest row, <= kSellW) and its base
+// in the sliced arrays, which hold tile b's entry k of row i at
+// (base_b + k) * 256 + i: the FP64 values (scattered on the device), each
+// entry's LDS slot in the tile's windows (uint16) and one length byte per
+// row.  Nothing is built -- the kernels keep the delta-coded or int32
+// index -- when windows are switched off, the schedule has long or wide
+// rows, or any tile does not fit.  Synchronises `st`.
+static int build_xw(num_csr* A, hipStream_t st)
+{
+    free_windows(A);
+    const Schedule& s = A->sched;
+    if (!A->xw_enabled || A->nnz == 0 || s.nblocks == 0 || s.long_rows != 0 ||
+        s.rows_per_block > kSpmvBS || A->max_row_nnz > kSellW || s.row_begin != 0 ||
+        s.row_end != A->nrows)
+        return NUM_OK;
+    const int64_t nnz = A->nnz;
+    std::vector<int32_t> col(nnz);
+    std::vector<TileEntry> tiles(s.nblocks + 1);
+    NUM_HIP(hipMemcpyAsync(col.data(), A->colind, nnz * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           st));
+    NUM_HIP(hipMemcpyAsync(tiles.data(), s.d_tiles, tiles.size() * sizeof(TileEntry),
+                           hipMemcpyDeviceToHost, st));
+    NUM_HIP(hipStreamSynchronize(st));
+    const int32_t* rp = A->h_rowptr.data();
+    std::vector<int32_t> rec((size_t)s.nblocks * kXwRec, 0);
+    int64_t slices = 0;  // 256-entry slices before the current tile
+    for (int b = 0; b < s.nblocks; ++b) {
+        int32_t* R = &rec[(size_t)b * kXwRec];
+        const int32_t r0 = tiles[b].row, r1 = tiles[b + 1].row;
+        if (!tile_windows(rp, col.data(), r0, r1, A->ncols, R)) return NUM_OK;
+        int32_t width = 0;
+        for (int32_t r = r0; r < r1; ++r) width = std::max(width, rp[r + 1] - rp[r]);
+        if (slices + width > INT32_MAX / kSpmvBS) return NUM_OK;
+        R[2 * kXwWin] = (int32_t)slices;
+        R[2 * kXwWin + 1] = width;
+        slices += width;
+    }
+    // each entry's slot in its tile's windows, in the sliced order
+    const size_t nslot = (size_t)slices * kSpmvBS;
+    std::vector<uint16_t> slot(nslot, 0);
+    std::vector<uint8_t> lens(A->nrows);
+    for (int b = 0; b < s.nblocks; ++b) {
+        const int32_t* R = &rec[(size_t)b * kXwRec];
+        const size_t base = (size_t)R[2 * kXwWin] * kSpmvBS;
+        for (int32_t r = tiles[b].row; r < tiles[b + 1].row; ++r) {
+            const int32_t i = r - tiles[b].row;
+            lens[r] = (uint8_t)(rp[r + 1] - rp[r]);
+            for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+                const int32_t c = col[j];
+                int w = 0;
+                while (w + 1 < kXwWin && R[kXwWin + w] < R[kXwWin + w + 1] && c >= R[w + 1])
+                    ++w;
+                const int32_t start = w == 0 ? 0 : R[kXwWin + w - 1];
+                slot[base + (size_t)(j - rp[r]) * kSpmvBS + i] = (uint16_t)(start + c - R[w]);
+            }
+        }
+    }
+    hipError_t e = hipMalloc(&A->xw, rec.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&A->sell_vals, std::max<size_t>(nslot, 1) * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMalloc(&A->sell_codes, std::max<size_t>(nslot, 1) * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(&A->sell_lens, std::max<size_t>(A->nrows, 1));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(A->xw, rec.data(), rec.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(A->sell_codes, slot.data(), nslot * sizeof(uint16_t),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(A->sell_lens, lens.data(), lens.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        sell_scatter_kernel<<<s.nblocks, kSpmvBS, 0, st>>>(
+            A->rowptr, A->vals, static_cast<const TileEntry*>(s.d_tiles), A->xw, A->sell_vals);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        free_windows(A);
+        return fail(NUM_EHIP, "windowed layout build failed: %s", hipGetErrorString(e));
+    }
+    return NUM_OK;
+}
+
 }  // namespace numamd
 
 using namespace numamd;
@@ -531,7 +723,25 @@ int num_csr_attach_d16(num_csr* A, uint16_t* d_codes, void* stream, int32_t* att
         return rc;
     }
     if (!A->d16) A->d16_buf = nullptr;
+    rc = build_xw(A, (hipStream_t)stream);
+    if (rc) return rc;
     *attached = A->d16 ? 1 : 0;
+    return NUM_OK;
+}
+
+int num_csr_set_windows(num_csr* A, int32_t enable, void* stream)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    int rc = check_device(A);
+    if (rc) return rc;
+    A->xw_enabled = enable != 0;
+    return build_xw(A, (hipStream_t)stream);
+}
+
+int num_csr_window_info(const num_csr* A, int32_t* windowed)
+{
+    if (!A || !windowed) return fail(NUM_EINVAL, "matrix or windowed is NULL");
+    *windowed = A->xw ? 1 : 0;
     return NUM_OK;
 }
 
@@ -620,6 +830,7 @@ int num_csr_destroy(num_csr* A)
         return fail(NUM_ERUNTIME, "matrix is held by %d engine system(s); remove them first",
                     (int)A->engine_holds.load());
     if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
+    free_windows(A);
     free_schedule(A->xcd_sched);
     free_schedule(A->sched);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
@@ -657,6 +868,8 @@ int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_blo
     rc = build_d16(A, nullptr);  // the codes depend on the tiles
     if (rc) return rc;
     if (!A->d16) A->d16_buf = nullptr;
+    rc = build_xw(A, nullptr);  // and so do the windows
+    if (rc) return rc;
     std::lock_guard<std::mutex> lk(A->mu);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
     A->range_sched.clear();

@@ -39,6 +39,16 @@ namespace numamd {
 // overlap the stage latency), resolve() after the stage barrier, before any
 // gather; when it returns false the workgroup walks no row.  Both are const
 // and keep what they compute in mutable members.
+// Optional epilogue entry of the windowed walk (spmv_xw_run): own(row, v,
+// ca, cb, xr) receives the accessor's value at the row's own column, read
+// from the tile's windows instead of from memory.
+template <class E, class = void>
+struct has_own : std::false_type {};
+template <class E>
+struct has_own<E, std::void_t<decltype(std::declval<const E&>().own(
+                      0, 0.0, std::declval<double&>(), std::declval<double&>(), 0.0))>>
+    : std::true_type {};
+
 template <class G, class = void>
 struct has_hook : std::false_type {};
 template <class G>
@@ -474,6 +484,97 @@ __device__ __forceinline__ void spmv_d16_run(
     epi(r0 + t, acc, ca, cb, x);
 }
 
+// Windowed sliced walk (the layout of num_csr_set_windows).  The tile's
+// entries come in sliced order -- entry k of the tile's row i at
+// (base + k) * BS + i -- so thread i loads its own row's values and window
+// slots straight into registers, every load instruction reading BS
+// consecutive entries: coalesced without staging the tile in LDS.  In the
+// same batch every thread fetches its share of the tile's column windows
+// through the x accessor (x[c], or the CG's r[c] + beta*p_old[c]) and parks
+// them in LDS, the only LDS the walk uses; after one barrier each row is
+// summed from its registers and the LDS windows.  One memory round trip per
+// tile, a stencil's neighbour values fetched once per tile instead of once
+// per entry.  Entries are summed in index order from 0.0 with separately
+// rounded products: the bits of spmv_tile_run and spmv_d16_run.
+template <int BS, bool NT, class Epi, class Gather>
+__device__ __forceinline__ void spmv_sell_run(
+    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rec,
+    const uint8_t* __restrict__ lens, const uint16_t* __restrict__ code,
+    const double* __restrict__ vals, const Gather& x, const Epi& epi, double* __restrict__ xw,
+    double& ca, double& cb)
+{
+    constexpr int NXW = (kXwSlots + BS - 1) / BS;
+    static_assert(!has_hook<Gather>::value, "window fill has no hook point");
+    const int t = threadIdx.x;
+    const int r0 = e0.row, nr = e1.row - r0;
+    ca = 0.0;
+    cb = 0.0;
+    const bool active = t < nr;
+    // the record is uniform over the workgroup (scalar loads)
+    int wlo[kXwWin], wend[kXwWin];
+#pragma unroll
+    for (int q = 0; q < kXwWin; ++q) {
+        wlo[q] = rec[q];
+        wend[q] = rec[kXwWin + q];
+    }
+    const int64_t base = (int64_t)rec[2 * kXwWin] * BS;
+    const int width = rec[2 * kXwWin + 1];
+    const int total = wend[kXwWin - 1];
+    const int len = active ? (int)lens[r0 + t] : 0;
+    // this row's entries: every slot of the tile's width exists (padding)
+    double v[kSellW];
+    int c[kSellW];
+#pragma unroll
+    for (int k = 0; k < kSellW; ++k) {
+        if (k < width) {
+            v[k] = ld_mat<NT>(vals + base + k * BS + t);
+            c[k] = ld_mat<NT>(code + base + k * BS + t);
+        }
+    }
+    // window slots: slot s of window q reads column lo[q] + s - end[q-1]
+    double xv_w[NXW];
+#pragma unroll
+    for (int u = 0; u < NXW; ++u) {
+        const int s = t + u * BS;
+        if (s < total) {
+            int lo = wlo[0], start = 0;
+#pragma unroll
+            for (int q = 1; q < kXwWin; ++q)
+                if (s >= wend[q - 1]) {
+                    lo = wlo[q];
+                    start = wend[q - 1];
+                }
+            xv_w[u] = x(lo + s - start);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NXW; ++u) {
+        const int s = t + u * BS;
+        if (s < total) xw[s] = xv_w[u];
+    }
+    __syncthreads();
+    if (!active) return;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSellW; ++k)
+        if (k < len) acc += v[k] * xw[c[k]];
+    const int row = r0 + t;
+    if constexpr (has_own<Epi>::value) {
+        // the own rows are a window range of every tile of a square matrix
+        int os = -1;
+#pragma unroll
+        for (int q = 0; q < kXwWin; ++q) {
+            const int start = q == 0 ? 0 : wend[q - 1];
+            if (row >= wlo[q] && row - wlo[q] < wend[q] - start) os = start + row - wlo[q];
+        }
+        if (os >= 0) {
+            epi.own(row, acc, ca, cb, xw[os]);
+            return;
+        }
+    }
+    epi(row, acc, ca, cb, x);
+}
+
 template <typename T, int BS, int NPT, bool NT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_tile_body(
     int tile, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
@@ -531,6 +632,35 @@ __global__ __launch_bounds__(BS) void spmv_d16_kernel(
     double ca, cb;
     spmv_d16_run<BS, NPT, NT, Epi, Gather>(sched[tile], sched[tile + 1], lens, code, vals, x, epi,
                                            lv, lc, ca, cb);
+    if constexpr (Epi::kRed == 1) {
+        const double s = block_sum<BS>(ca, red);
+        if (threadIdx.x == 0) epi.partial_a[tile] = s;
+    } else if constexpr (Epi::kRed == 2) {
+        block_sum2<BS>(ca, cb, red);
+        if (threadIdx.x == 0) {
+            epi.partial_a[tile] = ca;
+            epi.partial_b[tile] = cb;
+        }
+    }
+}
+
+// The windowed sliced walk (spmv_sell_run): LDS for the kXwSlots window
+// slots only.
+template <int BS, bool NT, class Epi, class Gather>
+__global__ __launch_bounds__(BS) void spmv_sell_kernel(
+    const int32_t* __restrict__ rec, const uint8_t* __restrict__ lens,
+    const uint16_t* __restrict__ code, const double* __restrict__ vals, Gather x,
+    const TileEntry* __restrict__ sched, int ntiles, int chunk, Epi epi)
+{
+    __shared__ double xw[kXwSlots];
+    __shared__ double red[2 * BS / kWave];
+    const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
+    if (tile < 0) return;
+    if (epi.skip()) return;
+    x.prepare();
+    double ca, cb;
+    spmv_sell_run<BS, NT, Epi, Gather>(sched[tile], sched[tile + 1], rec + tile * kXwRec, lens,
+                                       code, vals, x, epi, xw, ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
         if (threadIdx.x == 0) epi.partial_a[tile] = s;

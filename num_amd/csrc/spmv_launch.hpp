@@ -53,11 +53,32 @@ inline bool stream_nt(const Schedule& s)
     return sizeof(T) == 8 ? s.nt64 != 0 : s.nt32 != 0;
 }
 
+// The windowed sliced layout, when built, replaces every index form for the
+// FP64 values on the whole-matrix schedule.
+template <typename T, bool NT, class Epi, class Gather>
+inline bool launch_sell(const num_csr* A, const Schedule& s, const T* vals, const Gather& x,
+                        const Epi& epi, hipStream_t stream)
+{
+    if constexpr (sizeof(T) == 8 && !has_hook<Gather>::value) {
+        if (A->xw && &s == &A->sched && vals == A->vals) {
+            spmv_sell_kernel<kSpmvBS, NT, Epi, Gather>
+                <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
+                    A->xw, A->sell_lens, A->sell_codes, A->sell_vals, x,
+                    static_cast<const TileEntry*>(s.d_tiles), s.nblocks, s.chunk, epi);
+            return true;
+        }
+    }
+    return false;
+}
+
 template <typename T, class Epi, class Gather>
 inline void launch_tiles(const num_csr* A, const Schedule& s, const T* vals, const Gather& x,
                          const Epi& epi, hipStream_t stream)
 {
     if (s.nblocks == 0) return;
+    if (stream_nt<T>(s) ? launch_sell<T, true>(A, s, vals, x, epi, stream)
+                        : launch_sell<T, false>(A, s, vals, x, epi, stream))
+        return;
     if (stream_nt<T>(s)) launch_tiles_nt<T, true>(A, s, vals, x, epi, stream);
     else launch_tiles_nt<T, false>(A, s, vals, x, epi, stream);
 }

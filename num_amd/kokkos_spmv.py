@@ -174,11 +174,16 @@ class CrsMatrix:
 
     @property
     def index_format(self) -> str:
-        """'delta16' when the FP64 SpMV / CG kernels read the delta-coded
-        column ids: they are attached (every row <= 32 entries in ascending
-        column order, codes within 16 bits) and the schedule streams the
-        FP64 matrix from HBM (nt64; a matrix that stays in the Infinity
-        Cache reads its int32 columns faster).  Else 'int32'."""
+        """What the FP64 SpMV / CG kernels read for the columns.
+        'window16': the windowed sliced layout (set_windows; every tile reads
+        x from a few column ranges fetched into LDS, each entry a 16-bit slot
+        in them; rows <= 8 entries).  'delta16': the delta-coded column ids
+        (every row <= 32 entries in ascending column order, codes within 16
+        bits) when the schedule streams the FP64 matrix from HBM (nt64; a
+        matrix that stays in the Infinity Cache reads its int32 columns
+        faster).  Else 'int32'."""
+        if self.windowed:
+            return "window16"
         if self._d16 is None or not self.schedule_info()["nt64"]:
             return "int32"
         return "delta16"
@@ -223,6 +228,25 @@ class CrsMatrix:
             call("num_csr_index_info", self._h, C.byref(on))
             if not on.value:
                 self._d16 = None
+
+    def set_windows(self, enable: bool = True) -> None:
+        """Switch the column-window index on or off (num_csr_set_windows):
+        a banded or stencil matrix whose tiles each read x from a few
+        contiguous column ranges fetches them once per tile into LDS.  Same
+        results either way (bit-identical)."""
+        call("num_csr_set_windows", self._h, int(bool(enable)), _stream())
+        if self._d16 is not None:
+            on = C.c_int32()
+            call("num_csr_index_info", self._h, C.byref(on))
+            if not on.value:
+                self._d16 = None
+
+    @property
+    def windowed(self) -> bool:
+        """True when the FP64 kernels read x through column windows."""
+        on = C.c_int32()
+        call("num_csr_window_info", self._h, C.byref(on))
+        return bool(on.value)
 
     def schedule_info(self) -> dict:
         v = [C.c_int32() for _ in range(8)]

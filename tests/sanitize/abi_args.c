@@ -80,6 +80,8 @@ int main(void)
     expect(num_csr_destroy(NULL), NUM_OK, "destroy: NULL");
     expect(num_csr_set_schedule(NULL, 0, 0, -1, -1, -1), NUM_EINVAL, "set_schedule: NULL");
     expect(num_csr_attach_f32(NULL, NULL), NUM_EINVAL, "attach_f32: NULL");
+    expect(num_csr_set_windows(NULL, 1, NULL), NUM_EINVAL, "set_windows: NULL");
+    expect(num_csr_window_info(NULL, NULL), NUM_EINVAL, "window_info: NULL");
     expect(num_spmv(NULL, x, y, NULL), NUM_EINVAL, "spmv: NULL matrix");
     expect(num_batch_spmv(NULL, X, Y, 2, NULL), NUM_EINVAL, "batch_spmv: NULL matrix");
     expect64(num_cg_workspace_bytes(NULL), -1, "cg workspace: NULL");

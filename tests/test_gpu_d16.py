@@ -22,7 +22,10 @@ def _dev(a):
 def _up(ks, rp, ci, va, n, m, index="auto"):
     # The codes are read only when the schedule streams the matrix (nt64);
     # these test matrices would stay in the Infinity Cache, so stream them.
+    # The windowed layout, which would replace the codes for these stencils,
+    # is switched off (tests/test_gpu_windows.py covers it).
     dA = ks.upload_csr(rp, ci, va, n, m, index=index)
+    dA.set_windows(False)
     dA.set_schedule(nt64=1)
     return dA
 
@@ -163,6 +166,7 @@ def test_d16_full_tiles_every_offset(ks, orc, w, npb):
     for lead in range(8):
         rp, ci, va = _banded(n, w, lead, seed=lead)
         dA = ks.upload_csr(rp, ci, va, n, n)
+        dA.set_windows(False)
         dA.set_schedule(rows_per_block=0, nnz_per_block=npb, nt64=1)
         assert dA.index_format == "delta16"
         assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp, ci, va, x)), lead

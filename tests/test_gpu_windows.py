@@ -1,0 +1,164 @@
+"""GPU parity of the windowed sliced layout (num_csr_set_windows): where every
+tile of the FP64 schedule reads x from at most 4 contiguous column ranges of
+at most 1152 columns in all and every row holds <= 8 entries, the FP64 SpMV
+and CG walk a sliced copy of the tile (entry k of row i at slice k, lane i)
+and read x from those ranges, fetched once per tile into LDS.  The entries
+and their summation order are unchanged, so every result must be
+bit-identical to the oracle's row loop (src/spmv_kernel.hpp:90-97) and to
+the delta-coded and int32-column kernels; matrices outside the layout must
+keep the other index forms.
+"""
+import numpy as np
+import pytest
+import torch
+
+from num_amd import generators as g
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _spmv(ks, dA, x):
+    y = torch.full((dA.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    ks.spmv(dA, _dev(x), y)
+    return y.cpu().numpy()
+
+
+def _with_empty_rows(A, every=7):
+    lens = np.diff(A.rowptr).copy()
+    keep = np.ones(A.nnz, bool)
+    for r in range(0, A.nrows, every):
+        keep[A.rowptr[r]:A.rowptr[r + 1]] = False
+    lens[::every] = 0
+    rp = np.zeros(A.nrows + 1, np.int32)
+    np.cumsum(lens, out=rp[1:])
+    return rp, A.colind[keep].astype(np.int32), A.values[keep]
+
+
+def _banded(n, m, w, seed):
+    """n x m, row r holds w consecutive columns from min(r, m - w)."""
+    rp = np.arange(0, w * (n + 1), w, dtype=np.int32)
+    ci = np.concatenate([min(r, m - w) + np.arange(w) for r in range(n)]).astype(np.int32)
+    va = np.random.default_rng(seed).uniform(-1, 1, ci.size)
+    return rp, ci, va
+
+
+CASES = {
+    "lap3d_20": lambda: g.laplacian_3d(20),
+    "lap3d_33": lambda: g.laplacian_3d(33),     # last tile partial, mixed widths
+    "lap2d_70": lambda: g.laplacian_2d(70),
+    "diffusion_16": lambda: g.diffusion_3d(16, sigma=1.0, seed=3),
+    "stencil_96x96x5": lambda: g.stencil((96, 96, 5), 0.25),  # plane 9216: 3 windows
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("nt64", [1, 0])
+def test_spmv_windows_bit_identical(ks, orc, name, nt64):
+    A = CASES[name]()
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    dA.set_schedule(nt64=nt64)
+    assert dA.index_format == "window16"
+    x = np.random.default_rng(7).uniform(-1, 1, A.ncols)
+    y = _spmv(ks, dA, x)
+    assert np.array_equal(y, orc.spmv(A.rowptr, A.colind, A.values, x))
+    dA.set_windows(False)
+    assert not dA.windowed
+    assert np.array_equal(_spmv(ks, dA, x), y)
+    dA.set_windows(True)
+    assert dA.windowed
+    assert np.array_equal(_spmv(ks, dA, x), y)
+
+
+def test_windows_empty_rows_and_specials(ks, orc):
+    A = g.laplacian_3d(14)
+    rp, ci, va = _with_empty_rows(A)
+    dA = ks.upload_csr(rp, ci, va, A.nrows, A.ncols)
+    assert dA.windowed
+    x = np.random.default_rng(8).standard_normal(A.ncols)
+    x[5] = np.inf
+    x[300] = np.nan
+    y = _spmv(ks, dA, x)
+    ref = orc.spmv(rp, ci, va, x)
+    assert np.array_equal(np.isnan(y), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    assert np.array_equal(y[ok], ref[ok])
+    assert np.all(y[::7] == 0.0)
+
+
+@pytest.mark.parametrize("n,m", [(3000, 2000), (2000, 3000)])
+def test_windows_rectangular(ks, orc, n, m):
+    rp, ci, va = _banded(n, m, 6, seed=n)
+    dA = ks.upload_csr(rp, ci, va, n, m)
+    assert dA.windowed
+    x = np.random.default_rng(9).uniform(-1, 1, m)
+    assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp, ci, va, x))
+
+
+def test_windows_refused_outside_the_layout(ks, orc):
+    x_of = lambda m: np.random.default_rng(10).uniform(-1, 1, m)
+    # scattered columns: more than 4 ranges per tile
+    L, _ = g.lcg_matrix(5000, 5, seed=3)
+    dL = ks.upload_csr(L.rowptr, L.colind, L.values, L.nrows, L.ncols)
+    assert not dL.windowed
+    assert np.array_equal(_spmv(ks, dL, x_of(L.ncols)),
+                          orc.spmv(L.rowptr, L.colind, L.values, x_of(L.ncols)))
+    # rows of 9 entries: wider than the 8 slices
+    rp, ci, va = _banded(2000, 2000, 9, seed=1)
+    dB = ks.upload_csr(rp, ci, va, 2000, 2000)
+    assert not dB.windowed and dB.index_format in ("delta16", "int32")
+    assert np.array_equal(_spmv(ks, dB, x_of(2000)), orc.spmv(rp, ci, va, x_of(2000)))
+    # 3-D stencils: rows 600 apart make 5 ranges per tile (over 4); rows 250
+    # apart merge into a middle range of 256 + 2*250, which with the two
+    # plane ranges needs 1268 > 1152 slots
+    for dims in ((600, 4, 3), (250, 4, 3)):
+        S = g.stencil(dims, 0.25)
+        dS = ks.upload_csr(S.rowptr, S.colind, S.values, S.nrows, S.ncols)
+        assert not dS.windowed, dims
+        assert np.array_equal(_spmv(ks, dS, x_of(S.ncols)),
+                              orc.spmv(S.rowptr, S.colind, S.values, x_of(S.ncols)))
+    # switched off on request
+    A = g.laplacian_3d(12)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    dA.set_windows(False)
+    assert not dA.windowed
+
+
+def test_windows_survive_reschedule(ks, orc):
+    A = g.laplacian_3d(24)
+    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    x = np.random.default_rng(11).uniform(-1, 1, A.ncols)
+    ref = orc.spmv(A.rowptr, A.colind, A.values, x)
+    for rpb, npb, nt in ((64, 0, 1), (200, 900, 0), (0, 0, -1)):
+        dA.set_schedule(rows_per_block=rpb, nnz_per_block=npb, nt64=nt)
+        assert dA.windowed, (rpb, npb, nt)
+        assert np.array_equal(_spmv(ks, dA, x), ref)
+    dA.set_windows(False)
+    dA.set_schedule(rows_per_block=128)
+    assert not dA.windowed  # the switch outlives the new tiles
+    assert np.array_equal(_spmv(ks, dA, x), ref)
+
+
+@pytest.mark.parametrize("name", ["lap3d_20", "diffusion_16", "lap3d_33"])
+@pytest.mark.parametrize("nt64", [1, 0])
+def test_cg_windows_bit_identical(ks, name, nt64):
+    # the windowed CG kernel reads p_new of its own rows from the windows:
+    # same bits as the delta-coded / int32 kernels, which compute it again
+    A = CASES[name]()
+    b = _dev(np.random.default_rng(12).uniform(-1, 1, A.nrows))
+    out = []
+    for on in (True, False):
+        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        dA.set_windows(on)
+        dA.set_schedule(nt64=nt64)
+        assert dA.windowed == on
+        x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+        res = ks.cg_solve_local(dA, b, x, tol=1e-10, max_iter=2000)
+        out.append((res, x.cpu().numpy()))
+    (r0, x0), (r1, x1) = out
+    assert r0["converged"] and r0["iters"] == r1["iters"]
+    assert r0["residual"] == r1["residual"]
+    assert np.array_equal(x0, x1)
