@@ -77,7 +77,9 @@ int num_csr_create(const int32_t* h_rowptr, const int32_t* d_rowptr,
 int num_csr_create_host(const int32_t* h_rowptr, const int32_t* h_colind,
                         const double* h_vals, int32_t nrows, int32_t ncols,
                         num_csr** out);
-/* Attach an FP32 copy of the values (same pattern) for the FP32 inner solve. */
+/* Attach an FP32 copy of the values (same pattern) for the FP32 inner solve.
+ * When the windowed layout is built (num_csr_set_windows) this also builds
+ * its FP32 sliced values (synchronises the default stream). */
 int num_csr_attach_f32(num_csr* A, const float* d_vals32);
 /* Build column - row offsets as int16 into d_rel16 [nnz] for the FP32 inner
  * solve (no reference counterpart): attached (*attached = 1) only for a

@@ -581,15 +581,22 @@ __device__ __forceinline__ void mix_after_ka(bool ru, double sa, double sb, MixS
 // Fold the r32.r32 of an update into the state: beta, or schedule a
 // refinement.  mix_fold_view gives what a workgroup needs (beta, psel, ru);
 // mix_after_kc applies the whole fold to the published record.
+// The p buffer of the next FP32 step: it flips when an update is pending.
+// It does not depend on the update's r.r, so a kernel can issue its loads of
+// p_old before it has summed the partials.
+__device__ __forceinline__ int mix_next_psel(const MixState& s)
+{
+    return (s.done || !s.pend) ? s.psel : s.psel ^ 1;
+}
+
 __device__ __forceinline__ void mix_fold_view(double rrn, const MixState& s, double& beta,
                                               int& psel, int& ru, int& fin)
 {
     beta = s.beta;
-    psel = s.psel;
+    psel = mix_next_psel(s);
     ru = s.ru;
     fin = s.final_;
     if (s.done || !s.pend) return;
-    psel ^= 1;
     const double est = sqrt(rrn) / s.bnorm;
     if (s.it + 1 >= s.max_iter) {
         ru = 1;
@@ -779,6 +786,123 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     }
     const double s = block_sum<kSpmvBS>(ca, red);
     if (threadIdx.x == 0) a.pa[p] = s;
+}
+
+// SpMV kernel of the mixed slot over the windowed sliced layout (the FP32
+// and FP64 values in sliced order, one window-slot code per entry shared by
+// both; see spmv_sell_run): one workgroup per schedule tile, partials one
+// per tile.  Every load is issued before the update kernel's partials are
+// summed: the FP32 values and slots of the thread's row, r32 and p_old over
+// the tile's windows (p_old's buffer does not depend on the sum) and the
+// partials; after the fold, p_new = r + beta*p_old goes into the windows in
+// LDS and each row is summed from registers and LDS (same values, same
+// order as the staged kernels: the same bits per row).  A refinement
+// (ru) walks the same slots with the FP64 values and x64 + x32 windows.
+template <bool NT32, bool NT64>
+__global__ __launch_bounds__(kSpmvBS, 8) void mix_sell_kernel(
+    const int32_t* __restrict__ recs, const uint8_t* __restrict__ lens,
+    const uint16_t* __restrict__ code, const float* __restrict__ v32,
+    const double* __restrict__ v64, const TileEntry* __restrict__ sched, int ntiles, int chunk,
+    MixArgs a)
+{
+    constexpr int BS = kSpmvBS;
+    constexpr int NXW = (kXwSlots + BS - 1) / BS;
+    __shared__ double xw64[kXwSlots];
+    __shared__ double red[2 * BS / kWave];
+    float* xw32 = reinterpret_cast<float*>(xw64);
+    const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
+    if (tile < 0) return;
+    const int t = threadIdx.x;
+    const int32_t* rec = recs + tile * kXwRec;
+    NUMAMD_XW_LOAD(rec);
+    const int64_t vb = (int64_t)rec[kXwBase] * BS;
+    const int width = rec[kXwWidth];
+    const int total = we3;
+    const int r0 = sched[tile].row, nr = sched[tile + 1].row - r0;
+    const bool active = t < nr;
+    const int len = active ? (int)lens[r0 + t] : 0;
+    // the row's FP32 values, packed in pairs (one 8-byte load each, then a
+    // single slice for an odd width), and its slots, packed; uniform
+    // branches on the tile's width
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const int n2 = width >> 1;
+    const f2v* __restrict__ vp = reinterpret_cast<const f2v*>(v32 + vb);
+    f2v v[kSellW / 2];
+#pragma unroll
+    for (int h = 0; h < kSellW / 2; ++h)
+        if (h < n2) v[h] = ld_mat<NT32>(vp + h * BS + t);
+    const float vl = (width & 1) ? ld_mat<NT32>(v32 + vb + 2 * n2 * BS + t) : 0.0f;
+    uint64_t q0, q1;
+    sell_slots<NT32, BS>(code + vb, width, t, q0, q1);
+    const float* __restrict__ p_old = mix_next_psel(*a.sa) ? a.p1 : a.p0;
+    const int sm = max(total - 1, 0);
+    float wr[NXW], wp[NXW];
+#pragma unroll
+    for (int u = 0; u < NXW; ++u) {
+        const int col = xw_col(min(t + u * BS, sm), wl0, wl1, wl2, wl3, we0, we1, we2);
+        wr[u] = a.r32[col];
+        wp[u] = p_old[col];
+    }
+    GatherMix g{a.r32, a.p0, a.p1, a.sa, a.sb, a.pb, a.npb};
+    g.issue();
+    g.resolve();  // a barrier; workgroup 0 publishes the folded state
+    if (g.done) return;
+    const int row = r0 + t;
+    if (g.ru) {
+        const int xs = g.xsel;
+        GatherXsum gx{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
+        EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, nullptr, nullptr};
+        double w64[kSellW];
+#pragma unroll
+        for (int k = 0; k < kSellW; ++k)
+            if (k < width) w64[k] = ld_mat<NT64>(v64 + vb + k * BS + t);
+#pragma unroll
+        for (int u = 0; u < NXW; ++u) {
+            const int s = t + u * BS;
+            if (s < total) xw64[s] = gx(xw_col(s, wl0, wl1, wl2, wl3, we0, we1, we2));
+        }
+        __syncthreads();
+        double ca = 0.0, cb = 0.0;
+        if (active) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < kSellW; ++k)
+                if (k < len) acc += w64[k] * xw64[sell_slot(k, q0, q1)];
+            epi(row, acc, ca, cb, gx);
+        }
+        block_sum2<BS>(ca, cb, red);
+        if (t == 0) {
+            a.pa[tile] = ca;
+            a.pc[tile] = cb;
+        }
+        return;
+    }
+    const float beta = g.beta;
+#pragma unroll
+    for (int u = 0; u < NXW; ++u) {
+        const int s = t + u * BS;
+        if (s < total) xw32[s] = wr[u] + beta * wp[u];
+    }
+    __syncthreads();
+    double ca = 0.0;
+    if (active) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kSellW; ++k)
+            if (k < len) {
+                const float vk = k < 2 * n2 ? ((k & 1) ? v[k / 2].y : v[k / 2].x) : vl;
+                acc += vk * xw32[sell_slot(k, q0, q1)];
+            }
+        // p_new of the own row: a window slot (the own rows are a window
+        // range of every tile of a square matrix)
+        const int os = xw_slot(row, wl0, wl1, wl2, wl3, we0, we1, we2, we3);
+        const float pn = os >= 0 ? xw32[os] : g(row);
+        __builtin_nontemporal_store(pn, (g.psel ? a.p0 : a.p1) + row);
+        __builtin_nontemporal_store(acc, a.q + row);
+        ca = (double)pn * (double)acc;
+    }
+    const double s = block_sum<BS>(ca, red);
+    if (t == 0) a.pa[tile] = s;
 }
 
 __global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol, double delta,
@@ -1007,8 +1131,36 @@ static bool mix_paired(const num_csr* A)
            (A->sched.npt == 5 || A->sched.npt == 7);
 }
 
+// The windowed sliced path applies when the layout carries the FP32 values
+// and the FP32 matrix streams from HBM (192^3 Laplacian: 88 ms against
+// 106 ms for the staged pair kernel).  An FP32 matrix that stays in the
+// Infinity Cache runs the pair kernel: two tiles per workgroup halve the
+// per-workgroup fold of the update kernel's partials, which a cache-resident
+// step pays relatively more for (128^3 diffusion: 91.8 ms against 97.9 ms).
+static bool mix_sliced(const num_csr* A)
+{
+    return A->xw && A->sell_vals32 && A->sched.nt32 != 0;
+}
+
 static void launch_mix_spmv(const num_csr* A, const MixArgs& a, hipStream_t st)
 {
+    if (mix_sliced(A)) {
+        const Schedule& s = A->sched;
+        const int grid = chunk_grid(s.nblocks, s.chunk);
+        const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
+        auto go = [&](auto k) {
+            k<<<grid, kSpmvBS, 0, st>>>(A->xw, A->sell_lens, A->sell_codes, A->sell_vals32,
+                                        A->sell_vals, tiles, s.nblocks, s.chunk, a);
+        };
+        if (s.nt32) {
+            if (s.nt64) go(mix_sell_kernel<true, true>);
+            else go(mix_sell_kernel<true, false>);
+        } else {
+            if (s.nt64) go(mix_sell_kernel<false, true>);
+            else go(mix_sell_kernel<false, false>);
+        }
+        return;
+    }
     if (mix_paired(A)) {
         if (A->sched.npt == 5) launch_mix_pair_npt<5>(A, a, st);
         else launch_mix_pair_npt<7>(A, a, st);
@@ -1148,7 +1300,8 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     if (rc) return rc;
     mix_init<<<1, 64, 0, st>>>(bb, a.sa, tol, delta, max_iter);
     NUM_LAUNCH_CHECK();
-    const int np_spmv = mix_paired(A) ? (A->sched.nblocks + 1) / 2 : A->sched.nblocks;
+    const int np_spmv =
+        !mix_sliced(A) && mix_paired(A) ? (A->sched.nblocks + 1) / 2 : A->sched.nblocks;
     auto slot = [&]() -> int {
         launch_mix_spmv(A, a, st);
         mix_kc<<<gk, kFin, 0, st>>>(n, np_spmv, a);

@@ -78,12 +78,37 @@ __host__ __device__ inline const uint8_t* d16_lens(const uint16_t* codes, int64_
 // most kXwWin contiguous column ranges holding every column the tile's
 // entries read, kXwSlots columns in all.  Per tile the record holds the
 // windows' first columns (lo[kXwWin]), their cumulative ends in LDS slots
-// (end[kXwWin]; an unused window has lo 0 and the previous end), the tile's
-// first 256-entry slice and its width (longest row, <= kSellW entries).
+// (end[kXwWin]; an unused window has lo 0 and the previous end), then the
+// tile's base in the sliced arrays (in 256-entry slices) and its width
+// (longest row, 1 to kSellW entries).  A tile of width w takes w slices of
+// each array, with no padding between tiles:
+//   FP64 values: entry k of row i at (base + k) * 256 + i;
+//   slots (uint16) and FP32 values: entry k of row i in packed groups,
+//     slots four per 8-byte load (then two per 4-byte, then one), FP32
+//     values two per 8-byte load (then one) -- sell_pos gives the position.
 constexpr int kXwWin = 4;
 constexpr int kXwSlots = 1152;
 constexpr int kSellW = 8;
-constexpr int kXwRec = 2 * kXwWin + 4;
+constexpr int kXwBase = 2 * kXwWin;
+constexpr int kXwWidth = 2 * kXwWin + 1;
+constexpr int kXwRec = 2 * kXwWin + 2;
+
+// Position of entry k of lane i (of bs lanes) in a tile's packed region of
+// w slices, elements of which `pack` (4 for slots, 2 for FP32 values) share
+// one load: full groups of `pack`, then groups of halves down to single
+// elements.  Positions count elements from the tile's region start.
+__host__ __device__ inline int64_t sell_pos(int k, int i, int w, int pack, int bs)
+{
+    int64_t base = 0;
+    for (int g = pack; g >= 1; g >>= 1) {
+        const int n = (w / g) * g;  // slots taken by groups of size g
+        if (k < n) return base + (int64_t)(k / g) * bs * g + (int64_t)i * g + (k % g);
+        base += (int64_t)n * bs;
+        k -= n;
+        w -= n;
+    }
+    return base;  // not reached for k < w
+}
 
 // Schedule entry of one SpMV row tile; the entry after the last tile holds
 // {r_end, 0, rowptr[r_end], 0}.

@@ -60,6 +60,7 @@ struct num_csr {
     // values and window slots in sliced order, one length byte per row.
     int32_t* xw = nullptr;
     double* sell_vals = nullptr;
+    float* sell_vals32 = nullptr;       // when vals32 is attached
     uint16_t* sell_codes = nullptr;
     uint8_t* sell_lens = nullptr;
     bool xw_enabled = true;

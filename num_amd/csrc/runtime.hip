@@ -445,33 +445,62 @@ static bool tile_windows(const int32_t* rp, const int32_t* col, int32_t r0, int3
     return total <= kXwSlots;
 }
 
-// Scatter of the FP64 values into the sliced layout: tile b's entry k of its
-// row i at base_b + k*256 + i, zeros in the padding (rows past the tile's
-// last and entries past a row's length, up to the tile's width).  One
-// workgroup per tile, one thread per row slot.
+// Scatter of the values into the sliced layout, one workgroup per tile and
+// one thread per row slot, zeros in the padding (rows past the tile's last,
+// entries past a row's length, up to the tile's width): FP64 values one
+// slice per entry index, FP32 values packed in pairs (sell_pos).
+template <typename T, int Pack>
 __global__ void sell_scatter_kernel(const int32_t* __restrict__ rowptr,
-                                    const double* __restrict__ vals,
+                                    const T* __restrict__ vals,
                                     const TileEntry* __restrict__ tiles,
-                                    const int32_t* __restrict__ rec, double* __restrict__ svals)
+                                    const int32_t* __restrict__ rec, T* __restrict__ svals)
 {
     const int b = blockIdx.x, i = threadIdx.x;
     const int r0 = tiles[b].row, r1 = tiles[b + 1].row;
-    const int64_t base = (int64_t)rec[(size_t)b * kXwRec + 2 * kXwWin] * kSpmvBS;
-    const int width = rec[(size_t)b * kXwRec + 2 * kXwWin + 1];
+    const int32_t* R = rec + (size_t)b * kXwRec;
+    T* __restrict__ out = svals + (int64_t)R[kXwBase] * kSpmvBS;
+    const int width = R[kXwWidth];
     const int js = r0 + i < r1 ? rowptr[r0 + i] : 0;
     const int len = r0 + i < r1 ? rowptr[r0 + i + 1] - js : 0;
     for (int k = 0; k < width; ++k)
-        svals[base + (int64_t)k * kSpmvBS + i] = k < len ? vals[js + k] : 0.0;
+        out[sell_pos(k, i, width, Pack, kSpmvBS)] = k < len ? vals[js + k] : T(0);
+}
+
+// The FP32 values of A in the sliced layout (A->xw built, vals32 attached).
+static int build_sell32(num_csr* A, hipStream_t st)
+{
+    if (A->sell_vals32) (void)hipFree(A->sell_vals32);
+    A->sell_vals32 = nullptr;
+    if (!A->xw || !A->vals32) return NUM_OK;
+    const Schedule& s = A->sched;
+    int32_t last[kXwRec];
+    NUM_HIP(hipMemcpyAsync(last, A->xw + (size_t)(s.nblocks - 1) * kXwRec, sizeof(last),
+                           hipMemcpyDeviceToHost, st));
+    NUM_HIP(hipStreamSynchronize(st));
+    const size_t nval = ((size_t)last[kXwBase] + last[kXwWidth]) * kSpmvBS;
+    NUM_HIP(hipMalloc(&A->sell_vals32, nval * sizeof(float)));
+    sell_scatter_kernel<float, 2><<<s.nblocks, kSpmvBS, 0, st>>>(
+        A->rowptr, A->vals32, static_cast<const TileEntry*>(s.d_tiles), A->xw, A->sell_vals32);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipFree(A->sell_vals32);
+        A->sell_vals32 = nullptr;
+        return fail(NUM_EHIP, "FP32 sliced values failed: %s", hipGetErrorString(e));
+    }
+    return NUM_OK;
 }
 
 void free_windows(num_csr* A)
 {
     if (A->xw) (void)hipFree(A->xw);
     if (A->sell_vals) (void)hipFree(A->sell_vals);
+    if (A->sell_vals32) (void)hipFree(A->sell_vals32);
     if (A->sell_codes) (void)hipFree(A->sell_codes);
     if (A->sell_lens) (void)hipFree(A->sell_lens);
     A->xw = nullptr;
     A->sell_vals = nullptr;
+    A->sell_vals32 = nullptr;
     A->sell_codes = nullptr;
     A->sell_lens = nullptr;
 }
@@ -507,48 +536,47 @@ static int build_xw(num_csr* A, hipStream_t st)
         int32_t* R = &rec[(size_t)b * kXwRec];
         const int32_t r0 = tiles[b].row, r1 = tiles[b + 1].row;
         if (!tile_windows(rp, col.data(), r0, r1, A->ncols, R)) return NUM_OK;
-        int32_t width = 0;
+        int32_t width = 1;  // at least one slice per tile
         for (int32_t r = r0; r < r1; ++r) width = std::max(width, rp[r + 1] - rp[r]);
         if (slices + width > INT32_MAX / kSpmvBS) return NUM_OK;
-        R[2 * kXwWin] = (int32_t)slices;
-        R[2 * kXwWin + 1] = width;
+        R[kXwBase] = (int32_t)slices;
+        R[kXwWidth] = width;
         slices += width;
     }
-    // each entry's slot in its tile's windows, in the sliced order
-    const size_t nslot = (size_t)slices * kSpmvBS;
-    std::vector<uint16_t> slot(nslot, 0);
+    // each entry's slot in its tile's windows, packed four per 8-byte load
+    const size_t nval = (size_t)slices * kSpmvBS;
+    std::vector<uint16_t> slot(nval, 0);
     std::vector<uint8_t> lens(A->nrows);
     for (int b = 0; b < s.nblocks; ++b) {
         const int32_t* R = &rec[(size_t)b * kXwRec];
-        const size_t base = (size_t)R[2 * kXwWin] * kSpmvBS;
+        uint16_t* out = slot.data() + (size_t)R[kXwBase] * kSpmvBS;
         for (int32_t r = tiles[b].row; r < tiles[b + 1].row; ++r) {
             const int32_t i = r - tiles[b].row;
             lens[r] = (uint8_t)(rp[r + 1] - rp[r]);
             for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
-                const int32_t c = col[j];
+                const int32_t c = col[j], k = j - rp[r];
                 int w = 0;
                 while (w + 1 < kXwWin && R[kXwWin + w] < R[kXwWin + w + 1] && c >= R[w + 1])
                     ++w;
                 const int32_t start = w == 0 ? 0 : R[kXwWin + w - 1];
-                slot[base + (size_t)(j - rp[r]) * kSpmvBS + i] = (uint16_t)(start + c - R[w]);
+                out[sell_pos(k, i, R[kXwWidth], 4, kSpmvBS)] = (uint16_t)(start + c - R[w]);
             }
         }
     }
     hipError_t e = hipMalloc(&A->xw, rec.size() * sizeof(int32_t));
-    if (e == hipSuccess) e = hipMalloc(&A->sell_vals, std::max<size_t>(nslot, 1) * sizeof(double));
-    if (e == hipSuccess)
-        e = hipMalloc(&A->sell_codes, std::max<size_t>(nslot, 1) * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(&A->sell_vals, nval * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&A->sell_codes, nval * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMalloc(&A->sell_lens, std::max<size_t>(A->nrows, 1));
     if (e == hipSuccess)
         e = hipMemcpyAsync(A->xw, rec.data(), rec.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(A->sell_codes, slot.data(), nslot * sizeof(uint16_t),
+        e = hipMemcpyAsync(A->sell_codes, slot.data(), nval * sizeof(uint16_t),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(A->sell_lens, lens.data(), lens.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        sell_scatter_kernel<<<s.nblocks, kSpmvBS, 0, st>>>(
+        sell_scatter_kernel<double, 1><<<s.nblocks, kSpmvBS, 0, st>>>(
             A->rowptr, A->vals, static_cast<const TileEntry*>(s.d_tiles), A->xw, A->sell_vals);
         e = hipGetLastError();
     }
@@ -557,7 +585,7 @@ static int build_xw(num_csr* A, hipStream_t st)
         free_windows(A);
         return fail(NUM_EHIP, "windowed layout build failed: %s", hipGetErrorString(e));
     }
-    return NUM_OK;
+    return build_sell32(A, st);
 }
 
 }  // namespace numamd
@@ -672,8 +700,10 @@ int num_csr_attach_f32(num_csr* A, const float* d_vals32)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
     if (!d_vals32 && A->nnz > 0) return fail(NUM_EINVAL, "d_vals32 is NULL");
+    int rc = check_device(A);
+    if (rc) return rc;
     A->vals32 = d_vals32;
-    return NUM_OK;
+    return build_sell32(A, nullptr);  // the sliced FP32 copy, if the layout is built
 }
 
 int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* attached)

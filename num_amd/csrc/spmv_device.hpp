@@ -484,11 +484,77 @@ __device__ __forceinline__ void spmv_d16_run(
     epi(r0 + t, acc, ca, cb, x);
 }
 
+// A tile's window record as scalars passed by value: the four windows'
+// first columns (wl0..wl3) and cumulative slot ends (we0..we3).  No array or
+// object whose address is taken: the compiler may move such a private
+// variable to LDS, and every address computation would then wait on an LDS
+// read.
+static_assert(kXwWin == 4, "the window helpers take four windows");
+#define NUMAMD_XW_LOAD(rec)                                                      \
+    const int wl0 = (rec)[0], wl1 = (rec)[1], wl2 = (rec)[2], wl3 = (rec)[3];    \
+    const int we0 = (rec)[4], we1 = (rec)[5], we2 = (rec)[6], we3 = (rec)[7]
+// column of slot s: slot s of window q reads column l_q + s - e_(q-1)
+__device__ __forceinline__ int xw_col(int s, int l0, int l1, int l2, int l3, int e0, int e1,
+                                      int e2)
+{
+    int lo = l0, start = 0;
+    if (s >= e0) {
+        lo = l1;
+        start = e0;
+    }
+    if (s >= e1) {
+        lo = l2;
+        start = e1;
+    }
+    if (s >= e2) {
+        lo = l3;
+        start = e2;
+    }
+    return lo + s - start;
+}
+// slot of column c, or -1 when no window holds it
+__device__ __forceinline__ int xw_slot(int c, int l0, int l1, int l2, int l3, int e0, int e1,
+                                       int e2, int e3)
+{
+    int s = -1;
+    if (c >= l0 && c - l0 < e0) s = c - l0;
+    if (c >= l1 && c - l1 < e1 - e0) s = e0 + c - l1;
+    if (c >= l2 && c - l2 < e2 - e1) s = e1 + c - l2;
+    if (c >= l3 && c - l3 < e3 - e2) s = e2 + c - l3;
+    return s;
+}
+
+// A row's window slots from its tile's packed slot region p (sell_pos with
+// four per group): slots 0-3 in q0, 4-7 in q1, 16 bits each.  At most three
+// loads (8 + 4 + 2 bytes per lane), uniform branches on the width.
+template <bool NT, int BS>
+__device__ __forceinline__ void sell_slots(const uint16_t* __restrict__ p, int w, int t,
+                                           uint64_t& q0, uint64_t& q1)
+{
+    const int n4 = w >> 2, r = w & 3;
+    uint64_t g0 = 0, g1 = 0, tl = 0;
+    if (n4 >= 1) g0 = ld_mat<NT>(reinterpret_cast<const uint64_t*>(p) + t);
+    if (n4 >= 2) g1 = ld_mat<NT>(reinterpret_cast<const uint64_t*>(p + 4 * BS) + t);
+    const uint16_t* __restrict__ pr = p + n4 * 4 * BS;
+    if (r >= 2) tl = ld_mat<NT>(reinterpret_cast<const uint32_t*>(pr) + t);
+    if (r & 1)
+        tl |= (uint64_t)ld_mat<NT>(pr + (r >= 2 ? 2 * BS : 0) + t) << (r >= 2 ? 32 : 0);
+    q0 = n4 >= 1 ? g0 : tl;
+    q1 = n4 >= 2 ? g1 : (n4 == 1 ? tl : 0);
+}
+
+// Slot k of a row from sell_slots' q0 / q1.
+__device__ __forceinline__ int sell_slot(int k, uint64_t q0, uint64_t q1)
+{
+    return (int)(((k < 4 ? q0 : q1) >> (16 * (k & 3))) & 0xffffu);
+}
+
 // Windowed sliced walk (the layout of num_csr_set_windows).  The tile's
 // entries come in sliced order -- entry k of the tile's row i at
-// (base + k) * BS + i -- so thread i loads its own row's values and window
-// slots straight into registers, every load instruction reading BS
-// consecutive entries: coalesced without staging the tile in LDS.  In the
+// (base + k) * BS + i, its window slot packed with the row's others
+// (sell_pos) -- so thread i loads its own row's values and slots straight
+// into registers, every load instruction reading BS consecutive elements:
+// coalesced without staging the tile in LDS.  In the
 // same batch every thread fetches its share of the tile's column windows
 // through the x accessor (x[c], or the CG's r[c] + beta*p_old[c]) and parks
 // them in LDS, the only LDS the walk uses; after one barrier each row is
@@ -511,41 +577,26 @@ __device__ __forceinline__ void spmv_sell_run(
     cb = 0.0;
     const bool active = t < nr;
     // the record is uniform over the workgroup (scalar loads)
-    int wlo[kXwWin], wend[kXwWin];
-#pragma unroll
-    for (int q = 0; q < kXwWin; ++q) {
-        wlo[q] = rec[q];
-        wend[q] = rec[kXwWin + q];
-    }
-    const int64_t base = (int64_t)rec[2 * kXwWin] * BS;
-    const int width = rec[2 * kXwWin + 1];
-    const int total = wend[kXwWin - 1];
+    NUMAMD_XW_LOAD(rec);
+    const int64_t vb = (int64_t)rec[kXwBase] * BS;
+    const int width = rec[kXwWidth];
+    const int total = we3;
     const int len = active ? (int)lens[r0 + t] : 0;
-    // this row's entries: every slot of the tile's width exists (padding)
+    // this row's entries: the tile's slices [0, width) (padding included;
+    // uniform branches) and its slots, packed
     double v[kSellW];
-    int c[kSellW];
 #pragma unroll
-    for (int k = 0; k < kSellW; ++k) {
-        if (k < width) {
-            v[k] = ld_mat<NT>(vals + base + k * BS + t);
-            c[k] = ld_mat<NT>(code + base + k * BS + t);
-        }
-    }
-    // window slots: slot s of window q reads column lo[q] + s - end[q-1]
+    for (int k = 0; k < kSellW; ++k)
+        if (k < width) v[k] = ld_mat<NT>(vals + vb + k * BS + t);
+    uint64_t q0, q1;
+    sell_slots<NT, BS>(code + vb, width, t, q0, q1);
+    // window slots (clamped to the last one: also branch-free)
+    const int sm = max(total - 1, 0);
     double xv_w[NXW];
 #pragma unroll
     for (int u = 0; u < NXW; ++u) {
-        const int s = t + u * BS;
-        if (s < total) {
-            int lo = wlo[0], start = 0;
-#pragma unroll
-            for (int q = 1; q < kXwWin; ++q)
-                if (s >= wend[q - 1]) {
-                    lo = wlo[q];
-                    start = wend[q - 1];
-                }
-            xv_w[u] = x(lo + s - start);
-        }
+        const int s = min(t + u * BS, sm);
+        xv_w[u] = x(xw_col(s, wl0, wl1, wl2, wl3, we0, we1, we2));
     }
 #pragma unroll
     for (int u = 0; u < NXW; ++u) {
@@ -557,16 +608,11 @@ __device__ __forceinline__ void spmv_sell_run(
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kSellW; ++k)
-        if (k < len) acc += v[k] * xw[c[k]];
+        if (k < len) acc += v[k] * xw[sell_slot(k, q0, q1)];
     const int row = r0 + t;
     if constexpr (has_own<Epi>::value) {
         // the own rows are a window range of every tile of a square matrix
-        int os = -1;
-#pragma unroll
-        for (int q = 0; q < kXwWin; ++q) {
-            const int start = q == 0 ? 0 : wend[q - 1];
-            if (row >= wlo[q] && row - wlo[q] < wend[q] - start) os = start + row - wlo[q];
-        }
+        const int os = xw_slot(row, wl0, wl1, wl2, wl3, we0, we1, we2, we3);
         if (os >= 0) {
             epi.own(row, acc, ca, cb, xw[os]);
             return;

@@ -73,6 +73,24 @@ def test_spmv_windows_bit_identical(ks, orc, name, nt64):
     assert np.array_equal(_spmv(ks, dA, x), y)
 
 
+@pytest.mark.parametrize("w", range(1, 9))
+def test_windows_every_width(ks, orc, w):
+    # tiles of width w: the packed slot groups (4 / 2 / 1 per load) and the
+    # FP32 pairs take every shape; rows of w - 1 entries mixed in
+    n = 256 * 6 + 37
+    rp, ci, va = _banded(n, n, w, seed=w)
+    lens = np.full(n, w)
+    lens[::3] = max(w - 1, 0)
+    keep = np.concatenate([np.arange(rp[r], rp[r] + lens[r]) for r in range(n)]).astype(np.int64)
+    rp2 = np.zeros(n + 1, np.int32)
+    np.cumsum(lens, out=rp2[1:])
+    ci2, va2 = ci[keep], va[keep]
+    dA = ks.upload_csr(rp2, ci2, va2, n, n)
+    assert dA.windowed
+    x = np.random.default_rng(w).uniform(-1, 1, n)
+    assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp2, ci2, va2, x))
+
+
 def test_windows_empty_rows_and_specials(ks, orc):
     A = g.laplacian_3d(14)
     rp, ci, va = _with_empty_rows(A)
@@ -162,3 +180,35 @@ def test_cg_windows_bit_identical(ks, name, nt64):
     assert r0["converged"] and r0["iters"] == r1["iters"]
     assert r0["residual"] == r1["residual"]
     assert np.array_equal(x0, x1)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("name", ["diffusion_16", "lap3d_33", "stencil_96x96x5"])
+def test_mixed_windows_streamed(ks, orc, name):
+    # nt32 = 1 (the policy of an FP32 matrix too large for the Infinity
+    # Cache) sends the mixed solver's FP32 steps and refinements through the
+    # windowed sliced kernel; nt32 = 0 through the staged pair kernel.  Both
+    # must reach the FP64 true residual and the oracle's FP64 solution; their
+    # per-tile partials are grouped differently (one per tile against one per
+    # pair), so the step counts may differ by rounding.
+    A = CASES[name]()
+    b = np.random.default_rng(21).uniform(-1, 1, A.nrows)
+    x64, _ = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-12, max_iter=20000)
+    steps = {}
+    for nt32 in (1, 0):
+        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        dA.ensure_f32()
+        dA.set_schedule(nt32=nt32)
+        assert dA.windowed
+        x = _dev(np.zeros(A.nrows))
+        r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+        assert r["converged"] and r["residual"] < 1e-10, (nt32, r)
+        xh = x.cpu().numpy()
+        true = np.linalg.norm(b - A.to_scipy() @ xh) / np.linalg.norm(b)
+        assert true < 1e-10 and abs(true - r["residual"]) <= 1e-12 * r["residual"] + 1e-16
+        assert _rel(xh, x64) < 1e-7
+        steps[nt32] = r["inner_total"]
+    assert abs(steps[1] - steps[0]) <= max(2, 0.02 * steps[0]), steps
