@@ -128,18 +128,22 @@ int num_spmv_xcd(const num_csr* A, const double* d_x, double* d_y, double* d_wor
                  void* stream);
 /* *delta16 = 1 if a delta-coded index is attached (num_csr_attach_d16). */
 int num_csr_index_info(const num_csr* A, int32_t* delta16);
-/* Column windows over the delta-coded index (no reference counterpart: an
- * index layout of this build, used by the FP64 SpMV and CG kernels in place
- * of the delta codes).  With enable = 1 (the default at num_csr_attach_d16)
- * every tile of the schedule must read its x from at most 4 contiguous
- * column ranges of at most 1536 columns in all -- a banded or stencil
- * matrix: the ranges are fetched once per tile into LDS with coalesced
- * loads, and the buffer of num_csr_attach_d16 is re-coded with each entry's
- * slot in them (same entries, same order: bit-identical results).  When any
- * tile does not fit the delta codes stay.  enable = 0 restores the delta
- * codes.  num_csr_set_schedule rebuilds the windows for the new tiles.
- * Synchronises the stream. */
-int num_csr_set_windows(num_csr* A, int32_t enable, void* stream);
+/* Windowed sliced layout (no reference counterpart: a data layout of this
+ * build for banded and stencil matrices, which the FP64 SpMV and CG kernels
+ * and the mixed solver's streamed FP32 steps use in place of the int32 or
+ * delta-coded columns).  It applies when every row holds <= 8 entries and
+ * every tile of the schedule reads x from at most 4 contiguous column
+ * ranges of at most 1152 columns in all: a tile fetches those ranges once
+ * into LDS with coalesced loads and reads its rows -- library-owned copies
+ * of the values and of each entry's 16-bit slot in the ranges, in sliced
+ * order -- straight into registers.  Same entries in the same order:
+ * bit-identical results.  mode 1 builds it (when the matrix fits), 0 drops
+ * it, -1 (the default, applied at num_csr_attach_d16) builds it for
+ * matrices of at least 65,536 rows (below that an SpMV is launch-latency
+ * bound and the build costs more than it saves).  num_csr_set_schedule
+ * rebuilds it for the new tiles.  The layout copies the values: they must
+ * not change while it is built.  Synchronises the stream. */
+int num_csr_set_windows(num_csr* A, int32_t mode, void* stream);
 /* *windowed = 1 if the column windows are in use. */
 int num_csr_window_info(const num_csr* A, int32_t* windowed);
 /* Fails with NUM_ERUNTIME (and frees nothing) while an engine system admitted

@@ -57,13 +57,15 @@ struct num_csr {
     // Windowed sliced layout (num_csr_set_windows), owned, built over
     // A->sched when every tile fits: per-tile records (kXwRec int32: window
     // first columns, cumulative slot ends, slice base, width), the FP64
-    // values and window slots in sliced order, one length byte per row.
+    // values and window slots in sliced order, one length byte per row
+    // (sell_codes and sell_lens point into the sell_vals allocation).
     int32_t* xw = nullptr;
     double* sell_vals = nullptr;
     float* sell_vals32 = nullptr;       // when vals32 is attached
     uint16_t* sell_codes = nullptr;
     uint8_t* sell_lens = nullptr;
-    bool xw_enabled = true;
+    int32_t sell_w = 0;                 // slices per tile (tile b starts at b * sell_w)
+    int32_t xw_mode = -1;               // 1 on, 0 off, -1 from kXwAutoRows rows
     bool owns = false;                  // rowptr/colind/vals allocated here
     std::vector<int32_t> h_rowptr;      // host copy, drives scheduling
     int32_t max_row_nnz = 0;

@@ -1,6 +1,7 @@
 // runtime.hip -- errors, device selection, CSR handle life cycle and the
 // row-block scheduler.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <new>
 
@@ -368,81 +369,132 @@ static int build_d16(num_csr* A, hipStream_t st)
     return NUM_OK;
 }
 
-// Windows of one tile (rows [r0, r1)): the columns its entries read plus
-// its own rows as columns (so the CG kernel finds p_new of its own rows
-// there), as ranges sorted by first column and merged where they overlap or
-// lie at most kXwGap columns apart (the columns in between lie inside
-// [0, ncols) too).  The ranges come from the exact column range of each
-// distinct column offset (column - row) -- a handful for a stencil -- or,
-// for a tile with more than kMaxOff offsets, from its sorted columns.
-// Fills rec (lo[kXwWin], end[kXwWin]) and returns false when the tile needs
-// more than kXwWin windows or kXwSlots slots.
-static bool tile_windows(const int32_t* rp, const int32_t* col, int32_t r0, int32_t r1,
-                         int32_t ncols, int32_t* rec)
+// Windows of one tile (rows [r0, r1)), one workgroup per tile: the columns
+// its entries read plus its own rows as columns (so the CG kernel finds
+// p_new of its own rows there).  Each entry's column offset (column - row)
+// goes into an LDS hash set of kXwOffsets offsets, which keeps each
+// offset's smallest and largest column; thread 0 then sorts those ranges
+// by first column and merges the ones that overlap or lie at most kXwGap
+// columns apart (the columns in between lie inside [0, ncols) too).  Writes
+// the tile's record (windows, first slice b * stride, width) and sets *fail
+// when the tile has more offsets than the set holds, more than
+// kXwWin windows or more than kXwSlots slots.
+constexpr int kXwOffsets = 32;
+constexpr int kXwGap = 16;
+
+__global__ __launch_bounds__(kSpmvBS) void xw_tile_kernel(const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind,
+                                                        const TileEntry* __restrict__ tiles,
+                                                        int32_t ncols, int32_t stride,
+                                                        int32_t* __restrict__ rec,
+                                                        int32_t* __restrict__ fail)
 {
-    constexpr int kMaxOff = 32;
-    constexpr int kXwGap = 16;
-    std::vector<std::pair<int32_t, int32_t>> rng;  // [lo, hi) column ranges
-    if (r0 < std::min(r1, ncols)) rng.push_back({r0, std::min(r1, ncols)});  // own rows
-    int32_t off[kMaxOff], lo[kMaxOff], hi[kMaxOff];
-    int nd = 0, k = 0;
-    bool many = false;
-    for (int32_t r = r0; r < r1 && !many; ++r) {
-        for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
-            const int32_t c = col[j], d = c - r;
-            // consecutive rows repeat one offset sequence: try the next first
-            k = (nd > 0 && k + 1 < nd) ? k + 1 : 0;
-            if (nd == 0 || off[k] != d) {
-                k = 0;
-                while (k < nd && off[k] != d) ++k;
-                if (k == nd) {
-                    if (nd == kMaxOff) {
-                        many = true;
-                        break;
-                    }
-                    off[nd] = d;
-                    lo[nd] = c;
-                    hi[nd] = c + 1;
-                    ++nd;
-                    continue;
-                }
+    __shared__ int s_off[kXwOffsets], s_lo[kXwOffsets], s_hi[kXwOffsets];
+    __shared__ int s_fail, s_width;
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int r0 = tiles[b].row, r1 = tiles[b + 1].row;
+    if (t < kXwOffsets) {
+        s_off[t] = INT_MIN;  // empty
+        s_lo[t] = INT_MAX;
+        s_hi[t] = INT_MIN;
+    }
+    if (t == 0) {
+        s_fail = 0;
+        s_width = 1;  // at least one slice per tile
+    }
+    __syncthreads();
+    auto insert = [&](int d, int lo, int hi) {
+        int h = (int)(((uint32_t)d * 2654435761u) >> 27) & (kXwOffsets - 1);
+        for (int probe = 0; probe < kXwOffsets; ++probe) {
+            const int old = atomicCAS(&s_off[h], INT_MIN, d);
+            if (old == INT_MIN || old == d) {
+                atomicMin(&s_lo[h], lo);
+                atomicMax(&s_hi[h], hi);
+                return;
             }
-            lo[k] = std::min(lo[k], c);
-            hi[k] = std::max(hi[k], c + 1);
+            h = (h + 1) & (kXwOffsets - 1);
+        }
+        s_fail = 1;
+    };
+    if (t == 0 && r0 < min(r1, ncols)) insert(0, r0, min(r1, ncols));  // the own rows
+    const int r = r0 + t;
+    if (r < r1) {
+        const int js = rowptr[r], je = rowptr[r + 1];
+        atomicMax(&s_width, je - js);
+        for (int j = js; j < je; ++j) {
+            const int c = colind[j];
+            insert(c - r, c, c + 1);
         }
     }
-    if (!many) {
-        for (int a = 0; a < nd; ++a) rng.push_back({lo[a], hi[a]});
-    } else {
-        std::vector<int32_t> cs(col + rp[r0], col + rp[r1]);
-        std::sort(cs.begin(), cs.end());
-        const size_t first = rng.size();  // column ranges follow the own rows
-        for (const int32_t c : cs) {
-            if (rng.size() > first && c <= rng.back().second + kXwGap)
-                rng.back().second = std::max(rng.back().second, c + 1);
-            else
-                rng.push_back({c, c + 1});
+    __syncthreads();
+    if (t != 0) return;
+    int lo[kXwOffsets], hi[kXwOffsets], n = 0;
+    for (int q = 0; q < kXwOffsets; ++q)
+        if (s_off[q] != INT_MIN) {
+            int k = n++;  // insertion by first column
+            while (k > 0 && lo[k - 1] > s_lo[q]) {
+                lo[k] = lo[k - 1];
+                hi[k] = hi[k - 1];
+                --k;
+            }
+            lo[k] = s_lo[q];
+            hi[k] = s_hi[q];
         }
-    }
-    std::sort(rng.begin(), rng.end());
-    int nw = 0, total = 0;
-    int32_t wl[kXwWin], wh[kXwWin];
-    for (const auto& q : rng) {
-        if (nw > 0 && q.first <= wh[nw - 1] + kXwGap) {
-            wh[nw - 1] = std::max(wh[nw - 1], q.second);
+    int nw = 0, total = 0, bad = s_fail;
+    int wl[kXwWin], wh[kXwWin];
+    for (int a = 0; a < n && !bad; ++a) {
+        if (nw > 0 && lo[a] <= wh[nw - 1] + kXwGap) {
+            wh[nw - 1] = max(wh[nw - 1], hi[a]);
             continue;
         }
-        if (nw == kXwWin) return false;
-        wl[nw] = q.first;
-        wh[nw] = q.second;
+        if (nw == kXwWin) {
+            bad = 1;
+            break;
+        }
+        wl[nw] = lo[a];
+        wh[nw] = hi[a];
         ++nw;
     }
+    int32_t* R = rec + (size_t)b * kXwRec;
     for (int w = 0; w < kXwWin; ++w) {
         if (w < nw) total += wh[w] - wl[w];
-        rec[w] = w < nw ? wl[w] : 0;
-        rec[kXwWin + w] = total;
+        R[w] = w < nw ? wl[w] : 0;
+        R[kXwWin + w] = total;
     }
-    return total <= kXwSlots;
+    R[kXwBase] = b * stride;
+    R[kXwWidth] = s_width;
+    if (bad || total > kXwSlots) atomicOr(fail, 1);
+}
+
+// Each entry's slot in its tile's windows at its packed position
+// (sell_pos, four per group), zeros in the padding, and one length byte
+// per row; one workgroup per tile, one thread per row slot.
+__global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind,
+                                                        const TileEntry* __restrict__ tiles,
+                                                        const int32_t* __restrict__ rec,
+                                                        uint16_t* __restrict__ codes,
+                                                        uint8_t* __restrict__ lens)
+{
+    const int b = blockIdx.x, i = threadIdx.x;
+    const int r0 = tiles[b].row, r1 = tiles[b + 1].row;
+    const int32_t* R = rec + (size_t)b * kXwRec;
+    const int width = R[kXwWidth];
+    uint16_t* __restrict__ out = codes + (int64_t)R[kXwBase] * kSpmvBS;
+    const int r = r0 + i;
+    const int js = r < r1 ? rowptr[r] : 0;
+    const int len = r < r1 ? rowptr[r + 1] - js : 0;
+    if (r < r1) lens[r] = (uint8_t)len;
+    for (int k = 0; k < width; ++k) {
+        int s = 0;
+        if (k < len) {
+            const int c = colind[js + k];
+            int w = 0;
+            while (w + 1 < kXwWin && R[kXwWin + w] < R[kXwWin + w + 1] && c >= R[w + 1]) ++w;
+            s = (w == 0 ? 0 : R[kXwWin + w - 1]) + c - R[w];
+        }
+        out[sell_pos(k, i, width, 4, kSpmvBS)] = (uint16_t)s;
+    }
 }
 
 // Scatter of the values into the sliced layout, one workgroup per tile and
@@ -473,11 +525,7 @@ static int build_sell32(num_csr* A, hipStream_t st)
     A->sell_vals32 = nullptr;
     if (!A->xw || !A->vals32) return NUM_OK;
     const Schedule& s = A->sched;
-    int32_t last[kXwRec];
-    NUM_HIP(hipMemcpyAsync(last, A->xw + (size_t)(s.nblocks - 1) * kXwRec, sizeof(last),
-                           hipMemcpyDeviceToHost, st));
-    NUM_HIP(hipStreamSynchronize(st));
-    const size_t nval = ((size_t)last[kXwBase] + last[kXwWidth]) * kSpmvBS;
+    const size_t nval = (size_t)s.nblocks * A->sell_w * kSpmvBS;
     NUM_HIP(hipMalloc(&A->sell_vals32, nval * sizeof(float)));
     sell_scatter_kernel<float, 2><<<s.nblocks, kSpmvBS, 0, st>>>(
         A->rowptr, A->vals32, static_cast<const TileEntry*>(s.d_tiles), A->xw, A->sell_vals32);
@@ -493,98 +541,75 @@ static int build_sell32(num_csr* A, hipStream_t st)
 
 void free_windows(num_csr* A)
 {
+    // sell_codes and sell_lens live in the sell_vals allocation
     if (A->xw) (void)hipFree(A->xw);
     if (A->sell_vals) (void)hipFree(A->sell_vals);
     if (A->sell_vals32) (void)hipFree(A->sell_vals32);
-    if (A->sell_codes) (void)hipFree(A->sell_codes);
-    if (A->sell_lens) (void)hipFree(A->sell_lens);
     A->xw = nullptr;
     A->sell_vals = nullptr;
     A->sell_vals32 = nullptr;
     A->sell_codes = nullptr;
     A->sell_lens = nullptr;
+    A->sell_w = 0;
 }
 
-// (Re)builds the windowed layout of A->sched (spmv_sell_run): every tile's
-// windows (tile_windows), its width (longest row, <= kSellW) and its base
-// in the sliced arrays, which hold tile b's entry k of row i at
-// (base_b + k) * 256 + i: the FP64 values (scattered on the device), each
-// entry's LDS slot in the tile's windows (uint16) and one length byte per
-// row.  Nothing is built -- the kernels keep the delta-coded or int32
-// index -- when windows are switched off, the schedule has long or wide
-// rows, or any tile does not fit.  Synchronises `st`.
+// Smallest matrix (rows) for which the default mode builds the layout.
+constexpr int32_t kXwAutoRows = 65536;
+
+// (Re)builds the windowed layout of A->sched (spmv_sell_run) on the device,
+// three kernels and one read of the failure flag: every tile's windows and
+// width (xw_tile_kernel), each entry's packed slot and the row lengths
+// (xw_code_kernel), the FP64 values in sliced order (sell_scatter_kernel);
+// the FP32 values too when attached.  Tile b's slices start at b * W, W =
+// the longest row (a stencil's tiles nearly all have that width; a tile
+// loads only its own width).  Nothing is kept -- the kernels keep the
+// delta-coded or int32 index -- when windows are switched off (or, in the
+// default mode, the matrix is small), the schedule has long or wide rows,
+// or any tile does not fit.  Synchronises `st`.
 static int build_xw(num_csr* A, hipStream_t st)
 {
     free_windows(A);
     const Schedule& s = A->sched;
-    if (!A->xw_enabled || A->nnz == 0 || s.nblocks == 0 || s.long_rows != 0 ||
+    if (A->xw_mode == 0 || (A->xw_mode < 0 && A->nrows < kXwAutoRows))
+        return NUM_OK;
+    if (A->nnz == 0 || s.nblocks == 0 || s.long_rows != 0 ||
         s.rows_per_block > kSpmvBS || A->max_row_nnz > kSellW || s.row_begin != 0 ||
         s.row_end != A->nrows)
         return NUM_OK;
-    const int64_t nnz = A->nnz;
-    std::vector<int32_t> col(nnz);
-    std::vector<TileEntry> tiles(s.nblocks + 1);
-    NUM_HIP(hipMemcpyAsync(col.data(), A->colind, nnz * sizeof(int32_t), hipMemcpyDeviceToHost,
-                           st));
-    NUM_HIP(hipMemcpyAsync(tiles.data(), s.d_tiles, tiles.size() * sizeof(TileEntry),
-                           hipMemcpyDeviceToHost, st));
-    NUM_HIP(hipStreamSynchronize(st));
-    const int32_t* rp = A->h_rowptr.data();
-    std::vector<int32_t> rec((size_t)s.nblocks * kXwRec, 0);
-    int64_t slices = 0;  // 256-entry slices before the current tile
-    for (int b = 0; b < s.nblocks; ++b) {
-        int32_t* R = &rec[(size_t)b * kXwRec];
-        const int32_t r0 = tiles[b].row, r1 = tiles[b + 1].row;
-        if (!tile_windows(rp, col.data(), r0, r1, A->ncols, R)) return NUM_OK;
-        int32_t width = 1;  // at least one slice per tile
-        for (int32_t r = r0; r < r1; ++r) width = std::max(width, rp[r + 1] - rp[r]);
-        if (slices + width > INT32_MAX / kSpmvBS) return NUM_OK;
-        R[kXwBase] = (int32_t)slices;
-        R[kXwWidth] = width;
-        slices += width;
-    }
-    // each entry's slot in its tile's windows, packed four per 8-byte load
-    const size_t nval = (size_t)slices * kSpmvBS;
-    std::vector<uint16_t> slot(nval, 0);
-    std::vector<uint8_t> lens(A->nrows);
-    for (int b = 0; b < s.nblocks; ++b) {
-        const int32_t* R = &rec[(size_t)b * kXwRec];
-        uint16_t* out = slot.data() + (size_t)R[kXwBase] * kSpmvBS;
-        for (int32_t r = tiles[b].row; r < tiles[b + 1].row; ++r) {
-            const int32_t i = r - tiles[b].row;
-            lens[r] = (uint8_t)(rp[r + 1] - rp[r]);
-            for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
-                const int32_t c = col[j], k = j - rp[r];
-                int w = 0;
-                while (w + 1 < kXwWin && R[kXwWin + w] < R[kXwWin + w + 1] && c >= R[w + 1])
-                    ++w;
-                const int32_t start = w == 0 ? 0 : R[kXwWin + w - 1];
-                out[sell_pos(k, i, R[kXwWidth], 4, kSpmvBS)] = (uint16_t)(start + c - R[w]);
-            }
-        }
-    }
-    hipError_t e = hipMalloc(&A->xw, rec.size() * sizeof(int32_t));
-    if (e == hipSuccess) e = hipMalloc(&A->sell_vals, nval * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&A->sell_codes, nval * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMalloc(&A->sell_lens, std::max<size_t>(A->nrows, 1));
+    const int32_t W = std::max(1, A->max_row_nnz);
+    if ((int64_t)s.nblocks * W > INT32_MAX / kSpmvBS) return NUM_OK;
+    const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
+    const size_t nrec = (size_t)s.nblocks * kXwRec;
+    const size_t nval = (size_t)s.nblocks * W * kSpmvBS;
+    int32_t h_fail = 0;
+    // the records, then the failure flag; the FP64 values, then the slots,
+    // then the row lengths
+    hipError_t e = hipMalloc(&A->xw, (nrec + 1) * sizeof(int32_t));
+    int32_t* d_fail = A->xw ? A->xw + nrec : nullptr;
+    if (e == hipSuccess) e = hipMemsetAsync(d_fail, 0, sizeof(int32_t), st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(A->xw, rec.data(), rec.size() * sizeof(int32_t),
-                           hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(A->sell_codes, slot.data(), nval * sizeof(uint16_t),
-                           hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(A->sell_lens, lens.data(), lens.size(), hipMemcpyHostToDevice, st);
+        e = hipMalloc(&A->sell_vals, nval * (sizeof(double) + sizeof(uint16_t)) + A->nrows);
     if (e == hipSuccess) {
-        sell_scatter_kernel<double, 1><<<s.nblocks, kSpmvBS, 0, st>>>(
-            A->rowptr, A->vals, static_cast<const TileEntry*>(s.d_tiles), A->xw, A->sell_vals);
+        A->sell_codes = reinterpret_cast<uint16_t*>(A->sell_vals + nval);
+        A->sell_lens = reinterpret_cast<uint8_t*>(A->sell_codes + nval);
+        xw_tile_kernel<<<s.nblocks, kSpmvBS, 0, st>>>(A->rowptr, A->colind, tiles, A->ncols, W,
+                                                      A->xw, d_fail);
+        xw_code_kernel<<<s.nblocks, kSpmvBS, 0, st>>>(A->rowptr, A->colind, tiles, A->xw,
+                                                      A->sell_codes, A->sell_lens);
+        sell_scatter_kernel<double, 1><<<s.nblocks, kSpmvBS, 0, st>>>(A->rowptr, A->vals, tiles,
+                                                                      A->xw, A->sell_vals);
         e = hipGetLastError();
     }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&h_fail, d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
+    if (e != hipSuccess || h_fail) {
         free_windows(A);
-        return fail(NUM_EHIP, "windowed layout build failed: %s", hipGetErrorString(e));
+        if (e != hipSuccess)
+            return fail(NUM_EHIP, "windowed layout build failed: %s", hipGetErrorString(e));
+        return NUM_OK;  // a tile does not fit: no windows
     }
+    A->sell_w = W;
     return build_sell32(A, st);
 }
 
@@ -759,12 +784,13 @@ int num_csr_attach_d16(num_csr* A, uint16_t* d_codes, void* stream, int32_t* att
     return NUM_OK;
 }
 
-int num_csr_set_windows(num_csr* A, int32_t enable, void* stream)
+int num_csr_set_windows(num_csr* A, int32_t mode, void* stream)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
     int rc = check_device(A);
     if (rc) return rc;
-    A->xw_enabled = enable != 0;
+    if (mode < -1 || mode > 1) return fail(NUM_EINVAL, "mode must be -1, 0 or 1, got %d", mode);
+    A->xw_mode = mode;
     return build_xw(A, (hipStream_t)stream);
 }
 

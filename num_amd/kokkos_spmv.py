@@ -229,17 +229,15 @@ class CrsMatrix:
             if not on.value:
                 self._d16 = None
 
-    def set_windows(self, enable: bool = True) -> None:
-        """Switch the column-window index on or off (num_csr_set_windows):
-        a banded or stencil matrix whose tiles each read x from a few
-        contiguous column ranges fetches them once per tile into LDS.  Same
+    def set_windows(self, enable: bool | None = True) -> None:
+        """Build (True) or drop (False) the windowed sliced layout
+        (num_csr_set_windows): a banded or stencil matrix whose tiles each
+        read x from a few contiguous column ranges fetches them once per
+        tile into LDS and streams its rows in sliced order.  None restores
+        the default (built for matrices of at least 65,536 rows).  Same
         results either way (bit-identical)."""
-        call("num_csr_set_windows", self._h, int(bool(enable)), _stream())
-        if self._d16 is not None:
-            on = C.c_int32()
-            call("num_csr_index_info", self._h, C.byref(on))
-            if not on.value:
-                self._d16 = None
+        mode = -1 if enable is None else int(bool(enable))
+        call("num_csr_set_windows", self._h, mode, _stream())
 
     @property
     def windowed(self) -> bool:

@@ -6,7 +6,8 @@ and read x from those ranges, fetched once per tile into LDS.  The entries
 and their summation order are unchanged, so every result must be
 bit-identical to the oracle's row loop (src/spmv_kernel.hpp:90-97) and to
 the delta-coded and int32-column kernels; matrices outside the layout must
-keep the other index forms.
+keep the other index forms.  The layout is built by default from 65,536
+rows; the smaller test matrices ask for it (_up).
 """
 import numpy as np
 import pytest
@@ -19,6 +20,12 @@ pytestmark = pytest.mark.gpu
 
 def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _up(ks, rowptr, colind, values, nrows, ncols):
+    dA = ks.upload_csr(rowptr, colind, values, nrows, ncols)
+    dA.set_windows(True)
+    return dA
 
 
 def _spmv(ks, dA, x):
@@ -59,7 +66,7 @@ CASES = {
 @pytest.mark.parametrize("nt64", [1, 0])
 def test_spmv_windows_bit_identical(ks, orc, name, nt64):
     A = CASES[name]()
-    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
     dA.set_schedule(nt64=nt64)
     assert dA.index_format == "window16"
     x = np.random.default_rng(7).uniform(-1, 1, A.ncols)
@@ -85,7 +92,7 @@ def test_windows_every_width(ks, orc, w):
     rp2 = np.zeros(n + 1, np.int32)
     np.cumsum(lens, out=rp2[1:])
     ci2, va2 = ci[keep], va[keep]
-    dA = ks.upload_csr(rp2, ci2, va2, n, n)
+    dA = _up(ks, rp2, ci2, va2, n, n)
     assert dA.windowed
     x = np.random.default_rng(w).uniform(-1, 1, n)
     assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp2, ci2, va2, x))
@@ -94,7 +101,7 @@ def test_windows_every_width(ks, orc, w):
 def test_windows_empty_rows_and_specials(ks, orc):
     A = g.laplacian_3d(14)
     rp, ci, va = _with_empty_rows(A)
-    dA = ks.upload_csr(rp, ci, va, A.nrows, A.ncols)
+    dA = _up(ks, rp, ci, va, A.nrows, A.ncols)
     assert dA.windowed
     x = np.random.default_rng(8).standard_normal(A.ncols)
     x[5] = np.inf
@@ -107,10 +114,20 @@ def test_windows_empty_rows_and_specials(ks, orc):
     assert np.all(y[::7] == 0.0)
 
 
+def _banded_wrap(n, m, w, seed):
+    """n x m, row r holds w consecutive columns from r mod (m - w): a band
+    that wraps to column 0 past the last column (a few column offsets per
+    tile, as the window build needs)."""
+    rp = np.arange(0, w * (n + 1), w, dtype=np.int32)
+    ci = np.concatenate([r % (m - w) + np.arange(w) for r in range(n)]).astype(np.int32)
+    va = np.random.default_rng(seed).uniform(-1, 1, ci.size)
+    return rp, ci, va
+
+
 @pytest.mark.parametrize("n,m", [(3000, 2000), (2000, 3000)])
 def test_windows_rectangular(ks, orc, n, m):
-    rp, ci, va = _banded(n, m, 6, seed=n)
-    dA = ks.upload_csr(rp, ci, va, n, m)
+    rp, ci, va = _banded_wrap(n, m, 6, seed=n)
+    dA = _up(ks, rp, ci, va, n, m)
     assert dA.windowed
     x = np.random.default_rng(9).uniform(-1, 1, m)
     assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp, ci, va, x))
@@ -118,15 +135,15 @@ def test_windows_rectangular(ks, orc, n, m):
 
 def test_windows_refused_outside_the_layout(ks, orc):
     x_of = lambda m: np.random.default_rng(10).uniform(-1, 1, m)
-    # scattered columns: more than 4 ranges per tile
+    # scattered columns: more than 32 column offsets per tile
     L, _ = g.lcg_matrix(5000, 5, seed=3)
-    dL = ks.upload_csr(L.rowptr, L.colind, L.values, L.nrows, L.ncols)
+    dL = _up(ks, L.rowptr, L.colind, L.values, L.nrows, L.ncols)
     assert not dL.windowed
     assert np.array_equal(_spmv(ks, dL, x_of(L.ncols)),
                           orc.spmv(L.rowptr, L.colind, L.values, x_of(L.ncols)))
     # rows of 9 entries: wider than the 8 slices
     rp, ci, va = _banded(2000, 2000, 9, seed=1)
-    dB = ks.upload_csr(rp, ci, va, 2000, 2000)
+    dB = _up(ks, rp, ci, va, 2000, 2000)
     assert not dB.windowed and dB.index_format in ("delta16", "int32")
     assert np.array_equal(_spmv(ks, dB, x_of(2000)), orc.spmv(rp, ci, va, x_of(2000)))
     # 3-D stencils: rows 600 apart make 5 ranges per tile (over 4); rows 250
@@ -134,20 +151,20 @@ def test_windows_refused_outside_the_layout(ks, orc):
     # plane ranges needs 1268 > 1152 slots
     for dims in ((600, 4, 3), (250, 4, 3)):
         S = g.stencil(dims, 0.25)
-        dS = ks.upload_csr(S.rowptr, S.colind, S.values, S.nrows, S.ncols)
+        dS = _up(ks, S.rowptr, S.colind, S.values, S.nrows, S.ncols)
         assert not dS.windowed, dims
         assert np.array_equal(_spmv(ks, dS, x_of(S.ncols)),
                               orc.spmv(S.rowptr, S.colind, S.values, x_of(S.ncols)))
     # switched off on request
     A = g.laplacian_3d(12)
-    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
     dA.set_windows(False)
     assert not dA.windowed
 
 
 def test_windows_survive_reschedule(ks, orc):
     A = g.laplacian_3d(24)
-    dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
     x = np.random.default_rng(11).uniform(-1, 1, A.ncols)
     ref = orc.spmv(A.rowptr, A.colind, A.values, x)
     for rpb, npb, nt in ((64, 0, 1), (200, 900, 0), (0, 0, -1)):
@@ -169,7 +186,7 @@ def test_cg_windows_bit_identical(ks, name, nt64):
     b = _dev(np.random.default_rng(12).uniform(-1, 1, A.nrows))
     out = []
     for on in (True, False):
-        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
         dA.set_windows(on)
         dA.set_schedule(nt64=nt64)
         assert dA.windowed == on
@@ -199,7 +216,7 @@ def test_mixed_windows_streamed(ks, orc, name):
     x64, _ = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-12, max_iter=20000)
     steps = {}
     for nt32 in (1, 0):
-        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
         dA.ensure_f32()
         dA.set_schedule(nt32=nt32)
         assert dA.windowed
@@ -212,3 +229,21 @@ def test_mixed_windows_streamed(ks, orc, name):
         assert _rel(xh, x64) < 1e-7
         steps[nt32] = r["inner_total"]
     assert abs(steps[1] - steps[0]) <= max(2, 0.02 * steps[0]), steps
+
+
+def test_windows_default_from_65536_rows(ks):
+    # built by default for matrices of at least 65,536 rows only
+    small = g.laplacian_3d(40)   # 64,000 rows
+    large = g.laplacian_3d(41)   # 68,921 rows
+    ds = ks.upload_csr(small.rowptr, small.colind, small.values, small.nrows, small.ncols)
+    dl = ks.upload_csr(large.rowptr, large.colind, large.values, large.nrows, large.ncols)
+    assert not ds.windowed and dl.windowed
+    ds.set_windows(True)
+    dl.set_windows(False)
+    assert ds.windowed and not dl.windowed
+    ds.set_windows(None)
+    dl.set_windows(None)
+    assert not ds.windowed and dl.windowed
+    from num_amd._lib import call
+    with pytest.raises(ValueError):
+        call("num_csr_set_windows", dl.handle, 2, None)
