@@ -123,16 +123,22 @@ def test_edge_cases_per_system(ks):
     v = np.array([1.0, -1.0, -1.0, 1.0])
     dS = ks.upload_csr(rp, ci, v, 2, 2)
     s2 = eng.add(dS, _dev(np.ones(2)), _dev(np.zeros(2)), tol=1e-10, max_iter=10)
+    # a NaN in b: that system stops unconverged, nothing else is touched
+    A4, dA4 = _upload(ks, (40,), 0.5)
+    b4 = np.ones(40)
+    b4[17] = np.nan
+    s4 = eng.add(dA4, _dev(b4), _dev(np.zeros(40)), tol=1e-10, max_iter=50)
     # a healthy system beside them
     A3, dA3 = _upload(ks, (30, 20), 0.1)
     b3 = np.random.default_rng(1).uniform(-1, 1, A3.nrows)
     x3 = _dev(np.zeros(A3.nrows))
     s3 = eng.add(dA3, _dev(b3), x3, tol=1e-10, max_iter=2000)
     eng.run(4)
-    r0, r1, r2, r3 = (eng.result(s) for s in (s0, s1, s2, s3))
+    r0, r1, r2, r3, r4 = (eng.result(s) for s in (s0, s1, s2, s3, s4))
     assert r0["iters"] == 0 and r0["converged"] and r0["residual"] == 0.0
     assert r1["iters"] == 5 and not r1["converged"]
     assert r2["breakdown"] and not r2["converged"]
+    assert not r4["converged"] and r4["iters"] <= 50
     assert r3["converged"]
     true = np.linalg.norm(b3 - A3.to_scipy() @ x3.cpu().numpy()) / np.linalg.norm(b3)
     assert true < 2e-10
