@@ -108,6 +108,41 @@ def test_zipf_long_rows(ks, orc):
     assert np.array_equal(y[short], y_ref[short])
 
 
+def test_arrow_matrix(ks, orc):
+    # An arrow matrix: row 0 holds every column (a single row of 200,000
+    # entries, far past a tile's capacity: a workgroup of its own), column 0
+    # is read by every row, and the diagonal.  Row 0 within 1e-12 of the
+    # oracle, every other row (2 entries) bit-identical; the multi-RHS
+    # kernels on the same matrix (k = 3: lane groups, k = 128: a wave per
+    # row) bit-identical on the short rows too.
+    n = 200_000
+    rng = np.random.default_rng(21)
+    rp = np.empty(n + 1, np.int64)
+    rp[0], rp[1] = 0, n
+    rp[2:] = n + 2 * np.arange(1, n)
+    ci = np.empty(rp[-1], np.int32)
+    ci[:n] = np.arange(n)
+    ci[n::2] = 0
+    ci[n + 1::2] = np.arange(1, n)
+    va = rng.uniform(-1, 1, rp[-1])
+    rp = rp.astype(np.int32)
+    dA = ks.upload_csr(rp, ci, va, n, n)
+    assert dA.schedule_info()["long_rows"] == 1
+    x = rng.uniform(-1, 1, n)
+    y = _spmv(ks, dA, x)
+    y_ref = orc.spmv(rp, ci, va, x)
+    assert abs(y[0] - y_ref[0]) <= 1e-12 * np.abs(va[:n] * x).sum()
+    assert np.array_equal(y[1:], y_ref[1:])
+    for k in (3, 128):
+        X = rng.standard_normal((n, k))
+        Y = torch.zeros((n, k), dtype=torch.float64, device="cuda")
+        ks.batch_spmv(dA, _dev(X), Y)
+        Yr = orc.batch_spmv(rp, ci, va, X)
+        Yg = Y.cpu().numpy()
+        assert np.array_equal(Yg[1:], Yr[1:])
+        assert np.all(np.abs(Yg[0] - Yr[0]) <= 1e-12 * (np.abs(va[:n, None] * X).sum(axis=0)))
+
+
 @pytest.mark.parametrize("lo,hi,frac_short", [(33, 300, 0.0), (1, 1800, 0.5), (30, 40, 0.3)])
 def test_medium_rows(ks, orc, lo, hi, frac_short):
     """Tiles with many rows longer than 32 entries (each goes to a wave, the
