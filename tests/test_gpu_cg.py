@@ -259,7 +259,7 @@ def test_full_width_update_groups(ks, orc, mixed):
         assert abs(r["iters"] - ro["iters"]) <= max(1, int(0.02 * ro["iters"]))
 
 
-@pytest.mark.parametrize("max_iter", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_iterate_after_each_iteration_count(ks, orc, max_iter):
     # the x update runs every other iteration (two at once); a solve that
     # stops after an odd or an even number of iterations must return the
@@ -305,7 +305,7 @@ def test_breakdown_keeps_last_iterate(ks, mixed):
     assert x.cpu().numpy().tolist() == [2.0, 2.0]
 
 
-@pytest.mark.parametrize("max_iter", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 3, 4, 5, 8])
 def test_mixed_stop_after_each_step_count(ks, max_iter):
     # the mixed solver ends with an FP64 refinement whose true residual it
     # reports: the returned x must be the iterate that residual belongs to,
