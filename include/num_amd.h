@@ -146,6 +146,14 @@ int num_csr_index_info(const num_csr* A, int32_t* delta16);
 int num_csr_set_windows(num_csr* A, int32_t mode, void* stream);
 /* *windowed = 1 if the column windows are in use. */
 int num_csr_window_info(const num_csr* A, int32_t* windowed);
+/* Which kernel takes the FP32 steps of num_cg_solve_mixed (no reference
+ * counterpart): mode 1 the windowed sliced step (when the windowed layout
+ * and the FP32 values are attached), 0 the staged row-tile / pair kernels,
+ * -1 (the default) the windowed step when the FP32 copy streams from HBM and
+ * the matrix's dominant long-range column offset is at least 16,384 (a 3-D
+ * grid plane of that many rows; on 2-D and short-plane grids the staged
+ * pair kernel is faster).  Never changes which layout is built. */
+int num_csr_set_mixed_path(num_csr* A, int32_t mode);
 /* Fails with NUM_ERUNTIME (and frees nothing) while an engine system admitted
  * with A (num_engine_add) has not been removed. */
 int num_csr_destroy(num_csr* A);

@@ -1131,15 +1131,22 @@ static bool mix_paired(const num_csr* A)
            (A->sched.npt == 5 || A->sched.npt == 7);
 }
 
-// The windowed sliced path applies when the layout carries the FP32 values
-// and the FP32 matrix streams from HBM (192^3 Laplacian: 88 ms against
-// 106 ms for the staged pair kernel).  An FP32 matrix that stays in the
-// Infinity Cache runs the pair kernel: two tiles per workgroup halve the
-// per-workgroup fold of the update kernel's partials, which a cache-resident
-// step pays relatively more for (128^3 diffusion: 91.8 ms against 97.9 ms).
+// The windowed sliced path applies (by default) when the layout carries the
+// FP32 values, the FP32 matrix streams from HBM and its dominant long-range
+// column offset is at least kMixSellOffset rows.  Per FP32 step, windowed
+// against the staged pair kernel: 192^3 (offset 36,864) 127 against 152 us,
+// 96x96x1200 (9,216) 200 against 191 us, 2-D 4000^2 (4,000) 262 against 235
+// us.  An FP32 matrix that stays in the Infinity Cache runs the pair kernel:
+// two tiles per workgroup halve the per-workgroup fold of the update
+// kernel's partials, which a cache-resident step pays relatively more for
+// (128^3 diffusion: 91.8 ms against 97.9 ms).  num_csr_set_mixed_path
+// overrides the choice.
+constexpr int32_t kMixSellOffset = 16384;
 static bool mix_sliced(const num_csr* A)
 {
-    return A->xw && A->sell_vals32 && A->sched.nt32 != 0;
+    if (!A->xw || !A->sell_vals32) return false;
+    if (A->mix_path >= 0) return A->mix_path == 1;
+    return A->sched.nt32 != 0 && A->offset_d >= kMixSellOffset;
 }
 
 static void launch_mix_spmv(const num_csr* A, const MixArgs& a, hipStream_t st)
@@ -1245,6 +1252,14 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol, i
     res->residual = h.rel;
     if (h.breakdown)
         return fail(NUM_ERUNTIME, "CG: p'Ap is near zero - A may not be SPD");
+    return NUM_OK;
+}
+
+int num_csr_set_mixed_path(num_csr* A, int32_t mode)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    if (mode < -1 || mode > 1) return fail(NUM_EINVAL, "mode must be -1, 0 or 1, got %d", mode);
+    A->mix_path = mode;
     return NUM_OK;
 }
 

@@ -66,6 +66,7 @@ struct num_csr {
     uint8_t* sell_lens = nullptr;
     int32_t sell_w = 0;                 // slices per tile (tile b starts at b * sell_w)
     int32_t xw_mode = -1;               // 1 on, 0 off, -1 from kXwAutoRows rows
+    int32_t mix_path = -1;              // mixed FP32 step: 1 windowed, 0 staged, -1 auto
     bool owns = false;                  // rowptr/colind/vals allocated here
     std::vector<int32_t> h_rowptr;      // host copy, drives scheduling
     int32_t max_row_nnz = 0;

@@ -239,6 +239,13 @@ class CrsMatrix:
         mode = -1 if enable is None else int(bool(enable))
         call("num_csr_set_windows", self._h, mode, _stream())
 
+    def set_mixed_path(self, mode: int = -1) -> None:
+        """Kernel of cg_solve_mixed's FP32 steps (num_csr_set_mixed_path):
+        1 the windowed sliced step, 0 the staged pair / row-tile kernels, -1
+        (default) windowed when the FP32 copy streams from HBM and the
+        matrix's long-range column offset is at least 16,384 rows."""
+        call("num_csr_set_mixed_path", self._h, int(mode))
+
     @property
     def windowed(self) -> bool:
         """True when the FP64 kernels read x through column windows."""

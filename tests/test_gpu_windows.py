@@ -205,12 +205,13 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("name", ["diffusion_16", "lap3d_33", "stencil_96x96x5"])
 def test_mixed_windows_streamed(ks, orc, name):
-    # nt32 = 1 (the policy of an FP32 matrix too large for the Infinity
-    # Cache) sends the mixed solver's FP32 steps and refinements through the
-    # windowed sliced kernel; nt32 = 0 through the staged pair kernel.  Both
-    # must reach the FP64 true residual and the oracle's FP64 solution; their
-    # per-tile partials are grouped differently (one per tile against one per
-    # pair), so the step counts may differ by rounding.
+    # Mixed path 1 sends the mixed solver's FP32 steps and refinements
+    # through the windowed sliced kernel (FP32 copy streamed: nt32 = 1, the
+    # policy of an FP32 matrix too large for the Infinity Cache); path 0,
+    # with the FP32 copy cached (nt32 = 0), through the staged pair kernel.
+    # Both must reach the FP64 true residual and the oracle's FP64 solution;
+    # their per-tile partials are grouped differently (one per tile against
+    # one per pair), so the step counts may differ by rounding.
     A = CASES[name]()
     b = np.random.default_rng(21).uniform(-1, 1, A.nrows)
     x64, _ = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-12, max_iter=20000)
@@ -219,6 +220,7 @@ def test_mixed_windows_streamed(ks, orc, name):
         dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
         dA.ensure_f32()
         dA.set_schedule(nt32=nt32)
+        dA.set_mixed_path(nt32)
         assert dA.windowed
         x = _dev(np.zeros(A.nrows))
         r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
@@ -247,3 +249,5 @@ def test_windows_default_from_65536_rows(ks):
     from num_amd._lib import call
     with pytest.raises(ValueError):
         call("num_csr_set_windows", dl.handle, 2, None)
+    with pytest.raises(ValueError):
+        dl.set_mixed_path(2)
