@@ -59,6 +59,9 @@ CASES = {
     "lap2d_70": lambda: g.laplacian_2d(70),
     "diffusion_16": lambda: g.diffusion_3d(16, sigma=1.0, seed=3),
     "stencil_96x96x5": lambda: g.stencil((96, 96, 5), 0.25),  # plane 9216: 3 windows
+    # periodic 600 x 600 torus: the tiles of the first and last grid rows
+    # read the far side's columns (a fourth window), the others three
+    "lattice_600": lambda: g.lattice_gauge(600),
 }
 
 
@@ -177,7 +180,7 @@ def test_windows_survive_reschedule(ks, orc):
     assert np.array_equal(_spmv(ks, dA, x), ref)
 
 
-@pytest.mark.parametrize("name", ["lap3d_20", "diffusion_16", "lap3d_33"])
+@pytest.mark.parametrize("name", ["lap3d_20", "diffusion_16", "lap3d_33", "lattice_600"])
 @pytest.mark.parametrize("nt64", [1, 0])
 def test_cg_windows_bit_identical(ks, name, nt64):
     # the windowed CG kernel reads p_new of its own rows from the windows:
