@@ -137,14 +137,19 @@ int num_csr_index_info(const num_csr* A, int32_t* delta16);
  * into LDS with coalesced loads and reads its rows -- library-owned copies
  * of the values and of each entry's 16-bit slot in the ranges, in sliced
  * order -- straight into registers.  Same entries in the same order:
- * bit-identical results.  mode 1 builds it (when the matrix fits), 0 drops
+ * bit-identical results.  Where every tile holds at most 16 distinct row
+ * patterns (a row's slots minus its position in the tile: a stencil's
+ * interior and boundary rows), each row keeps one pattern id byte into its
+ * tile's table instead of a 2-byte slot per entry.  mode 1 builds it (when
+ * the matrix fits), 2 builds it with a slot per entry only, 0 drops
  * it, -1 (the default, applied at num_csr_attach_d16) builds it for
  * matrices of at least 65,536 rows (below that an SpMV is launch-latency
  * bound and the build costs more than it saves).  num_csr_set_schedule
  * rebuilds it for the new tiles.  The layout copies the values: they must
  * not change while it is built.  Synchronises the stream. */
 int num_csr_set_windows(num_csr* A, int32_t mode, void* stream);
-/* *windowed = 1 if the column windows are in use. */
+/* *windowed = 2 if the column windows are in use with row patterns, 1 with
+ * a slot per entry, 0 if not built. */
 int num_csr_window_info(const num_csr* A, int32_t* windowed);
 /* Which kernel takes the FP32 steps of num_cg_solve_mixed (no reference
  * counterpart): mode 1 the windowed sliced step (when the windowed layout

@@ -92,6 +92,14 @@ constexpr int kSellW = 8;
 constexpr int kXwBase = 2 * kXwWin;
 constexpr int kXwWidth = 2 * kXwWin + 1;
 constexpr int kXwRec = 2 * kXwWin + 2;
+// Row patterns of the windowed layout: a row's window slots minus its lane
+// in the tile (plus kXwPatBias, as uint16; 0 past the row's length) are the
+// same for every row of a tile that reads the same column offsets (a
+// stencil's interior rows), so a tile keeps a table of at most kXwPat
+// distinct patterns, two uint64 of four 16-bit slots each, and one id byte
+// per row replaces the row's 2-byte slots.
+constexpr int kXwPat = 16;
+constexpr int kXwPatBias = 256;
 
 // Position of entry k of lane i (of bs lanes) in a tile's packed region of
 // w slices, elements of which `pack` (4 for slots, 2 for FP32 values) share

@@ -65,6 +65,12 @@ struct num_csr {
     uint16_t* sell_codes = nullptr;
     uint8_t* sell_lens = nullptr;
     int32_t sell_w = 0;                 // slices per tile (tile b starts at b * sell_w)
+    // Row-pattern form of the window slots (when every tile holds at most
+    // kXwPat distinct patterns), owned: per tile 2 * kXwPat uint64, then
+    // one pattern id per row.
+    uint64_t* sell_pat = nullptr;
+    uint8_t* sell_pid = nullptr;
+    int32_t pat_mode = -1;              // row patterns: 0 off (the slots), else when they fit
     int32_t xw_mode = -1;               // 1 on, 0 off, -1 from kXwAutoRows rows
     int32_t mix_path = -1;              // mixed FP32 step: 1 windowed, 0 staged, -1 auto
     bool owns = false;                  // rowptr/colind/vals allocated here

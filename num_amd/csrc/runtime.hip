@@ -468,14 +468,25 @@ __global__ __launch_bounds__(kSpmvBS) void xw_tile_kernel(const int32_t* __restr
 
 // Each entry's slot in its tile's windows at its packed position
 // (sell_pos, four per group), zeros in the padding, and one length byte
-// per row; one workgroup per tile, one thread per row slot.
+// per row; one workgroup per tile, one thread per row slot.  With `pat`:
+// also the row-pattern form (kXwPat): each row's slots minus its lane plus
+// kXwPatBias (0 past its length) are compared in rounds, the tile's
+// distinct patterns numbered in order of their first row, then the rows'
+// id bytes and the tile's table are written; a tile with more than kXwPat
+// patterns sets *pfail.
 __global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restrict__ rowptr,
                                                         const int32_t* __restrict__ colind,
                                                         const TileEntry* __restrict__ tiles,
                                                         const int32_t* __restrict__ rec,
                                                         uint16_t* __restrict__ codes,
-                                                        uint8_t* __restrict__ lens)
+                                                        uint8_t* __restrict__ lens,
+                                                        uint64_t* __restrict__ pat,
+                                                        uint8_t* __restrict__ pid,
+                                                        int32_t* __restrict__ pfail)
 {
+    __shared__ unsigned long long s_q[2 * kSpmvBS];
+    __shared__ unsigned long long s_tab[2 * kXwPat];
+    __shared__ int s_lead;
     const int b = blockIdx.x, i = threadIdx.x;
     const int r0 = tiles[b].row, r1 = tiles[b + 1].row;
     const int32_t* R = rec + (size_t)b * kXwRec;
@@ -485,6 +496,7 @@ __global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restr
     const int js = r < r1 ? rowptr[r] : 0;
     const int len = r < r1 ? rowptr[r + 1] - js : 0;
     if (r < r1) lens[r] = (uint8_t)len;
+    uint64_t q0 = 0, q1 = 0;
     for (int k = 0; k < width; ++k) {
         int s = 0;
         if (k < len) {
@@ -492,9 +504,46 @@ __global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restr
             int w = 0;
             while (w + 1 < kXwWin && R[kXwWin + w] < R[kXwWin + w + 1] && c >= R[w + 1]) ++w;
             s = (w == 0 ? 0 : R[kXwWin + w - 1]) + c - R[w];
+            const uint64_t rel = (uint64_t)(uint16_t)(s - i + kXwPatBias);
+            if (k < 4) q0 |= rel << (16 * k);
+            else q1 |= rel << (16 * (k - 4));
         }
         out[sell_pos(k, i, width, 4, kSpmvBS)] = (uint16_t)s;
     }
+    if (!pat) return;
+    // number the distinct patterns in order of first row: each round the
+    // first row without an id leads, every row with its pattern takes the
+    // next id (at most kXwPat + 1 rounds, all threads in step)
+    const bool active = r < r1;
+    s_q[2 * i] = q0;
+    s_q[2 * i + 1] = q1;
+    int myid = -1, np = 0, over = 0;
+    for (;;) {
+        if (i == 0) s_lead = INT_MAX;
+        __syncthreads();
+        if (active && myid < 0) atomicMin(&s_lead, i);
+        __syncthreads();
+        const int lead = s_lead;
+        if (lead == INT_MAX) break;
+        if (np == kXwPat) {
+            over = 1;
+            break;
+        }
+        const unsigned long long a = s_q[2 * lead], c = s_q[2 * lead + 1];
+        if (active && myid < 0 && q0 == a && q1 == c) myid = np;
+        if (i == 0) {
+            s_tab[2 * np] = a;
+            s_tab[2 * np + 1] = c;
+        }
+        ++np;
+        __syncthreads();  // s_lead is reset only after every thread read it
+    }
+    if (over) {
+        if (i == 0) atomicOr(pfail, 1);
+        return;
+    }
+    if (active) pid[r] = (uint8_t)myid;
+    if (i < 2 * kXwPat) pat[(size_t)b * 2 * kXwPat + i] = i < 2 * np ? s_tab[i] : 0ull;
 }
 
 // Scatter of the values into the sliced layout, one workgroup per tile and
@@ -539,12 +588,21 @@ static int build_sell32(num_csr* A, hipStream_t st)
     return NUM_OK;
 }
 
+static void free_patterns(num_csr* A)
+{
+    // sell_pid lives in the sell_pat allocation
+    if (A->sell_pat) (void)hipFree(A->sell_pat);
+    A->sell_pat = nullptr;
+    A->sell_pid = nullptr;
+}
+
 void free_windows(num_csr* A)
 {
     // sell_codes and sell_lens live in the sell_vals allocation
     if (A->xw) (void)hipFree(A->xw);
     if (A->sell_vals) (void)hipFree(A->sell_vals);
     if (A->sell_vals32) (void)hipFree(A->sell_vals32);
+    free_patterns(A);
     A->xw = nullptr;
     A->sell_vals = nullptr;
     A->sell_vals32 = nullptr;
@@ -581,34 +639,40 @@ static int build_xw(num_csr* A, hipStream_t st)
     const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
     const size_t nrec = (size_t)s.nblocks * kXwRec;
     const size_t nval = (size_t)s.nblocks * W * kSpmvBS;
-    int32_t h_fail = 0;
-    // the records, then the failure flag; the FP64 values, then the slots,
-    // then the row lengths
-    hipError_t e = hipMalloc(&A->xw, (nrec + 1) * sizeof(int32_t));
+    const size_t npat = (size_t)s.nblocks * 2 * kXwPat;
+    int32_t h_fail[2] = {0, 0};
+    // the records, then the failure flags (windows, patterns); the FP64
+    // values, then the slots, then the row lengths; the pattern tables,
+    // then the row pattern ids
+    hipError_t e = hipMalloc(&A->xw, (nrec + 2) * sizeof(int32_t));
     int32_t* d_fail = A->xw ? A->xw + nrec : nullptr;
-    if (e == hipSuccess) e = hipMemsetAsync(d_fail, 0, sizeof(int32_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_fail, 0, 2 * sizeof(int32_t), st);
     if (e == hipSuccess)
         e = hipMalloc(&A->sell_vals, nval * (sizeof(double) + sizeof(uint16_t)) + A->nrows);
+    if (e == hipSuccess) e = hipMalloc(&A->sell_pat, npat * sizeof(uint64_t) + A->nrows);
     if (e == hipSuccess) {
         A->sell_codes = reinterpret_cast<uint16_t*>(A->sell_vals + nval);
         A->sell_lens = reinterpret_cast<uint8_t*>(A->sell_codes + nval);
+        A->sell_pid = reinterpret_cast<uint8_t*>(A->sell_pat + npat);
         xw_tile_kernel<<<s.nblocks, kSpmvBS, 0, st>>>(A->rowptr, A->colind, tiles, A->ncols, W,
                                                       A->xw, d_fail);
         xw_code_kernel<<<s.nblocks, kSpmvBS, 0, st>>>(A->rowptr, A->colind, tiles, A->xw,
-                                                      A->sell_codes, A->sell_lens);
+                                                      A->sell_codes, A->sell_lens, A->sell_pat,
+                                                      A->sell_pid, d_fail + 1);
         sell_scatter_kernel<double, 1><<<s.nblocks, kSpmvBS, 0, st>>>(A->rowptr, A->vals, tiles,
                                                                       A->xw, A->sell_vals);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(&h_fail, d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(h_fail, d_fail, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess || h_fail) {
+    if (e != hipSuccess || h_fail[0]) {
         free_windows(A);
         if (e != hipSuccess)
             return fail(NUM_EHIP, "windowed layout build failed: %s", hipGetErrorString(e));
         return NUM_OK;  // a tile does not fit: no windows
     }
+    if (h_fail[1] || A->pat_mode == 0) free_patterns(A);  // the slots serve instead
     A->sell_w = W;
     return build_sell32(A, st);
 }
@@ -789,15 +853,17 @@ int num_csr_set_windows(num_csr* A, int32_t mode, void* stream)
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
     int rc = check_device(A);
     if (rc) return rc;
-    if (mode < -1 || mode > 1) return fail(NUM_EINVAL, "mode must be -1, 0 or 1, got %d", mode);
-    A->xw_mode = mode;
+    if (mode < -1 || mode > 2)
+        return fail(NUM_EINVAL, "mode must be -1, 0, 1 or 2, got %d", mode);
+    A->xw_mode = mode == 2 ? 1 : mode;
+    A->pat_mode = mode == 2 ? 0 : -1;
     return build_xw(A, (hipStream_t)stream);
 }
 
 int num_csr_window_info(const num_csr* A, int32_t* windowed)
 {
     if (!A || !windowed) return fail(NUM_EINVAL, "matrix or windowed is NULL");
-    *windowed = A->xw ? 1 : 0;
+    *windowed = A->xw ? (A->sell_pat ? 2 : 1) : 0;
     return NUM_OK;
 }
 

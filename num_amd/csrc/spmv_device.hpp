@@ -562,12 +562,16 @@ __device__ __forceinline__ int sell_slot(int k, uint64_t q0, uint64_t q1)
 // tile, a stencil's neighbour values fetched once per tile instead of once
 // per entry.  Entries are summed in index order from 0.0 with separately
 // rounded products: the bits of spmv_tile_run and spmv_d16_run.
-template <int BS, bool NT, class Epi, class Gather>
+// With PAT the row's slots come from its tile's pattern table (one id byte
+// per row, kXwPat) instead of 2 bytes per entry: slot = pattern value -
+// kXwPatBias + lane.
+template <int BS, bool NT, bool PAT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_sell_run(
     const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rec,
     const uint8_t* __restrict__ lens, const uint16_t* __restrict__ code,
+    const uint64_t* __restrict__ pat, const uint8_t* __restrict__ pid,
     const double* __restrict__ vals, const Gather& x, const Epi& epi, double* __restrict__ xw,
-    double& ca, double& cb)
+    uint64_t* __restrict__ lpat, double& ca, double& cb)
 {
     constexpr int NXW = (kXwSlots + BS - 1) / BS;
     static_assert(!has_hook<Gather>::value, "window fill has no hook point");
@@ -588,8 +592,16 @@ __device__ __forceinline__ void spmv_sell_run(
 #pragma unroll
     for (int k = 0; k < kSellW; ++k)
         if (k < width) v[k] = ld_mat<NT>(vals + vb + k * BS + t);
-    uint64_t q0, q1;
-    sell_slots<NT, BS>(code + vb, width, t, q0, q1);
+    uint64_t q0 = 0, q1 = 0;
+    int id = 0;
+    if constexpr (PAT) {
+        // the row's pattern id now; the tile's table goes to LDS with the
+        // windows and is read after the barrier (no dependent global load)
+        id = active ? (int)ld_mat<NT>(pid + r0 + t) : 0;
+        if (t < 2 * kXwPat) lpat[t] = pat[t];
+    } else {
+        sell_slots<NT, BS>(code + vb, width, t, q0, q1);
+    }
     // window slots (clamped to the last one: also branch-free)
     const int sm = max(total - 1, 0);
     double xv_w[NXW];
@@ -605,10 +617,21 @@ __device__ __forceinline__ void spmv_sell_run(
     }
     __syncthreads();
     if (!active) return;
+    if constexpr (PAT) {
+        q0 = lpat[2 * id];
+        q1 = lpat[2 * id + 1];
+    }
     double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < kSellW; ++k)
-        if (k < len) acc += v[k] * xw[sell_slot(k, q0, q1)];
+    for (int k = 0; k < kSellW; ++k) {
+        if constexpr (PAT) {
+            // slot 0 for the padding keeps every LDS address inside the windows
+            const int sl = k < len ? sell_slot(k, q0, q1) - kXwPatBias + t : 0;
+            if (k < len) acc += v[k] * xw[sl];
+        } else {
+            if (k < len) acc += v[k] * xw[sell_slot(k, q0, q1)];
+        }
+    }
     const int row = r0 + t;
     if constexpr (has_own<Epi>::value) {
         // the own rows are a window range of every tile of a square matrix
@@ -692,21 +715,24 @@ __global__ __launch_bounds__(BS) void spmv_d16_kernel(
 
 // The windowed sliced walk (spmv_sell_run): LDS for the kXwSlots window
 // slots only.
-template <int BS, bool NT, class Epi, class Gather>
+template <int BS, bool NT, bool PAT, class Epi, class Gather>
 __global__ __launch_bounds__(BS) void spmv_sell_kernel(
     const int32_t* __restrict__ rec, const uint8_t* __restrict__ lens,
-    const uint16_t* __restrict__ code, const double* __restrict__ vals, Gather x,
+    const uint16_t* __restrict__ code, const uint64_t* __restrict__ pat,
+    const uint8_t* __restrict__ pid, const double* __restrict__ vals, Gather x,
     const TileEntry* __restrict__ sched, int ntiles, int chunk, Epi epi)
 {
     __shared__ double xw[kXwSlots];
     __shared__ double red[2 * BS / kWave];
+    __shared__ uint64_t lpat[PAT ? 2 * kXwPat : 1];
     const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
     if (tile < 0) return;
     if (epi.skip()) return;
     x.prepare();
     double ca, cb;
-    spmv_sell_run<BS, NT, Epi, Gather>(sched[tile], sched[tile + 1], rec + tile * kXwRec, lens,
-                                       code, vals, x, epi, xw, ca, cb);
+    spmv_sell_run<BS, NT, PAT, Epi, Gather>(sched[tile], sched[tile + 1], rec + tile * kXwRec,
+                                            lens, code, pat + (PAT ? tile * 2 * kXwPat : 0),
+                                            pid, vals, x, epi, xw, lpat, ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
         if (threadIdx.x == 0) epi.partial_a[tile] = s;

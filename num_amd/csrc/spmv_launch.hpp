@@ -61,10 +61,17 @@ inline bool launch_sell(const num_csr* A, const Schedule& s, const T* vals, cons
 {
     if constexpr (sizeof(T) == 8 && !has_hook<Gather>::value) {
         if (A->xw && &s == &A->sched && vals == A->vals) {
-            spmv_sell_kernel<kSpmvBS, NT, Epi, Gather>
-                <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
-                    A->xw, A->sell_lens, A->sell_codes, A->sell_vals, x,
-                    static_cast<const TileEntry*>(s.d_tiles), s.nblocks, s.chunk, epi);
+            const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
+            if (A->sell_pat)
+                spmv_sell_kernel<kSpmvBS, NT, true, Epi, Gather>
+                    <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
+                        A->xw, A->sell_lens, A->sell_codes, A->sell_pat, A->sell_pid,
+                        A->sell_vals, x, tiles, s.nblocks, s.chunk, epi);
+            else
+                spmv_sell_kernel<kSpmvBS, NT, false, Epi, Gather>
+                    <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
+                        A->xw, A->sell_lens, A->sell_codes, A->sell_pat, A->sell_pid,
+                        A->sell_vals, x, tiles, s.nblocks, s.chunk, epi);
             return true;
         }
     }

@@ -175,15 +175,19 @@ class CrsMatrix:
     @property
     def index_format(self) -> str:
         """What the FP64 SpMV / CG kernels read for the columns.
-        'window16': the windowed sliced layout (set_windows; every tile reads
-        x from a few column ranges fetched into LDS, each entry a 16-bit slot
-        in them; rows <= 8 entries).  'delta16': the delta-coded column ids
+        'windowpat': the windowed sliced layout with row patterns (every
+        tile reads x from a few column ranges fetched into LDS; each row one
+        id byte into its tile's table of slot patterns; rows <= 8 entries).
+        'window16': the same layout with a 16-bit slot per entry (a tile with
+        more than 16 distinct row patterns, or set_windows(patterns=False)).
+        'delta16': the delta-coded column ids
         (every row <= 32 entries in ascending column order, codes within 16
         bits) when the schedule streams the FP64 matrix from HBM (nt64; a
         matrix that stays in the Infinity Cache reads its int32 columns
         faster).  Else 'int32'."""
-        if self.windowed:
-            return "window16"
+        w = self._window_info()
+        if w:
+            return "windowpat" if w == 2 else "window16"
         if self._d16 is None or not self.schedule_info()["nt64"]:
             return "int32"
         return "delta16"
@@ -229,14 +233,17 @@ class CrsMatrix:
             if not on.value:
                 self._d16 = None
 
-    def set_windows(self, enable: bool | None = True) -> None:
+    def set_windows(self, enable: bool | None = True, patterns: bool = True) -> None:
         """Build (True) or drop (False) the windowed sliced layout
         (num_csr_set_windows): a banded or stencil matrix whose tiles each
         read x from a few contiguous column ranges fetches them once per
         tile into LDS and streams its rows in sliced order.  None restores
-        the default (built for matrices of at least 65,536 rows).  Same
-        results either way (bit-identical)."""
+        the default (built for matrices of at least 65,536 rows).
+        ``patterns=False`` keeps a 16-bit slot per entry instead of the row
+        patterns.  Same results either way (bit-identical)."""
         mode = -1 if enable is None else int(bool(enable))
+        if mode == 1 and not patterns:
+            mode = 2
         call("num_csr_set_windows", self._h, mode, _stream())
 
     def set_mixed_path(self, mode: int = -1) -> None:
@@ -246,12 +253,15 @@ class CrsMatrix:
         matrix's long-range column offset is at least 16,384 rows."""
         call("num_csr_set_mixed_path", self._h, int(mode))
 
+    def _window_info(self) -> int:
+        on = C.c_int32()
+        call("num_csr_window_info", self._h, C.byref(on))
+        return on.value
+
     @property
     def windowed(self) -> bool:
         """True when the FP64 kernels read x through column windows."""
-        on = C.c_int32()
-        call("num_csr_window_info", self._h, C.byref(on))
-        return bool(on.value)
+        return self._window_info() != 0
 
     def schedule_info(self) -> dict:
         v = [C.c_int32() for _ in range(8)]

@@ -67,9 +67,15 @@ def test_spmv_lap3d_192_bitexact(ks, lap192):
     A, x, y_ref = lap192
     assert (A.nrows, A.nnz) == (7_077_888, 49_324_032)
     dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+    assert dA.index_format == "windowpat"
+    _say("spmv 192^3 (windowpat)")
+    y = torch.full((A.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    ks.spmv(dA, _dev(x), y)
+    assert np.array_equal(y.cpu().numpy(), y_ref)
+    dA.set_windows(True, patterns=False)
     assert dA.index_format == "window16"
     _say("spmv 192^3 (window16)")
-    y = torch.full((A.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    y.fill_(float("nan"))
     ks.spmv(dA, _dev(x), y)
     assert np.array_equal(y.cpu().numpy(), y_ref)
     # the delta-coded and int32-column kernels give the same bits

@@ -68,19 +68,47 @@ CASES = {
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("nt64", [1, 0])
 def test_spmv_windows_bit_identical(ks, orc, name, nt64):
+    # row patterns (every case holds at most 16 per tile), a slot per entry,
+    # no windows: the same bits as the oracle's row loop
     A = CASES[name]()
     dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
     dA.set_schedule(nt64=nt64)
-    assert dA.index_format == "window16"
+    assert dA.index_format == "windowpat"
     x = np.random.default_rng(7).uniform(-1, 1, A.ncols)
     y = _spmv(ks, dA, x)
     assert np.array_equal(y, orc.spmv(A.rowptr, A.colind, A.values, x))
+    dA.set_windows(True, patterns=False)
+    assert dA.index_format == "window16"
+    assert np.array_equal(_spmv(ks, dA, x), y)
     dA.set_windows(False)
     assert not dA.windowed
     assert np.array_equal(_spmv(ks, dA, x), y)
     dA.set_windows(True)
-    assert dA.windowed
+    assert dA.index_format == "windowpat"
     assert np.array_equal(_spmv(ks, dA, x), y)
+
+
+def test_row_patterns_fall_back_to_slots(ks, orc):
+    # 300 rows: the diagonal and one neighbour at a random offset in
+    # +-1..+-12 (25 column offsets: the windows still fit), so a tile holds
+    # up to 24 distinct row patterns, more than 16: the windows keep a slot
+    # per entry; still bit-identical
+    rng = np.random.default_rng(5)
+    n = 300
+    offs = np.concatenate([-np.arange(1, 13), np.arange(1, 13)])
+    rows = []
+    for r in range(n):
+        d = int(rng.choice(offs))
+        c = min(max(r + d, 0), n - 1)
+        rows.append(sorted({r, c}))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    va = rng.uniform(-1, 1, ci.size)
+    dA = _up(ks, rp, ci, va, n, n)
+    assert dA.index_format == "window16"
+    x = rng.uniform(-1, 1, n)
+    assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp, ci, va, x))
 
 
 @pytest.mark.parametrize("w", range(1, 9))
@@ -184,22 +212,26 @@ def test_windows_survive_reschedule(ks, orc):
 @pytest.mark.parametrize("nt64", [1, 0])
 def test_cg_windows_bit_identical(ks, name, nt64):
     # the windowed CG kernel reads p_new of its own rows from the windows:
-    # same bits as the delta-coded / int32 kernels, which compute it again
+    # same bits as the delta-coded / int32 kernels, which compute it again,
+    # with row patterns or a slot per entry
     A = CASES[name]()
     b = _dev(np.random.default_rng(12).uniform(-1, 1, A.nrows))
     out = []
-    for on in (True, False):
+    for on, pat in ((True, True), (True, False), (False, True)):
         dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
-        dA.set_windows(on)
+        dA.set_windows(on, patterns=pat)
         dA.set_schedule(nt64=nt64)
         assert dA.windowed == on
+        if on:
+            assert dA.index_format == ("windowpat" if pat else "window16")
         x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
         res = ks.cg_solve_local(dA, b, x, tol=1e-10, max_iter=2000)
         out.append((res, x.cpu().numpy()))
-    (r0, x0), (r1, x1) = out
-    assert r0["converged"] and r0["iters"] == r1["iters"]
-    assert r0["residual"] == r1["residual"]
-    assert np.array_equal(x0, x1)
+    (r0, x0) = out[0]
+    assert r0["converged"]
+    for r1, x1 in out[1:]:
+        assert r0["iters"] == r1["iters"] and r0["residual"] == r1["residual"]
+        assert np.array_equal(x0, x1)
 
 
 def _rel(a, b):
@@ -251,6 +283,6 @@ def test_windows_default_from_65536_rows(ks):
     assert not ds.windowed and dl.windowed
     from num_amd._lib import call
     with pytest.raises(ValueError):
-        call("num_csr_set_windows", dl.handle, 2, None)
+        call("num_csr_set_windows", dl.handle, 3, None)
     with pytest.raises(ValueError):
         dl.set_mixed_path(2)
