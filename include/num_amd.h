@@ -153,12 +153,20 @@ int num_csr_set_windows(num_csr* A, int32_t mode, void* stream);
 int num_csr_window_info(const num_csr* A, int32_t* windowed);
 /* Which kernel takes the FP32 steps of num_cg_solve_mixed (no reference
  * counterpart): mode 1 the windowed sliced step (when the windowed layout
- * and the FP32 values are attached), 0 the staged row-tile / pair kernels,
- * -1 (the default) the windowed step when the FP32 copy streams from HBM and
+ * and the FP32 values are attached), 0 the staged row-tile / pair kernels
+ * (the pair kernel reading the int16 offsets' row patterns where they were
+ * built: one id byte per row into a table of at most 16 offset patterns per
+ * pair of tiles), 2 the staged kernels with 2 offset bytes per entry, -1
+ * (the default) the windowed step when the FP32 copy streams from HBM and
  * the matrix's dominant long-range column offset is at least 16,384 (a 3-D
  * grid plane of that many rows; on 2-D and short-plane grids the staged
- * pair kernel is faster).  Never changes which layout is built. */
+ * pair kernel is faster), else as 0.  Never changes which layout is built;
+ * every mode gives the same bits as another mode on the same kernel. */
 int num_csr_set_mixed_path(num_csr* A, int32_t mode);
+/* *windows = 1 if the windowed layout reads row patterns, *offsets = 1 if the
+ * int16 column offsets have their row-pattern form (no reference
+ * counterpart; either pointer may be NULL). */
+int num_csr_pattern_info(const num_csr* A, int32_t* windows, int32_t* offsets);
 /* Fails with NUM_ERUNTIME (and frees nothing) while an engine system admitted
  * with A (num_engine_add) has not been removed. */
 int num_csr_destroy(num_csr* A);

@@ -46,6 +46,7 @@ SIGNATURES = {
     "num_csr_set_windows": (C.c_int, [vp, i32, vp]),
     "num_csr_window_info": (C.c_int, [vp, pi32]),
     "num_csr_set_mixed_path": (C.c_int, [vp, i32]),
+    "num_csr_pattern_info": (C.c_int, [vp, pi32, pi32]),
     "num_csr_attach_xcd": (C.c_int, [vp, vp, i32, vp, vp, vp]),
     "num_spmv_xcd": (C.c_int, [vp, vp, vp, vp, vp]),
     "num_csr_destroy": (C.c_int, [vp]),

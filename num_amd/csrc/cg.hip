@@ -759,25 +759,34 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_spmv_kernel(
 // both tiles are staged at once (FP32 values + int16 offsets fill the LDS of
 // one FP64 tile); FP64 refinement: the two tiles in turn.  Partials: one per
 // pair.
-template <int NPT, bool NT32, bool NT64>
+// PAT: the offsets come as row patterns (rel_pat / rel_pid: one id byte per
+// row into the pair's table) instead of 2 bytes per entry; same bits.
+template <int NPT, bool NT32, bool NT64, bool PAT>
 __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const double* __restrict__ v64, const float* __restrict__ v32,
-    const int16_t* __restrict__ rel, const TileEntry* __restrict__ sched, int ntiles,
+    const int16_t* __restrict__ rel, const uint64_t* __restrict__ rpat,
+    const uint8_t* __restrict__ rpid, const TileEntry* __restrict__ sched, int ntiles,
     int npairs, int chunk, MixArgs a)
 {
     __shared__ double lv[kSpmvBS * NPT];   // FP32 pairs: 2 * NPT * BS floats
     __shared__ int32_t lc[kSpmvBS * NPT];  // FP32 pairs: 2 * NPT * BS int16
     __shared__ double red[2 * kSpmvBS / kWave];
+    __shared__ uint64_t lpat[PAT ? 2 * kRelPat : 1];
     const int p = chunk_tile(blockIdx.x, npairs, chunk);
     if (p < 0) return;
     const int t0 = 2 * p, t1 = min(2 * p + 2, ntiles);
     GatherMix g{a.r32, a.p0, a.p1, a.sa, a.sb, a.pb, a.npb};
     EpiPq epi{a.p0, a.p1, a.q, nullptr, nullptr};
     double ca, cb;
-    spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
-                                          reinterpret_cast<float*>(lv),
-                                          reinterpret_cast<int16_t*>(lc), ca, cb);
+    if constexpr (PAT)
+        spmv_relpat_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rpid,
+                                               rpat + (size_t)p * 2 * kRelPat, v32, g, epi,
+                                               reinterpret_cast<float*>(lv), lpat, ca, cb);
+    else
+        spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
+                                              reinterpret_cast<float*>(lv),
+                                              reinterpret_cast<int16_t*>(lc), ca, cb);
     if (g.done) return;
     if (g.ru) {
         mix_refine_tiles<NPT, NT64>(t0, t1, p, rowptr, colind, v64, sched, a, g.xsel, lv, lc,
@@ -1102,24 +1111,27 @@ static void launch_mix_pair_npt(const num_csr* A, const MixArgs& a, hipStream_t 
     const int chunk = pc >= 2 && npairs >= 8 * pc ? pc : tile_chunk(npairs);
     const int grid = chunk_grid(npairs, chunk);
     const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
-    if (s.nt32) {
-        if (s.nt64)
-            mix_pair_kernel<NPT, true, true><<<grid, kSpmvBS, 0, st>>>(
-                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
-                chunk, a);
-        else
-            mix_pair_kernel<NPT, true, false><<<grid, kSpmvBS, 0, st>>>(
-                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
-                chunk, a);
+    const bool pat = A->rel_pat && A->mix_path != 2;
+    auto go = [&](auto k) {
+        k<<<grid, kSpmvBS, 0, st>>>(A->rowptr, A->colind, A->vals, A->vals32, A->rel16,
+                                    A->rel_pat, A->rel_pid, tiles, s.nblocks, npairs, chunk, a);
+    };
+    if (pat) {
+        if (s.nt32) {
+            if (s.nt64) go(mix_pair_kernel<NPT, true, true, true>);
+            else go(mix_pair_kernel<NPT, true, false, true>);
+        } else {
+            if (s.nt64) go(mix_pair_kernel<NPT, false, true, true>);
+            else go(mix_pair_kernel<NPT, false, false, true>);
+        }
     } else {
-        if (s.nt64)
-            mix_pair_kernel<NPT, false, true><<<grid, kSpmvBS, 0, st>>>(
-                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
-                chunk, a);
-        else
-            mix_pair_kernel<NPT, false, false><<<grid, kSpmvBS, 0, st>>>(
-                A->rowptr, A->colind, A->vals, A->vals32, A->rel16, tiles, s.nblocks, npairs,
-                chunk, a);
+        if (s.nt32) {
+            if (s.nt64) go(mix_pair_kernel<NPT, true, true, false>);
+            else go(mix_pair_kernel<NPT, true, false, false>);
+        } else {
+            if (s.nt64) go(mix_pair_kernel<NPT, false, true, false>);
+            else go(mix_pair_kernel<NPT, false, false, false>);
+        }
     }
 }
 
@@ -1145,7 +1157,7 @@ constexpr int32_t kMixSellOffset = 16384;
 static bool mix_sliced(const num_csr* A)
 {
     if (!A->xw || !A->sell_vals32) return false;
-    if (A->mix_path >= 0) return A->mix_path == 1;
+    if (A->mix_path >= 0) return A->mix_path == 1;  // 0 and 2: the staged kernels
     return A->sched.nt32 != 0 && A->offset_d >= kMixSellOffset;
 }
 
@@ -1258,7 +1270,8 @@ int num_cg_solve(const num_csr* A, const double* d_b, double* d_x, double tol, i
 int num_csr_set_mixed_path(num_csr* A, int32_t mode)
 {
     if (!A) return fail(NUM_EINVAL, "matrix is NULL");
-    if (mode < -1 || mode > 1) return fail(NUM_EINVAL, "mode must be -1, 0 or 1, got %d", mode);
+    if (mode < -1 || mode > 2)
+        return fail(NUM_EINVAL, "mode must be -1, 0, 1 or 2, got %d", mode);
     A->mix_path = mode;
     return NUM_OK;
 }

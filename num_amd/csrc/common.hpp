@@ -100,6 +100,12 @@ constexpr int kXwRec = 2 * kXwWin + 2;
 // per row replaces the row's 2-byte slots.
 constexpr int kXwPat = 16;
 constexpr int kXwPatBias = 256;
+// Row patterns of the int16 column offsets (num_csr_attach_rel16): a row's
+// offsets column - row (at most 8, zeros past its length, as uint16) per
+// pair of schedule tiles, at most kRelPat distinct patterns per pair, one id
+// byte per row (the FP32 pair kernel reads them instead of 2 bytes per entry).
+constexpr int kRelPat = 16;
+constexpr int kRelPatW = 8;
 
 // Position of entry k of lane i (of bs lanes) in a tile's packed region of
 // w slices, elements of which `pack` (4 for slots, 2 for FP32 values) share

@@ -48,6 +48,10 @@ struct num_csr {
     const double* vals = nullptr;
     const float* vals32 = nullptr;
     const int16_t* rel16 = nullptr;     // column - row as int16 (FP32 path), if attached
+    // Row-pattern form of rel16 per pair of schedule tiles (kRelPat), owned:
+    // 2 * kRelPat uint64 per pair, then one pattern id per row.
+    uint64_t* rel_pat = nullptr;
+    uint8_t* rel_pid = nullptr;
     // Delta-coded column ids (FP64 path on the whole-matrix schedule), if
     // attached: see num_csr_attach_d16.  d16_buf is the caller's buffer
     // (nnz + 8 codes, then one length byte per row: d16_lens), kept so a

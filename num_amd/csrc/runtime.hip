@@ -306,6 +306,74 @@ __global__ void rel16_kernel(int32_t nrows, const int32_t* __restrict__ rowptr,
     if (flag) atomicOr(bad, 1);
 }
 
+// Row patterns of the int16 offsets, one workgroup per pair of schedule
+// tiles (at most 2 * 256 rows, two per thread): each row's offsets column -
+// row (at most kRelPatW, zeros past its length) packed in two uint64; the
+// pair's distinct patterns numbered in order of their first row, in rounds
+// (the first row without an id leads, every row with its pattern takes the
+// next id); the rows' id bytes and the pair's table written.  *bad = 1 if a
+// row holds more than kRelPatW entries or the pair more than kRelPat
+// patterns.
+__global__ __launch_bounds__(256) void relpat_kernel(const int32_t* __restrict__ rowptr,
+                                                     const int16_t* __restrict__ rel,
+                                                     const TileEntry* __restrict__ tiles,
+                                                     int32_t ntiles, uint64_t* __restrict__ pat,
+                                                     uint8_t* __restrict__ pid,
+                                                     int32_t* __restrict__ bad)
+{
+    __shared__ unsigned long long s_q[4 * 256];
+    __shared__ unsigned long long s_tab[2 * kRelPat];
+    __shared__ int s_lead;
+    const int p = blockIdx.x, t = threadIdx.x;
+    const int r0 = tiles[2 * p].row, r1 = tiles[min(2 * p + 2, ntiles)].row;
+    uint64_t q[2][2] = {{0, 0}, {0, 0}};
+    int myid[2] = {-1, -1};
+    bool act[2];
+    int over = 0;
+    for (int h = 0; h < 2; ++h) {
+        const int lr = t + h * 256, r = r0 + lr;
+        act[h] = r < r1;
+        if (act[h]) {
+            const int js = rowptr[r], len = rowptr[r + 1] - js;
+            if (len > kRelPatW) over = 1;
+            for (int k = 0; k < min(len, kRelPatW); ++k) {
+                const uint64_t d = (uint64_t)(uint16_t)rel[js + k];
+                if (k < 4) q[h][0] |= d << (16 * k);
+                else q[h][1] |= d << (16 * (k - 4));
+            }
+        }
+        s_q[2 * lr] = q[h][0];
+        s_q[2 * lr + 1] = q[h][1];
+    }
+    if (over) atomicOr(bad, 1);  // the host drops the table; the ids are not used
+    int np = 0;
+    for (;;) {
+        if (t == 0) s_lead = INT_MAX;
+        __syncthreads();
+        for (int h = 0; h < 2; ++h)
+            if (act[h] && myid[h] < 0) atomicMin(&s_lead, t + h * 256);
+        __syncthreads();
+        const int lead = s_lead;
+        if (lead == INT_MAX) break;
+        if (np == kRelPat) {
+            if (t == 0) atomicOr(bad, 1);
+            return;  // uniform: every thread saw the same lead and np
+        }
+        const unsigned long long a = s_q[2 * lead], c = s_q[2 * lead + 1];
+        for (int h = 0; h < 2; ++h)
+            if (act[h] && myid[h] < 0 && q[h][0] == a && q[h][1] == c) myid[h] = np;
+        if (t == 0) {
+            s_tab[2 * np] = a;
+            s_tab[2 * np + 1] = c;
+        }
+        ++np;
+        __syncthreads();  // s_lead is reset only after every thread read it
+    }
+    for (int h = 0; h < 2; ++h)
+        if (act[h]) pid[r0 + t + h * 256] = (uint8_t)myid[h];
+    if (t < 2 * kRelPat) pat[(size_t)p * 2 * kRelPat + t] = t < 2 * np ? s_tab[t] : 0ull;
+}
+
 // Delta codes, one workgroup per tile (rows_per_block <= blockDim): the
 // tile's base column is the smallest first column of its rows; each row's
 // first entry is coded against the base, every later entry against the
@@ -366,6 +434,51 @@ static int build_d16(num_csr* A, hipStream_t st)
     (void)hipFree(d_bad);
     if (e != hipSuccess) return fail(NUM_EHIP, "delta-code build failed: %s", hipGetErrorString(e));
     if (h_bad == 0) A->d16 = A->d16_buf;
+    return NUM_OK;
+}
+
+static void free_relpat(num_csr* A)
+{
+    // rel_pid lives in the rel_pat allocation
+    if (A->rel_pat) (void)hipFree(A->rel_pat);
+    A->rel_pat = nullptr;
+    A->rel_pid = nullptr;
+}
+
+// (Re)builds the row patterns of A->rel16 over the pairs of A->sched's tiles
+// (relpat_kernel); nothing is kept when a row is longer than kRelPatW, a
+// pair holds more than kRelPat patterns, or a tile has more than 256 rows.
+// Synchronises `st`.
+static int build_relpat(num_csr* A, hipStream_t st)
+{
+    free_relpat(A);
+    const Schedule& s = A->sched;
+    if (!A->rel16 || A->max_row_nnz > kRelPatW || s.nblocks == 0 ||
+        s.rows_per_block > 256 || s.long_rows != 0)
+        return NUM_OK;
+    const int32_t npairs = (s.nblocks + 1) / 2;
+    const size_t npat = (size_t)npairs * 2 * kRelPat;
+    int32_t* d_bad = nullptr;
+    hipError_t e = hipMalloc(&d_bad, sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(int32_t), st);
+    if (e == hipSuccess) e = hipMalloc(&A->rel_pat, npat * sizeof(uint64_t) + A->nrows);
+    if (e == hipSuccess) {
+        A->rel_pid = reinterpret_cast<uint8_t*>(A->rel_pat + npat);
+        relpat_kernel<<<npairs, 256, 0, st>>>(A->rowptr, A->rel16,
+                                              static_cast<const TileEntry*>(s.d_tiles),
+                                              s.nblocks, A->rel_pat, A->rel_pid, d_bad);
+        e = hipGetLastError();
+    }
+    int32_t h_bad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h_bad, d_bad, sizeof(int32_t),
+                                            hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_bad) (void)hipFree(d_bad);
+    if (e != hipSuccess || h_bad) {
+        free_relpat(A);
+        if (e != hipSuccess)
+            return fail(NUM_EHIP, "offset pattern build failed: %s", hipGetErrorString(e));
+    }
     return NUM_OK;
 }
 
@@ -822,6 +935,7 @@ int num_csr_attach_rel16(num_csr* A, int16_t* d_rel16, void* stream, int32_t* at
     if (h_bad == 0) {
         A->rel16 = d_rel16;
         *attached = 1;
+        return build_relpat(A, st);
     }
     return NUM_OK;
 }
@@ -864,6 +978,14 @@ int num_csr_window_info(const num_csr* A, int32_t* windowed)
 {
     if (!A || !windowed) return fail(NUM_EINVAL, "matrix or windowed is NULL");
     *windowed = A->xw ? (A->sell_pat ? 2 : 1) : 0;
+    return NUM_OK;
+}
+
+int num_csr_pattern_info(const num_csr* A, int32_t* windows, int32_t* offsets)
+{
+    if (!A) return fail(NUM_EINVAL, "matrix is NULL");
+    if (windows) *windows = A->sell_pat ? 1 : 0;
+    if (offsets) *offsets = A->rel_pat ? 1 : 0;
     return NUM_OK;
 }
 
@@ -953,6 +1075,7 @@ int num_csr_destroy(num_csr* A)
                     (int)A->engine_holds.load());
     if (A->xcd_tiles) (void)hipFree(A->xcd_tiles);
     free_windows(A);
+    free_relpat(A);
     free_schedule(A->xcd_sched);
     free_schedule(A->sched);
     for (auto& kv : A->range_sched) free_schedule(kv.second);
@@ -991,6 +1114,8 @@ int num_csr_set_schedule(num_csr* A, int32_t rows_per_block, int32_t nnz_per_blo
     if (rc) return rc;
     if (!A->d16) A->d16_buf = nullptr;
     rc = build_xw(A, nullptr);  // and so do the windows
+    if (rc) return rc;
+    rc = build_relpat(A, nullptr);  // and the offset patterns (per tile pair)
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(A->mu);
     for (auto& kv : A->range_sched) free_schedule(kv.second);

@@ -343,6 +343,80 @@ __device__ __forceinline__ void spmv_rel16_run(
     }
 }
 
+// spmv_rel16_run with the offsets' row patterns (num_csr_attach_rel16,
+// kRelPat): only the values are staged; each row takes its pattern id (one
+// byte) and, after the stage barrier, its offsets from the pair's table in
+// LDS (`lpat`, staged with the values).  Same entries, same order, same
+// bits as spmv_rel16_run.  Rows hold at most kRelPatW entries.
+template <int BS, int NPT, int RPT, bool NT, class Epi, class Gather>
+__device__ __forceinline__ void spmv_relpat_run(
+    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
+    const uint8_t* __restrict__ pid, const uint64_t* __restrict__ pat,
+    const float* __restrict__ vals, const Gather& x, const Epi& epi, float* __restrict__ lv,
+    uint64_t* __restrict__ lpat, double& ca, double& cb)
+{
+    constexpr int M = NPT * RPT;
+    const int t = threadIdx.x;
+    const int r0 = e0.row, j0 = e0.nz0;
+    const int r1 = e1.row, j1 = e1.nz0;
+    const int nz = j1 - j0, nr = r1 - r0;
+    int rb[RPT], re[RPT], id[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int lr = min(t + k * BS, nr);
+        rb[k] = rowptr[r0 + lr];
+        re[k] = rowptr[r0 + lr + (t + k * BS < nr ? 1 : 0)];
+        id[k] = t + k * BS < nr ? (int)pid[r0 + t + k * BS] : 0;
+    }
+    uint64_t tab = 0;
+    if (t < 2 * kRelPat) tab = pat[t];
+    if (nz > 0) {
+        const int last = j1 - 1;
+        float v[M];
+#pragma unroll
+        for (int s = 0; s < M; ++s) v[s] = ld_mat<NT>(vals + min(j0 + t + s * BS, last));
+        if constexpr (has_hook<Gather>::value) x.issue();
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const int k = t + s * BS;
+            if (k < nz) lv[k] = v[s];
+        }
+    } else if constexpr (has_hook<Gather>::value) {
+        x.issue();
+    }
+    if (t < 2 * kRelPat) lpat[t] = tab;
+    __syncthreads();
+    ca = 0.0;
+    cb = 0.0;
+    if constexpr (has_hook<Gather>::value) {
+        if (!x.resolve()) return;
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int lr = t + k * BS;
+        if (lr >= nr) continue;
+        const int row = r0 + lr;
+        const int b = rb[k] - j0, len = re[k] - rb[k];
+        const uint64_t q0 = lpat[2 * id[k]], q1 = lpat[2 * id[k] + 1];
+        float vv[kRelPatW], xv[kRelPatW];
+#pragma unroll
+        for (int u = 0; u < kRelPatW; ++u) {
+            const bool ok = u < len;
+            vv[u] = lv[ok ? b + u : 0];  // slot 0 is always inside the LDS array
+            const int off = (int)(int16_t)(uint16_t)(((u < 4 ? q0 : q1) >> (16 * (u & 3))) & 0xffff);
+            xv[u] = x(row + (ok ? off : 0));  // zero offset past the row: its own column
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kRelPatW; ++u)
+            if (u < len) acc += vv[u] * xv[u];
+        double ta = 0.0, tb = 0.0;
+        epi(row, acc, ta, tb, x);
+        ca += ta;
+        cb += tb;
+    }
+}
+
 // FP64 walk with delta-coded column ids (num_csr_attach_d16): a row's first
 // code is its column minus the tile's base column (TileEntry.base), every
 // later code its column minus the previous entry's, as uint16.  Half the

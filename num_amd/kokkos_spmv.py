@@ -248,10 +248,21 @@ class CrsMatrix:
 
     def set_mixed_path(self, mode: int = -1) -> None:
         """Kernel of cg_solve_mixed's FP32 steps (num_csr_set_mixed_path):
-        1 the windowed sliced step, 0 the staged pair / row-tile kernels, -1
-        (default) windowed when the FP32 copy streams from HBM and the
-        matrix's long-range column offset is at least 16,384 rows."""
+        1 the windowed sliced step, 0 the staged pair / row-tile kernels (the
+        pair kernel with the int16 offsets' row patterns where built), 2 the
+        staged kernels with 2 offset bytes per entry, -1 (default) windowed
+        when the FP32 copy streams from HBM and the matrix's long-range
+        column offset is at least 16,384 rows, else as 0."""
         call("num_csr_set_mixed_path", self._h, int(mode))
+
+    @property
+    def patterns(self) -> dict:
+        """Which row-pattern forms are built (num_csr_pattern_info): 'windows'
+        for the windowed layout, 'offsets' for the FP32 pair kernel's int16
+        column offsets."""
+        w, o = C.c_int32(), C.c_int32()
+        call("num_csr_pattern_info", self._h, C.byref(w), C.byref(o))
+        return {"windows": bool(w.value), "offsets": bool(o.value)}
 
     def _window_info(self) -> int:
         on = C.c_int32()

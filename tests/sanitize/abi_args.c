@@ -83,6 +83,7 @@ int main(void)
     expect(num_csr_set_windows(NULL, 1, NULL), NUM_EINVAL, "set_windows: NULL");
     expect(num_csr_window_info(NULL, NULL), NUM_EINVAL, "window_info: NULL");
     expect(num_csr_set_mixed_path(NULL, 0), NUM_EINVAL, "set_mixed_path: NULL");
+    expect(num_csr_pattern_info(NULL, NULL, NULL), NUM_EINVAL, "pattern_info: NULL");
     expect(num_spmv(NULL, x, y, NULL), NUM_EINVAL, "spmv: NULL matrix");
     expect(num_batch_spmv(NULL, X, Y, 2, NULL), NUM_EINVAL, "batch_spmv: NULL matrix");
     expect64(num_cg_workspace_bytes(NULL), -1, "cg workspace: NULL");

@@ -227,6 +227,14 @@ def test_mixed_paths(ks, orc, case):
     assert rel(xh, x64) < 1e-7
     # the FP32 steps stay close to the FP64 iteration count
     assert r["inner_total"] <= 1.5 * r64["iters"] + 10
+    if case != "hub":
+        # the pair kernel read the offsets' row patterns; with 2 offset bytes
+        # per entry (mixed path 2) it must give the same bits
+        assert dA.patterns["offsets"]
+        dA.set_mixed_path(2)
+        x2 = _dev(np.zeros(A.nrows))
+        r2 = ks.cg_solve_mixed(dA, _dev(b), x2, tol=1e-10)
+        assert r2 == r and np.array_equal(x2.cpu().numpy(), xh)
 
 
 @pytest.mark.parametrize("mixed", [False, True])

@@ -73,7 +73,7 @@ def test_spmv_windows_bit_identical(ks, orc, name, nt64):
     A = CASES[name]()
     dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
     dA.set_schedule(nt64=nt64)
-    assert dA.index_format == "windowpat"
+    assert dA.index_format == "windowpat" and dA.patterns["windows"]
     x = np.random.default_rng(7).uniform(-1, 1, A.ncols)
     y = _spmv(ks, dA, x)
     assert np.array_equal(y, orc.spmv(A.rowptr, A.colind, A.values, x))
@@ -285,4 +285,4 @@ def test_windows_default_from_65536_rows(ks):
     with pytest.raises(ValueError):
         call("num_csr_set_windows", dl.handle, 3, None)
     with pytest.raises(ValueError):
-        dl.set_mixed_path(2)
+        dl.set_mixed_path(3)
