@@ -43,7 +43,8 @@ def _al256(b: int) -> int:
 def stencil_layout(dims, mixed: bool) -> dict:
     """Byte offsets inside an assembled stencil system's buffer (the layout
     num_engine_add_stencil carves): x, b, rowptr, colind, values[, values32,
-    rel16 (int16 column - row)], workspace."""
+    rpid (a row's pattern id: its neighbour bits, uint8), rpat (64 patterns
+    of 8 int16 column - row offsets)], workspace."""
     from .generators import stencil_nnz
     n = 1
     for d in dims:
@@ -52,7 +53,7 @@ def stencil_layout(dims, mixed: bool) -> dict:
     off, o = {}, 0
     for name, size in (("x", 8 * n), ("b", 8 * n), ("rowptr", 4 * (n + 1)),
                        ("colind", 4 * nnz), ("values", 8 * nnz)) + \
-            ((("values32", 4 * nnz), ("rel16", 2 * nnz)) if mixed else ()):
+            ((("values32", 4 * nnz), ("rpid", n), ("rpat", 64 * 16)) if mixed else ()):
         off[name] = o
         o += _al256(size)
     off["work"] = o

@@ -780,9 +780,10 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     EpiPq epi{a.p0, a.p1, a.q, nullptr, nullptr};
     double ca, cb;
     if constexpr (PAT)
-        spmv_relpat_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rpid,
-                                               rpat + (size_t)p * 2 * kRelPat, v32, g, epi,
-                                               reinterpret_cast<float*>(lv), lpat, ca, cb);
+        spmv_relpat_run<kSpmvBS, NPT, 2, NT32, 2 * kRelPat>(sched[t0], sched[t1], rowptr, rpid,
+                                                            rpat + (size_t)p * 2 * kRelPat, v32,
+                                                            g, epi, reinterpret_cast<float*>(lv),
+                                                            lpat, ca, cb);
     else
         spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
                                               reinterpret_cast<float*>(lv),

@@ -63,11 +63,17 @@ constexpr int kEngNpt = 7;        // LDS entries per thread of the engine's tile
 constexpr int kChunkRows = 2048;  // rows per KC workgroup (256 threads x 8), at least
 constexpr int kMaxChunks = 256;   // chunks per system, at most (each sums the tile partials)
 // Rows per tile of an assembled stencil (<= 7 entries per row): 256, or 512
-// in the mixed form, whose FP32 steps stage a 512-row tile at once (FP32
-// values and int16 column offsets fill the LDS of one 256-row FP64 tile) and
-// whose FP64 phases walk it as two 256-row halves.
+// in the mixed form, whose FP32 steps stage a 512-row tile at once (its FP32
+// values fill the LDS of one 256-row FP64 tile) and whose FP64 phases walk it
+// as two 256-row halves.
 constexpr int kStencilRpt = 256;
 constexpr int kStencilRptPair = 512;
+// Row patterns of the mixed form's FP32 steps: a row's column offsets are
+// fixed by which of its six grid neighbours exist, so a row carries one id
+// byte (bit 0: -x, 1: +x, 2: -y, 3: +y, 4: -z, 5: +z neighbour present) into
+// the system's table of 64 patterns, each kRelPatW int16 offsets in column
+// order (-sz, -sy, -1, 0, +1, +sy, +sz for those present; 0 past the row).
+constexpr int kStencilPat = 64;
 
 struct SysDesc {
     const int32_t* rowptr;
@@ -88,7 +94,8 @@ struct SysDesc {
     float* q32;
     float* p32a;
     float* p32b;
-    const int16_t* rel16;  // column - row (FP32 phases), with 512-row tiles; or nullptr
+    const uint8_t* rpid;   // mixed stencil: row pattern ids (FP32 phases), 512-row tiles
+    const uint64_t* rpat;  // and the table of kStencilPat patterns; or nullptr
     int32_t n, ntiles, mode, rpt;
     int32_t crows;         // rows per update-kernel chunk
     int32_t pad_;
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(kSpmvBS, 7) void eng_ka(const BTile* __restrict__ b
                                                   double* __restrict__ pb)
 {
     __shared__ double lv[kSpmvBS * kEngNpt];   // FP32 pairs: 2 * NPT * BS floats
-    __shared__ int32_t lc[kSpmvBS * kEngNpt];  // FP32 pairs: 2 * NPT * BS int16
+    __shared__ __align__(8) int32_t lc[kSpmvBS * kEngNpt];  // FP32 pairs: the pattern table
     __shared__ double red[2 * kSpmvBS / kWave];
     const int b = chunk_tile(blockIdx.x, nbt, chunk);
     if (b < 0) return;
@@ -229,15 +236,15 @@ __global__ __launch_bounds__(kSpmvBS, 7) void eng_ka(const BTile* __restrict__ b
     const TileEntry e0 = {e.r0, e.wide, e.nz0, 0};
     const TileEntry e1 = {e.r1, kWalkShort, e.nz1, 0};
     double ca = 0.0, cb = 0.0;
-    if (d->rel16) {
+    if (d->rpat) {
         // 512-row tile of a mixed stencil system
         if (ph == PH_RUN32) {
             const int ps = st->psel;
             EGatherPNew<float> g{d->r32, ps ? d->p32b : d->p32a, (float)st->beta};
             EEpiStep<float> epi{ps ? d->p32a : d->p32b, d->q32};
-            spmv_rel16_run<kSpmvBS, kEngNpt, 2, NT32>(e0, e1, d->rowptr, d->rel16, d->vals32, g,
-                                                      epi, reinterpret_cast<float*>(lv),
-                                                      reinterpret_cast<int16_t*>(lc), ca, cb);
+            spmv_relpat_run<kSpmvBS, kEngNpt, 2, NT32, 2 * kStencilPat>(
+                e0, e1, d->rowptr, d->rpid, d->rpat, d->vals32, g, epi,
+                reinterpret_cast<float*>(lv), reinterpret_cast<uint64_t*>(lc), ca, cb);
         } else {
             // FP64 refinement (the only other phase of a mixed system): the two
             // 256-row halves in turn
@@ -706,21 +713,41 @@ __device__ __forceinline__ int64_t stencil_rowptr_at(int ndim, int nx, int ny, i
 
 // One thread per row: rowptr, columns (-z, -y, -x, self, +x, +y, +z), values
 // (-1 off the diagonal, 2*ndim + shift on it), b, x = 0 and, for the mixed
-// form, the FP32 copy, the int16 column offsets and the zeroed x32a / p32a.
+// form, the FP32 copy, the row pattern ids and table (kStencilPat) and the
+// zeroed x32a / p32a.  Unused grid dimensions are 1, so their neighbour bits
+// are never set.  Offsets fit int16: lines and planes hold <= 32767 points.
 __global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double shift,
                                        uint64_t seed, int32_t* __restrict__ rowptr,
                                        int32_t* __restrict__ colind, double* __restrict__ vals,
-                                       float* __restrict__ vals32, int16_t* __restrict__ rel16,
-                                       double* __restrict__ b, double* __restrict__ x,
-                                       float* __restrict__ z32a, float* __restrict__ z32b)
+                                       float* __restrict__ vals32, uint8_t* __restrict__ rpid,
+                                       uint64_t* __restrict__ rpat, double* __restrict__ b,
+                                       double* __restrict__ x, float* __restrict__ z32a,
+                                       float* __restrict__ z32b)
 {
     const int64_t n = (int64_t)nx * ny * nz;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int sy = nx, sz = nx * ny;
+    if (rpat && i < kStencilPat) {
+        // pattern i: the offsets of a row whose neighbour bits are i
+        const int f = (int)i;
+        const int off[7] = {-sz, -sy, -1, 0, 1, sy, sz};
+        const int bit[7] = {16, 4, 1, 0, 2, 8, 32};
+        uint64_t q[2] = {0, 0};
+        int u = 0;
+        for (int k = 0; k < 7; ++k) {
+            if (bit[k] && !(f & bit[k])) continue;
+            q[u >> 2] |= (uint64_t)(uint16_t)(int16_t)off[k] << (16 * (u & 3));
+            ++u;
+        }
+        rpat[2 * f] = q[0];
+        rpat[2 * f + 1] = q[1];
+    }
     if (i >= n) return;
     const int ix = (int)(i % nx);
     const int iy = (int)((i / nx) % ny);
     const int iz = (int)(i / ((int64_t)nx * ny));
-    const int sy = nx, sz = nx * ny;
+    const int f = (ix > 0 ? 1 : 0) | (ix < nx - 1 ? 2 : 0) | (iy > 0 ? 4 : 0) |
+                  (iy < ny - 1 ? 8 : 0) | (iz > 0 ? 16 : 0) | (iz < nz - 1 ? 32 : 0);
     int p = (int)stencil_rowptr_at(ndim, nx, ny, nz, i);
     rowptr[i] = p;
     if (i == n - 1) rowptr[n] = (int)stencil_rowptr_at(ndim, nx, ny, nz, n);
@@ -729,16 +756,16 @@ __global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double 
         colind[p] = col;
         vals[p] = val;
         if (vals32) vals32[p] = (float)val;
-        if (rel16) rel16[p] = (int16_t)(col - row);
         ++p;
     };
-    if (ndim >= 3 && iz > 0) put(row - sz, -1.0);
-    if (ndim >= 2 && iy > 0) put(row - sy, -1.0);
-    if (ix > 0) put(row - 1, -1.0);
+    if (f & 16) put(row - sz, -1.0);
+    if (f & 4) put(row - sy, -1.0);
+    if (f & 1) put(row - 1, -1.0);
     put(row, 2.0 * ndim + shift);
-    if (ix < nx - 1) put(row + 1, -1.0);
-    if (ndim >= 2 && iy < ny - 1) put(row + sy, -1.0);
-    if (ndim >= 3 && iz < nz - 1) put(row + sz, -1.0);
+    if (f & 2) put(row + 1, -1.0);
+    if (f & 8) put(row + sy, -1.0);
+    if (f & 32) put(row + sz, -1.0);
+    if (rpid) rpid[i] = (uint8_t)f;
     b[i] = hash_uniform(seed, i);
     x[i] = 0.0;
     if (z32a) {
@@ -803,13 +830,13 @@ static int stencil_shape(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, Stenc
 }
 
 // Device bytes of an assembled stencil system: x, b, rowptr, colind, values
-// (+ FP32 values and int16 column offsets), workspace -- x first, so the
+// (+ FP32 values, row pattern ids and table), workspace -- x first, so the
 // solution is the first n doubles of the buffer.
 static int64_t stencil_bytes(const StencilShape& g, int32_t mode)
 {
     const size_t n = (size_t)g.n, nnz = (size_t)g.nnz;
     size_t b = 2 * al256(n * 8) + al256((n + 1) * 4) + al256(nnz * 4) + al256(nnz * 8);
-    if (mode == 1) b += al256(nnz * 4) + al256(nnz * 2);
+    if (mode == 1) b += al256(nnz * 4) + al256(n) + al256(kStencilPat * 16);
     return (int64_t)b + eng_work_bytes((int64_t)n, mode);
 }
 
@@ -1242,7 +1269,8 @@ int num_engine_add(num_engine* E, const num_csr* A, const double* d_b, double* d
     d.vals = A->vals;
     d.vals32 = A->vals32;
     d.tiles = static_cast<const TileEntry*>(A->sched.d_tiles);
-    d.rel16 = nullptr;
+    d.rpid = nullptr;
+    d.rpat = nullptr;
     d.b = d_b;
     d.x = d_x;
     d.n = A->nrows;
@@ -1318,12 +1346,15 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
     int32_t* colind = (int32_t*)(w + o); o += al256(nnz * 4);
     double* vals = (double*)(w + o); o += al256(nnz * 8);
     float* vals32 = nullptr;
-    int16_t* rel16 = nullptr;
+    uint8_t* rpid = nullptr;
+    uint64_t* rpat = nullptr;
     if (mode == 1) {
         vals32 = (float*)(w + o);
         o += al256(nnz * 4);
-        rel16 = (int16_t*)(w + o);
-        o += al256(nnz * 2);
+        rpid = (uint8_t*)(w + o);
+        o += al256(n);
+        rpat = (uint64_t*)(w + o);
+        o += al256(kStencilPat * 16);
     }
     carve_work(d, w + o, n, mode);
     d.rowptr = rowptr;
@@ -1331,13 +1362,14 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
     d.vals = vals;
     d.vals32 = vals32;
     d.tiles = nullptr;
-    d.rel16 = rel16;
+    d.rpid = rpid;
+    d.rpat = rpat;
     d.rpt = mode == 1 ? kStencilRptPair : kStencilRpt;
     d.n = (int32_t)n;
     d.ntiles = (int32_t)((n + d.rpt - 1) / d.rpt);
     d.mode = mode;
     stencil_request_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-        ndim, nx, ny, nz, shift, seed, rowptr, colind, vals, vals32, rel16, b, d.x,
+        ndim, nx, ny, nz, shift, seed, rowptr, colind, vals, vals32, rpid, rpat, b, d.x,
         mode == 1 ? d.x32a : nullptr, mode == 1 ? d.p32a : nullptr);
     NUM_LAUNCH_CHECK();
     SysState h = initial_state(tol, max_iter, mode, delta);

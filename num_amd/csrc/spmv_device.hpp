@@ -347,8 +347,9 @@ __device__ __forceinline__ void spmv_rel16_run(
 // kRelPat): only the values are staged; each row takes its pattern id (one
 // byte) and, after the stage barrier, its offsets from the pair's table in
 // LDS (`lpat`, staged with the values).  Same entries, same order, same
-// bits as spmv_rel16_run.  Rows hold at most kRelPatW entries.
-template <int BS, int NPT, int RPT, bool NT, class Epi, class Gather>
+// bits as spmv_rel16_run.  Rows hold at most kRelPatW entries.  NPW: the
+// table's uint64 words (two per pattern, at most BS).
+template <int BS, int NPT, int RPT, bool NT, int NPW, class Epi, class Gather>
 __device__ __forceinline__ void spmv_relpat_run(
     const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
     const uint8_t* __restrict__ pid, const uint64_t* __restrict__ pat,
@@ -368,8 +369,9 @@ __device__ __forceinline__ void spmv_relpat_run(
         re[k] = rowptr[r0 + lr + (t + k * BS < nr ? 1 : 0)];
         id[k] = t + k * BS < nr ? (int)pid[r0 + t + k * BS] : 0;
     }
+    static_assert(NPW <= BS, "one table word per thread");
     uint64_t tab = 0;
-    if (t < 2 * kRelPat) tab = pat[t];
+    if (t < NPW) tab = pat[t];
     if (nz > 0) {
         const int last = j1 - 1;
         float v[M];
@@ -384,7 +386,7 @@ __device__ __forceinline__ void spmv_relpat_run(
     } else if constexpr (has_hook<Gather>::value) {
         x.issue();
     }
-    if (t < 2 * kRelPat) lpat[t] = tab;
+    if (t < NPW) lpat[t] = tab;
     __syncthreads();
     ca = 0.0;
     cb = 0.0;
