@@ -68,7 +68,8 @@ constexpr int kMaxChunks = 256;   // chunks per system, at most (each sums the t
 // as two 256-row halves.
 constexpr int kStencilRpt = 256;
 constexpr int kStencilRptPair = 512;
-// Row patterns of the mixed form's FP32 steps: a row's column offsets are
+// Row patterns of an assembled stencil (FP64 tiles, and the mixed form's FP32
+// steps; its rare FP64 refinements read colind): a row's column offsets are
 // fixed by which of its six grid neighbours exist, so a row carries one id
 // byte (bit 0: -x, 1: +x, 2: -y, 3: +y, 4: -z, 5: +z neighbour present) into
 // the system's table of 64 patterns, each kRelPatW int16 offsets in column
@@ -94,7 +95,7 @@ struct SysDesc {
     float* q32;
     float* p32a;
     float* p32b;
-    const uint8_t* rpid;   // mixed stencil: row pattern ids (FP32 phases), 512-row tiles
+    const uint8_t* rpid;   // assembled stencil: row pattern ids
     const uint64_t* rpat;  // and the table of kStencilPat patterns; or nullptr
     int32_t n, ntiles, mode, rpt;
     int32_t crows;         // rows per update-kernel chunk
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(kSpmvBS, 7) void eng_ka(const BTile* __restrict__ b
     const TileEntry e0 = {e.r0, e.wide, e.nz0, 0};
     const TileEntry e1 = {e.r1, kWalkShort, e.nz1, 0};
     double ca = 0.0, cb = 0.0;
-    if (d->rpat) {
+    if (d->rpat && d->mode == 1) {
         // 512-row tile of a mixed stencil system
         if (ph == PH_RUN32) {
             const int ps = st->psel;
@@ -263,17 +264,25 @@ __global__ __launch_bounds__(kSpmvBS, 7) void eng_ka(const BTile* __restrict__ b
             ca += xa;
             cb += xb;
         }
-    } else if (ph == PH_INIT) {
-        GatherX<double> g{d->x};
-        EEpiInit epi{d->b, d->r, d->p0};
-        spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind, d->vals, g,
-                                                      epi, lv, lc, ca, cb);
-    } else if (ph == PH_RUN64) {
-        const int ps = st->psel;
-        EGatherPNew<double> g{d->r, ps ? d->p1 : d->p0, st->beta};
-        EEpiStep<double> epi{ps ? d->p0 : d->p1, d->Ap};
-        spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind, d->vals, g,
-                                                      epi, lv, lc, ca, cb);
+    } else if (ph == PH_INIT || ph == PH_RUN64) {
+        // FP64 system: an assembled stencil's 256-row tiles read the row
+        // patterns, a CSR system's tiles its columns
+        auto run = [&](const auto& g, const auto& epi) {
+            if (d->rpat)
+                spmv_relpat_run<kSpmvBS, kEngNpt, 1, NT64, 2 * kStencilPat>(
+                    e0, e1, d->rowptr, d->rpid, d->rpat, d->vals, g, epi, lv,
+                    reinterpret_cast<uint64_t*>(lc), ca, cb);
+            else
+                spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind,
+                                                              d->vals, g, epi, lv, lc, ca, cb);
+        };
+        if (ph == PH_INIT) {
+            run(GatherX<double>{d->x}, EEpiInit{d->b, d->r, d->p0});
+        } else {
+            const int ps = st->psel;
+            run(EGatherPNew<double>{d->r, ps ? d->p1 : d->p0, st->beta},
+                EEpiStep<double>{ps ? d->p0 : d->p1, d->Ap});
+        }
     } else if (ph == PH_REFINE) {
         const int xs = st->xsel;
         EGatherXsum g{xs ? d->x64b : d->x, xs ? d->x32b : d->x32a};
@@ -712,8 +721,8 @@ __device__ __forceinline__ int64_t stencil_rowptr_at(int ndim, int nx, int ny, i
 }
 
 // One thread per row: rowptr, columns (-z, -y, -x, self, +x, +y, +z), values
-// (-1 off the diagonal, 2*ndim + shift on it), b, x = 0 and, for the mixed
-// form, the FP32 copy, the row pattern ids and table (kStencilPat) and the
+// (-1 off the diagonal, 2*ndim + shift on it), the row pattern ids and table
+// (kStencilPat), b, x = 0 and, for the mixed form, the FP32 copy and the
 // zeroed x32a / p32a.  Unused grid dimensions are 1, so their neighbour bits
 // are never set.  Offsets fit int16: lines and planes hold <= 32767 points.
 __global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double shift,
@@ -830,13 +839,14 @@ static int stencil_shape(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, Stenc
 }
 
 // Device bytes of an assembled stencil system: x, b, rowptr, colind, values
-// (+ FP32 values, row pattern ids and table), workspace -- x first, so the
+// (+ FP32 values), row pattern ids and table, workspace -- x first, so the
 // solution is the first n doubles of the buffer.
 static int64_t stencil_bytes(const StencilShape& g, int32_t mode)
 {
     const size_t n = (size_t)g.n, nnz = (size_t)g.nnz;
     size_t b = 2 * al256(n * 8) + al256((n + 1) * 4) + al256(nnz * 4) + al256(nnz * 8);
-    if (mode == 1) b += al256(nnz * 4) + al256(n) + al256(kStencilPat * 16);
+    b += al256(n) + al256(kStencilPat * 16);
+    if (mode == 1) b += al256(nnz * 4);
     return (int64_t)b + eng_work_bytes((int64_t)n, mode);
 }
 
@@ -1346,16 +1356,14 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
     int32_t* colind = (int32_t*)(w + o); o += al256(nnz * 4);
     double* vals = (double*)(w + o); o += al256(nnz * 8);
     float* vals32 = nullptr;
-    uint8_t* rpid = nullptr;
-    uint64_t* rpat = nullptr;
     if (mode == 1) {
         vals32 = (float*)(w + o);
         o += al256(nnz * 4);
-        rpid = (uint8_t*)(w + o);
-        o += al256(n);
-        rpat = (uint64_t*)(w + o);
-        o += al256(kStencilPat * 16);
     }
+    uint8_t* rpid = (uint8_t*)(w + o);
+    o += al256(n);
+    uint64_t* rpat = (uint64_t*)(w + o);
+    o += al256(kStencilPat * 16);
     carve_work(d, w + o, n, mode);
     d.rowptr = rowptr;
     d.colind = colind;

@@ -348,12 +348,13 @@ __device__ __forceinline__ void spmv_rel16_run(
 // byte) and, after the stage barrier, its offsets from the pair's table in
 // LDS (`lpat`, staged with the values).  Same entries, same order, same
 // bits as spmv_rel16_run.  Rows hold at most kRelPatW entries.  NPW: the
-// table's uint64 words (two per pattern, at most BS).
-template <int BS, int NPT, int RPT, bool NT, int NPW, class Epi, class Gather>
+// table's uint64 words (two per pattern, at most BS).  T = double walks a
+// short-row FP64 tile with the same sums as spmv_tile_run.
+template <int BS, int NPT, int RPT, bool NT, int NPW, typename T, class Epi, class Gather>
 __device__ __forceinline__ void spmv_relpat_run(
     const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
     const uint8_t* __restrict__ pid, const uint64_t* __restrict__ pat,
-    const float* __restrict__ vals, const Gather& x, const Epi& epi, float* __restrict__ lv,
+    const T* __restrict__ vals, const Gather& x, const Epi& epi, T* __restrict__ lv,
     uint64_t* __restrict__ lpat, double& ca, double& cb)
 {
     constexpr int M = NPT * RPT;
@@ -374,7 +375,7 @@ __device__ __forceinline__ void spmv_relpat_run(
     if (t < NPW) tab = pat[t];
     if (nz > 0) {
         const int last = j1 - 1;
-        float v[M];
+        T v[M];
 #pragma unroll
         for (int s = 0; s < M; ++s) v[s] = ld_mat<NT>(vals + min(j0 + t + s * BS, last));
         if constexpr (has_hook<Gather>::value) x.issue();
@@ -400,7 +401,7 @@ __device__ __forceinline__ void spmv_relpat_run(
         const int row = r0 + lr;
         const int b = rb[k] - j0, len = re[k] - rb[k];
         const uint64_t q0 = lpat[2 * id[k]], q1 = lpat[2 * id[k] + 1];
-        float vv[kRelPatW], xv[kRelPatW];
+        T vv[kRelPatW], xv[kRelPatW];
 #pragma unroll
         for (int u = 0; u < kRelPatW; ++u) {
             const bool ok = u < len;
@@ -408,7 +409,7 @@ __device__ __forceinline__ void spmv_relpat_run(
             const int off = (int)(int16_t)(uint16_t)(((u < 4 ? q0 : q1) >> (16 * (u & 3))) & 0xffff);
             xv[u] = x(row + (ok ? off : 0));  // zero offset past the row: its own column
         }
-        float acc = 0.0f;
+        T acc = T(0);
 #pragma unroll
         for (int u = 0; u < kRelPatW; ++u)
             if (u < len) acc += vv[u] * xv[u];

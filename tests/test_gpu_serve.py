@@ -46,17 +46,17 @@ def test_engine_stencil_assembly(ks, dims, mixed):
     assert np.array_equal(arr("values", np.float64, nnz), A.values)
     if mixed:
         assert np.array_equal(arr("values32", np.float32, nnz), A.values.astype(np.float32))
-        # each row's pattern, decoded from its id byte and the table, lists
-        # exactly the row's column - row offsets, zero past the row
-        pid = arr("rpid", np.uint8, n).astype(np.int64)
-        tab = arr("rpat", np.int16, 64 * 8).reshape(64, 8).astype(np.int64)
-        lens = np.diff(A.rowptr)
-        assert lens.max() <= 8
-        rows = np.repeat(np.arange(n), lens)
-        slot = np.arange(nnz) - np.repeat(A.rowptr[:-1], lens)
-        assert np.array_equal(tab[pid[rows], slot], A.colind - rows)
-        mask = np.arange(8)[None, :] >= lens[:, None]
-        assert not tab[pid][mask].any()
+    # each row's pattern, decoded from its id byte and the table, lists
+    # exactly the row's column - row offsets, zero past the row
+    pid = arr("rpid", np.uint8, n).astype(np.int64)
+    tab = arr("rpat", np.int16, 64 * 8).reshape(64, 8).astype(np.int64)
+    lens = np.diff(A.rowptr)
+    assert lens.max() <= 8
+    rows = np.repeat(np.arange(n), lens)
+    slot = np.arange(nnz) - np.repeat(A.rowptr[:-1], lens)
+    assert np.array_equal(tab[pid[rows], slot], A.colind - rows)
+    mask = np.arange(8)[None, :] >= lens[:, None]
+    assert not tab[pid][mask].any()
     assert np.array_equal(arr("b", np.float64, n), g.hash_uniform(n, seed))
     assert np.array_equal(arr("x", np.float64, n), np.zeros(n))
     assert torch.equal(eng.solution(s).cpu(), torch.zeros(n, dtype=torch.float64))
