@@ -583,7 +583,8 @@ __global__ __launch_bounds__(kSpmvBS) void xw_tile_kernel(const int32_t* __restr
 // (sell_pos, four per group), zeros in the padding, and one length byte
 // per row; one workgroup per tile, one thread per row slot.  With `pat`:
 // also the row-pattern form (kXwPat): each row's slots minus its lane plus
-// kXwPatBias (0 past its length) are compared in rounds, the tile's
+// kXwPatBias (0 past its length), its length in bits 12-15 of the first
+// (a biased slot is < kXwSlots + kXwPatBias < 4096), are compared in rounds, the tile's
 // distinct patterns numbered in order of their first row, then the rows'
 // id bytes and the tile's table are written; a tile with more than kXwPat
 // patterns sets *pfail.
@@ -623,6 +624,7 @@ __global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restr
         }
         out[sell_pos(k, i, width, 4, kSpmvBS)] = (uint16_t)s;
     }
+    q0 |= (uint64_t)len << 12;
     if (!pat) return;
     // number the distinct patterns in order of first row: each round the
     // first row without an id leads, every row with its pattern takes the

@@ -639,9 +639,10 @@ __device__ __forceinline__ int sell_slot(int k, uint64_t q0, uint64_t q1)
 // tile, a stencil's neighbour values fetched once per tile instead of once
 // per entry.  Entries are summed in index order from 0.0 with separately
 // rounded products: the bits of spmv_tile_run and spmv_d16_run.
-// With PAT the row's slots come from its tile's pattern table (one id byte
-// per row, kXwPat) instead of 2 bytes per entry: slot = pattern value -
-// kXwPatBias + lane.
+// With PAT the row's slots and length come from its tile's pattern table
+// (one id byte per row, kXwPat) instead of 2 bytes per entry and a length
+// byte: slot = pattern value - kXwPatBias + lane (low 12 bits), length =
+// bits 12-15 of the first.
 template <int BS, bool NT, bool PAT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_sell_run(
     const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rec,
@@ -662,7 +663,8 @@ __device__ __forceinline__ void spmv_sell_run(
     const int64_t vb = (int64_t)rec[kXwBase] * BS;
     const int width = rec[kXwWidth];
     const int total = we3;
-    const int len = active ? (int)lens[r0 + t] : 0;
+    int len = 0;
+    if constexpr (!PAT) len = active ? (int)lens[r0 + t] : 0;
     // this row's entries: the tile's slices [0, width) (padding included;
     // uniform branches) and its slots, packed
     double v[kSellW];
@@ -697,13 +699,14 @@ __device__ __forceinline__ void spmv_sell_run(
     if constexpr (PAT) {
         q0 = lpat[2 * id];
         q1 = lpat[2 * id + 1];
+        len = (int)((q0 >> 12) & 15);
     }
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kSellW; ++k) {
         if constexpr (PAT) {
             // slot 0 for the padding keeps every LDS address inside the windows
-            const int sl = k < len ? sell_slot(k, q0, q1) - kXwPatBias + t : 0;
+            const int sl = k < len ? (sell_slot(k, q0, q1) & 0xfff) - kXwPatBias + t : 0;
             if (k < len) acc += v[k] * xw[sl];
         } else {
             if (k < len) acc += v[k] * xw[sell_slot(k, q0, q1)];

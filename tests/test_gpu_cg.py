@@ -237,6 +237,39 @@ def test_mixed_paths(ks, orc, case):
         assert r2 == r and np.array_equal(x2.cpu().numpy(), xh)
 
 
+def test_mixed_offsets_fall_back_past_16_patterns(ks, orc):
+    """Rows coupled at random offsets in +-1..+-12: a pair of tiles holds far
+    more than 16 distinct offset lists, so the matrix keeps the int16 offset
+    per entry (no pattern table), and the mixed solve still reaches the FP64
+    true residual and the oracle's solution."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(11)
+    n = 3000
+    offs = np.concatenate([-np.arange(1, 13), np.arange(1, 13)])
+    r = np.arange(n)
+    c = np.clip(r + rng.choice(offs, n), 0, n - 1)
+    keep = c != r
+    M = sp.coo_matrix((np.full(int(keep.sum()), -0.3), (r[keep], c[keep])), shape=(n, n))
+    M = (M + M.T).tocsr()
+    M = M + sp.diags(np.asarray(abs(M).sum(axis=1)).ravel() + 0.5)
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    A = g.CSR(M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64),
+              n, n)
+    assert np.diff(A.rowptr).max() <= 8
+    b = rng.uniform(-1, 1, n)
+    dA = _up(ks, A)
+    dA.ensure_f32()
+    assert dA._rel16 is not None and not dA.patterns["offsets"]
+    x = _dev(np.zeros(n))
+    res = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+    assert res["converged"]
+    xh = x.cpu().numpy()
+    assert np.linalg.norm(b - M @ xh) / np.linalg.norm(b) < 1e-10
+    x64, _ = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-10, max_iter=20000)
+    assert rel(xh, x64) < 1e-7
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_full_width_update_groups(ks, orc, mixed):
     """n >= 2^20 rows: the update kernel runs its full 256 workgroups, so the
