@@ -44,7 +44,8 @@ def stencil_layout(dims, mixed: bool) -> dict:
     """Byte offsets inside an assembled stencil system's buffer (the layout
     num_engine_add_stencil carves): x, b, rowptr, colind, values[, values32],
     rpid (a row's pattern id: its neighbour bits, uint8), rpat (64 patterns
-    of 8 int16 column - row offsets), workspace."""
+    of 8 int16 column - row offsets, then their lengths, 4 bits each),
+    workspace."""
     from .generators import stencil_nnz
     n = 1
     for d in dims:
@@ -53,7 +54,7 @@ def stencil_layout(dims, mixed: bool) -> dict:
     off, o = {}, 0
     for name, size in (("x", 8 * n), ("b", 8 * n), ("rowptr", 4 * (n + 1)),
                        ("colind", 4 * nnz), ("values", 8 * nnz)) + \
-            ((("values32", 4 * nnz),) if mixed else ()) + (("rpid", n), ("rpat", 64 * 16)):
+            ((("values32", 4 * nnz),) if mixed else ()) + (("rpid", n), ("rpat", 64 * 16 + 32)):
         off[name] = o
         o += _al256(size)
     off["work"] = o

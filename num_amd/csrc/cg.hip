@@ -759,8 +759,9 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_spmv_kernel(
 // both tiles are staged at once (FP32 values + int16 offsets fill the LDS of
 // one FP64 tile); FP64 refinement: the two tiles in turn.  Partials: one per
 // pair.
-// PAT: the offsets come as row patterns (rel_pat / rel_pid: one id byte per
-// row into the pair's table) instead of 2 bytes per entry; same bits.
+// PAT: the offsets and lengths come as row patterns (rel_pat / rel_pid: one
+// id byte per row into the pair's table) instead of 2 bytes per entry and
+// rowptr; same bits.
 template <int NPT, bool NT32, bool NT64, bool PAT>
 __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
@@ -780,10 +781,9 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     EpiPq epi{a.p0, a.p1, a.q, nullptr, nullptr};
     double ca, cb;
     if constexpr (PAT)
-        spmv_relpat_run<kSpmvBS, NPT, 2, NT32, 2 * kRelPat>(sched[t0], sched[t1], rowptr, rpid,
-                                                            rpat + (size_t)p * 2 * kRelPat, v32,
-                                                            g, epi, reinterpret_cast<float*>(lv),
-                                                            lpat, ca, cb);
+        spmv_relpat_run<kSpmvBS, NPT, 2, NT32, 2 * kRelPat>(
+            sched[t0], sched[t1], rpid, rpat + (size_t)p * relpat_words<2 * kRelPat>(), v32, g,
+            epi, reinterpret_cast<float*>(lv), lpat, ca, cb);
     else
         spmv_rel16_run<kSpmvBS, NPT, 2, NT32>(sched[t0], sched[t1], rowptr, rel, v32, g, epi,
                                               reinterpret_cast<float*>(lv),

@@ -103,11 +103,19 @@ constexpr int kXwPat = 16;
 constexpr int kXwPatBias = 256;
 static_assert(kXwSlots + kXwPatBias <= 4096 && kSellW < 16, "length field of a row pattern");
 // Row patterns of the int16 column offsets (num_csr_attach_rel16): a row's
-// offsets column - row (at most 8, zeros past its length, as uint16) per
-// pair of schedule tiles, at most kRelPat distinct patterns per pair, one id
-// byte per row (the FP32 pair kernel reads them instead of 2 bytes per entry).
+// offsets column - row (at most 8, zeros past its length, as uint16) and its
+// length per pair of schedule tiles, at most kRelPat distinct patterns per
+// pair (the table: 2 * kRelPat pattern words, then one word of 4-bit
+// lengths), one id byte per row (the FP32 pair kernel reads them instead of
+// 2 bytes per entry and rowptr).
 constexpr int kRelPat = 16;
 constexpr int kRelPatW = 8;
+// Words of a row-pattern table of NPW pattern words (two per pattern): the
+// patterns, then their lengths, 4 bits each, 16 to a word.
+template <int NPW>
+constexpr int relpat_len_words() { return (NPW / 2 + 15) / 16; }
+template <int NPW>
+constexpr int relpat_words() { return NPW + relpat_len_words<NPW>(); }
 
 // Position of entry k of lane i (of bs lanes) in a tile's packed region of
 // w slices, elements of which `pack` (4 for slots, 2 for FP32 values) share

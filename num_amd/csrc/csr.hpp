@@ -49,7 +49,8 @@ struct num_csr {
     const float* vals32 = nullptr;
     const int16_t* rel16 = nullptr;     // column - row as int16 (FP32 path), if attached
     // Row-pattern form of rel16 per pair of schedule tiles (kRelPat), owned:
-    // 2 * kRelPat uint64 per pair, then one pattern id per row.
+    // relpat_words<2 * kRelPat>() uint64 per pair (patterns, then lengths),
+    // then one pattern id per row.
     uint64_t* rel_pat = nullptr;
     uint8_t* rel_pid = nullptr;
     // Delta-coded column ids (FP64 path on the whole-matrix schedule), if

@@ -73,8 +73,10 @@ constexpr int kStencilRptPair = 512;
 // fixed by which of its six grid neighbours exist, so a row carries one id
 // byte (bit 0: -x, 1: +x, 2: -y, 3: +y, 4: -z, 5: +z neighbour present) into
 // the system's table of 64 patterns, each kRelPatW int16 offsets in column
-// order (-sz, -sy, -1, 0, +1, +sy, +sz for those present; 0 past the row).
+// order (-sz, -sy, -1, 0, +1, +sy, +sz for those present; 0 past the row),
+// followed by their lengths (relpat_words: 4 bits each, 4 words).
 constexpr int kStencilPat = 64;
+constexpr int kStencilPatBytes = relpat_words<2 * kStencilPat>() * 8;
 
 struct SysDesc {
     const int32_t* rowptr;
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kSpmvBS, 7) void eng_ka(const BTile* __restrict__ b
             EGatherPNew<float> g{d->r32, ps ? d->p32b : d->p32a, (float)st->beta};
             EEpiStep<float> epi{ps ? d->p32a : d->p32b, d->q32};
             spmv_relpat_run<kSpmvBS, kEngNpt, 2, NT32, 2 * kStencilPat>(
-                e0, e1, d->rowptr, d->rpid, d->rpat, d->vals32, g, epi,
+                e0, e1, d->rpid, d->rpat, d->vals32, g, epi,
                 reinterpret_cast<float*>(lv), reinterpret_cast<uint64_t*>(lc), ca, cb);
         } else {
             // FP64 refinement (the only other phase of a mixed system): the two
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(kSpmvBS, 7) void eng_ka(const BTile* __restrict__ b
         auto run = [&](const auto& g, const auto& epi) {
             if (d->rpat)
                 spmv_relpat_run<kSpmvBS, kEngNpt, 1, NT64, 2 * kStencilPat>(
-                    e0, e1, d->rowptr, d->rpid, d->rpat, d->vals, g, epi, lv,
+                    e0, e1, d->rpid, d->rpat, d->vals, g, epi, lv,
                     reinterpret_cast<uint64_t*>(lc), ca, cb);
             else
                 spmv_tile_run<double, kSpmvBS, kEngNpt, NT64>(e0, e1, d->rowptr, d->colind,
@@ -750,6 +752,14 @@ __global__ void stencil_request_kernel(int ndim, int nx, int ny, int nz, double 
         }
         rpat[2 * f] = q[0];
         rpat[2 * f + 1] = q[1];
+        if (f < relpat_len_words<2 * kStencilPat>()) {
+            // lengths of patterns 16f .. 16f + 15: the row itself plus its
+            // neighbours
+            uint64_t lw = 0;
+            for (int g = 0; g < 16; ++g)
+                lw |= (uint64_t)(1 + __popc(16 * f + g)) << (4 * g);
+            rpat[2 * kStencilPat + f] = lw;
+        }
     }
     if (i >= n) return;
     const int ix = (int)(i % nx);
@@ -845,7 +855,7 @@ static int64_t stencil_bytes(const StencilShape& g, int32_t mode)
 {
     const size_t n = (size_t)g.n, nnz = (size_t)g.nnz;
     size_t b = 2 * al256(n * 8) + al256((n + 1) * 4) + al256(nnz * 4) + al256(nnz * 8);
-    b += al256(n) + al256(kStencilPat * 16);
+    b += al256(n) + al256(kStencilPatBytes);
     if (mode == 1) b += al256(nnz * 4);
     return (int64_t)b + eng_work_bytes((int64_t)n, mode);
 }
@@ -1363,7 +1373,7 @@ int num_engine_add_stencil(num_engine* E, int32_t ndim, int32_t nx, int32_t ny, 
     uint8_t* rpid = (uint8_t*)(w + o);
     o += al256(n);
     uint64_t* rpat = (uint64_t*)(w + o);
-    o += al256(kStencilPat * 16);
+    o += al256(kStencilPatBytes);
     carve_work(d, w + o, n, mode);
     d.rowptr = rowptr;
     d.colind = colind;

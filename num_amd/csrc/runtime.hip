@@ -308,12 +308,13 @@ __global__ void rel16_kernel(int32_t nrows, const int32_t* __restrict__ rowptr,
 
 // Row patterns of the int16 offsets, one workgroup per pair of schedule
 // tiles (at most 2 * 256 rows, two per thread): each row's offsets column -
-// row (at most kRelPatW, zeros past its length) packed in two uint64; the
-// pair's distinct patterns numbered in order of their first row, in rounds
-// (the first row without an id leads, every row with its pattern takes the
-// next id); the rows' id bytes and the pair's table written.  *bad = 1 if a
-// row holds more than kRelPatW entries or the pair more than kRelPat
-// patterns.
+// row (at most kRelPatW, zeros past its length) packed in two uint64, with
+// its length (a zero offset is the diagonal, so the length is part of the
+// pattern); the pair's distinct patterns numbered in order of their first
+// row, in rounds (the first row without an id leads, every row with its
+// pattern takes the next id); the rows' id bytes and the pair's table
+// (relpat_words: patterns, then 4-bit lengths) written.  *bad = 1 if a row
+// holds more than kRelPatW entries or the pair more than kRelPat patterns.
 __global__ __launch_bounds__(256) void relpat_kernel(const int32_t* __restrict__ rowptr,
                                                      const int16_t* __restrict__ rel,
                                                      const TileEntry* __restrict__ tiles,
@@ -322,12 +323,14 @@ __global__ __launch_bounds__(256) void relpat_kernel(const int32_t* __restrict__
                                                      int32_t* __restrict__ bad)
 {
     __shared__ unsigned long long s_q[4 * 256];
+    __shared__ int s_l[2 * 256];
     __shared__ unsigned long long s_tab[2 * kRelPat];
+    __shared__ int s_tl[kRelPat];
     __shared__ int s_lead;
     const int p = blockIdx.x, t = threadIdx.x;
     const int r0 = tiles[2 * p].row, r1 = tiles[min(2 * p + 2, ntiles)].row;
     uint64_t q[2][2] = {{0, 0}, {0, 0}};
-    int myid[2] = {-1, -1};
+    int myid[2] = {-1, -1}, ln[2] = {0, 0};
     bool act[2];
     int over = 0;
     for (int h = 0; h < 2; ++h) {
@@ -335,6 +338,7 @@ __global__ __launch_bounds__(256) void relpat_kernel(const int32_t* __restrict__
         act[h] = r < r1;
         if (act[h]) {
             const int js = rowptr[r], len = rowptr[r + 1] - js;
+            ln[h] = len;
             if (len > kRelPatW) over = 1;
             for (int k = 0; k < min(len, kRelPatW); ++k) {
                 const uint64_t d = (uint64_t)(uint16_t)rel[js + k];
@@ -344,6 +348,7 @@ __global__ __launch_bounds__(256) void relpat_kernel(const int32_t* __restrict__
         }
         s_q[2 * lr] = q[h][0];
         s_q[2 * lr + 1] = q[h][1];
+        s_l[lr] = ln[h];
     }
     if (over) atomicOr(bad, 1);  // the host drops the table; the ids are not used
     int np = 0;
@@ -360,18 +365,28 @@ __global__ __launch_bounds__(256) void relpat_kernel(const int32_t* __restrict__
             return;  // uniform: every thread saw the same lead and np
         }
         const unsigned long long a = s_q[2 * lead], c = s_q[2 * lead + 1];
+        const int l = s_l[lead];
         for (int h = 0; h < 2; ++h)
-            if (act[h] && myid[h] < 0 && q[h][0] == a && q[h][1] == c) myid[h] = np;
+            if (act[h] && myid[h] < 0 && q[h][0] == a && q[h][1] == c && ln[h] == l)
+                myid[h] = np;
         if (t == 0) {
             s_tab[2 * np] = a;
             s_tab[2 * np + 1] = c;
+            s_tl[np] = l;
         }
         ++np;
         __syncthreads();  // s_lead is reset only after every thread read it
     }
     for (int h = 0; h < 2; ++h)
         if (act[h]) pid[r0 + t + h * 256] = (uint8_t)myid[h];
-    if (t < 2 * kRelPat) pat[(size_t)p * 2 * kRelPat + t] = t < 2 * np ? s_tab[t] : 0ull;
+    uint64_t* out = pat + (size_t)p * relpat_words<2 * kRelPat>();
+    if (t < 2 * kRelPat) out[t] = t < 2 * np ? s_tab[t] : 0ull;
+    if (t == 0) {
+        uint64_t lw = 0;
+        for (int i = 0; i < np; ++i) lw |= (uint64_t)s_tl[i] << (4 * i);
+        out[2 * kRelPat] = lw;
+    }
+    static_assert(relpat_len_words<2 * kRelPat>() == 1, "one length word per pair");
 }
 
 // Delta codes, one workgroup per tile (rows_per_block <= blockDim): the
@@ -457,7 +472,7 @@ static int build_relpat(num_csr* A, hipStream_t st)
         s.rows_per_block > 256 || s.long_rows != 0)
         return NUM_OK;
     const int32_t npairs = (s.nblocks + 1) / 2;
-    const size_t npat = (size_t)npairs * 2 * kRelPat;
+    const size_t npat = (size_t)npairs * relpat_words<2 * kRelPat>();
     int32_t* d_bad = nullptr;
     hipError_t e = hipMalloc(&d_bad, sizeof(int32_t));
     if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(int32_t), st);

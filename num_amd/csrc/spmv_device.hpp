@@ -345,32 +345,38 @@ __device__ __forceinline__ void spmv_rel16_run(
 
 // spmv_rel16_run with the offsets' row patterns (num_csr_attach_rel16,
 // kRelPat): only the values are staged; each row takes its pattern id (one
-// byte) and, after the stage barrier, its offsets from the pair's table in
-// LDS (`lpat`, staged with the values).  Same entries, same order, same
-// bits as spmv_rel16_run.  Rows hold at most kRelPatW entries.  NPW: the
-// table's uint64 words (two per pattern, at most BS).  T = double walks a
-// short-row FP64 tile with the same sums as spmv_tile_run.
+// byte), its length from the table's length words (loaded up front, uniform)
+// and, after the stage barrier, its offsets from the table in LDS (`lpat`,
+// staged with the values).  No rowptr: a row's first entry is the tile's
+// first (e0.nz0) plus the lengths of the rows before it, a workgroup scan of
+// the lengths packed 16 bits per row slot (row t + k*BS in field k), its
+// wave totals published before the stage barrier.  Same entries, same
+// order, same bits as spmv_rel16_run.  Rows hold at most kRelPatW entries.
+// pat: NPW pattern words (at most BS) then relpat_len_words<NPW>() length
+// words.  T = double walks a short-row FP64 tile with the same sums as
+// spmv_tile_run.
 template <int BS, int NPT, int RPT, bool NT, int NPW, typename T, class Epi, class Gather>
 __device__ __forceinline__ void spmv_relpat_run(
-    const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rowptr,
-    const uint8_t* __restrict__ pid, const uint64_t* __restrict__ pat,
-    const T* __restrict__ vals, const Gather& x, const Epi& epi, T* __restrict__ lv,
-    uint64_t* __restrict__ lpat, double& ca, double& cb)
+    const TileEntry e0, const TileEntry e1, const uint8_t* __restrict__ pid,
+    const uint64_t* __restrict__ pat, const T* __restrict__ vals, const Gather& x,
+    const Epi& epi, T* __restrict__ lv, uint64_t* __restrict__ lpat, double& ca, double& cb)
 {
     constexpr int M = NPT * RPT;
+    constexpr int NL = relpat_len_words<NPW>();
+    constexpr int NW = BS / kWave;
+    static_assert(NPW <= BS, "one table word per thread");
+    static_assert(RPT <= 2 && BS * kRelPatW < 65536, "row starts packed 16 bits per row slot");
+    __shared__ int lscan[NW];
     const int t = threadIdx.x;
     const int r0 = e0.row, j0 = e0.nz0;
     const int r1 = e1.row, j1 = e1.nz0;
     const int nz = j1 - j0, nr = r1 - r0;
-    int rb[RPT], re[RPT], id[RPT];
+    uint64_t lw[NL];
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int lr = min(t + k * BS, nr);
-        rb[k] = rowptr[r0 + lr];
-        re[k] = rowptr[r0 + lr + (t + k * BS < nr ? 1 : 0)];
-        id[k] = t + k * BS < nr ? (int)pid[r0 + t + k * BS] : 0;
-    }
-    static_assert(NPW <= BS, "one table word per thread");
+    for (int w = 0; w < NL; ++w) lw[w] = pat[NPW + w];
+    int id[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) id[k] = t + k * BS < nr ? (int)pid[r0 + t + k * BS] : 0;
     uint64_t tab = 0;
     if (t < NPW) tab = pat[t];
     if (nz > 0) {
@@ -387,6 +393,24 @@ __device__ __forceinline__ void spmv_relpat_run(
     } else if constexpr (has_hook<Gather>::value) {
         x.issue();
     }
+    int len[RPT], pk = 0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        uint64_t w = lw[0];
+#pragma unroll
+        for (int u = 1; u < NL; ++u)
+            if ((id[k] >> 4) == u) w = lw[u];
+        len[k] = t + k * BS < nr ? (int)((w >> (4 * (id[k] & 15))) & 15) : 0;
+        pk |= len[k] << (16 * k);
+    }
+    const int lane = t & (kWave - 1);
+    int inc = pk;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int u = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += u;
+    }
+    if (lane == kWave - 1) lscan[t / kWave] = inc;
     if (t < NPW) lpat[t] = tab;
     __syncthreads();
     ca = 0.0;
@@ -394,17 +418,29 @@ __device__ __forceinline__ void spmv_relpat_run(
     if constexpr (has_hook<Gather>::value) {
         if (!x.resolve()) return;
     }
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const int sw = lscan[w];
+        if (w < t / kWave) before += sw;
+        tot += sw;
+    }
+    const int ex = before + inc - pk;  // exclusive prefix of every field
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int lr = t + k * BS;
         if (lr >= nr) continue;
         const int row = r0 + lr;
-        const int b = rb[k] - j0, len = re[k] - rb[k];
+        // fields below k hold whole earlier row slots
+        int b = (ex >> (16 * k)) & 0xffff;
+#pragma unroll
+        for (int f = 0; f < k; ++f) b += (tot >> (16 * f)) & 0xffff;
+        const int n = len[k];
         const uint64_t q0 = lpat[2 * id[k]], q1 = lpat[2 * id[k] + 1];
         T vv[kRelPatW], xv[kRelPatW];
 #pragma unroll
         for (int u = 0; u < kRelPatW; ++u) {
-            const bool ok = u < len;
+            const bool ok = u < n;
             vv[u] = lv[ok ? b + u : 0];  // slot 0 is always inside the LDS array
             const int off = (int)(int16_t)(uint16_t)(((u < 4 ? q0 : q1) >> (16 * (u & 3))) & 0xffff);
             xv[u] = x(row + (ok ? off : 0));  // zero offset past the row: its own column
@@ -412,7 +448,7 @@ __device__ __forceinline__ void spmv_relpat_run(
         T acc = T(0);
 #pragma unroll
         for (int u = 0; u < kRelPatW; ++u)
-            if (u < len) acc += vv[u] * xv[u];
+            if (u < n) acc += vv[u] * xv[u];
         double ta = 0.0, tb = 0.0;
         epi(row, acc, ta, tb, x);
         ca += ta;

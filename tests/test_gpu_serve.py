@@ -57,6 +57,10 @@ def test_engine_stencil_assembly(ks, dims, mixed):
     assert np.array_equal(tab[pid[rows], slot], A.colind - rows)
     mask = np.arange(8)[None, :] >= lens[:, None]
     assert not tab[pid][mask].any()
+    # and its length, a 4-bit field of the length words after the table
+    lw = arr("rpat", np.uint64, 64 * 2 + 4)[128:]
+    nib = np.array([(int(lw[f >> 4]) >> (4 * (f & 15))) & 15 for f in range(64)])
+    assert np.array_equal(nib[pid], lens)
     assert np.array_equal(arr("b", np.float64, n), g.hash_uniform(n, seed))
     assert np.array_equal(arr("x", np.float64, n), np.zeros(n))
     assert torch.equal(eng.solution(s).cpu(), torch.zeros(n, dtype=torch.float64))
