@@ -270,6 +270,35 @@ def test_mixed_offsets_fall_back_past_16_patterns(ks, orc):
     assert rel(xh, x64) < 1e-7
 
 
+def test_mixed_offsets_unsorted_duplicate_columns(ks, orc):
+    """A 2-D stencil stored with rows in one of four entry orders and the
+    diagonal split into two duplicate entries on every third row (the CSR
+    contract allows both): the offset patterns carry each row's stored
+    order and its length (a duplicated diagonal is two zero offsets), and
+    the mixed solve reaches the FP64 true residual and the oracle's
+    solution, with the same bits as the int16 offsets per entry."""
+    from _helpers import shuffled_stencil_rows
+    import scipy.sparse as sp
+    rp, ci, va, n = shuffled_stencil_rows(90, 60, seed=6)
+    A = g.CSR(rp, ci, va, n, n)
+    M = sp.csr_matrix((va.copy(), ci.copy(), rp.copy()), shape=(n, n))
+    b = np.random.default_rng(8).uniform(-1, 1, n)
+    dA = _up(ks, A)
+    dA.ensure_f32()
+    assert dA.patterns["offsets"]
+    x = _dev(np.zeros(n))
+    res = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+    assert res["converged"]
+    xh = x.cpu().numpy()
+    assert np.linalg.norm(b - M @ xh) / np.linalg.norm(b) < 1e-10
+    x64, _ = orc.cg_solve_local(rp, ci, va, b, tol=1e-10, max_iter=20000)
+    assert rel(xh, x64) < 1e-7
+    dA.set_mixed_path(2)
+    x2 = _dev(np.zeros(n))
+    res2 = ks.cg_solve_mixed(dA, _dev(b), x2, tol=1e-10)
+    assert res2 == res and np.array_equal(x2.cpu().numpy(), xh)
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_full_width_update_groups(ks, orc, mixed):
     """n >= 2^20 rows: the update kernel runs its full 256 workgroups, so the

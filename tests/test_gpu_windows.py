@@ -14,6 +14,7 @@ import pytest
 import torch
 
 from num_amd import generators as g
+from _helpers import shuffled_stencil_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -109,6 +110,21 @@ def test_row_patterns_fall_back_to_slots(ks, orc):
     assert dA.index_format == "window16"
     x = rng.uniform(-1, 1, n)
     assert np.array_equal(_spmv(ks, dA, x), orc.spmv(rp, ci, va, x))
+
+
+def test_windows_unsorted_duplicate_columns(ks, orc):
+    # the row patterns keep each row's entries in their stored order, so the
+    # sums stay the row loop's: bit-identical with patterns, slots and none
+    rp, ci, va, n = shuffled_stencil_rows(90, 60, seed=4)
+    x = np.random.default_rng(13).uniform(-1, 1, n)
+    ref = orc.spmv(rp, ci, va, x)
+    for on, pat, fmt in ((True, True, "windowpat"), (True, False, "window16"),
+                         (False, True, None)):
+        dA = _up(ks, rp, ci, va, n, n)
+        dA.set_windows(on, patterns=pat)
+        if fmt:
+            assert dA.index_format == fmt
+        assert np.array_equal(_spmv(ks, dA, x), ref), (on, pat)
 
 
 @pytest.mark.parametrize("w", range(1, 9))

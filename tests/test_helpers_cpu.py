@@ -27,3 +27,18 @@ def test_stencil_residual_of_exact_solve():
     x = sla.spsolve(A.to_scipy().tocsc(), b)
     assert stencil_residual(torch.from_numpy(x), dims, shift, seed) < 1e-14
     assert stencil_residual(torch.zeros(A.nrows, dtype=torch.float64), dims, shift, seed) == 1.0
+
+
+def test_shuffled_stencil_rows_is_the_stencil():
+    # the reordered matrix with duplicates summed is the 2-D stencil itself,
+    # and it really holds unsorted rows and duplicate columns
+    from _helpers import shuffled_stencil_rows
+    import scipy.sparse as sp
+    rp, ci, va, n = shuffled_stencil_rows(30, 20, seed=4)
+    M = sp.csr_matrix((va.copy(), ci.copy(), rp.copy()), shape=(n, n))  # scipy sorts in place
+    M.sum_duplicates()
+    S = g.stencil((30, 20), 0.5).to_scipy()
+    assert abs(M - S).max() == 0.0
+    rows = [ci[rp[i]:rp[i + 1]] for i in range(n)]
+    assert sum(np.any(np.diff(c) < 0) for c in rows) > n // 2
+    assert sum(len(set(c)) < len(c) for c in rows) >= n // 4
