@@ -483,12 +483,13 @@ def main():
     # bytes the kernel itself must move: with window slots (or delta-coded
     # column ids) it reads 2 index bytes per entry instead of the formula's
     # 4, and one length byte per row instead of the 4(n+1) bytes of rowptr;
-    # with row patterns no index bytes per entry, one pattern id byte per
-    # row (the pattern holds the row's length) and a 256-byte pattern table
-    # per tile
+    # with row patterns no index bytes per entry and no rowptr: a 4-bit
+    # pattern id per row (the pattern holds the row's length), 128 id bytes
+    # and a 4-byte table index per tile (tiles share the few distinct
+    # tables, which stay in the L2)
     fmt = dA.index_format
     if fmt == "windowpat":
-        Bk = B - (4 * A.nnz + 3 * A.nrows + 4) + 256 * dA.schedule_info()["nblocks"]
+        Bk = B - (4 * A.nnz + 4 * (A.nrows + 1)) + 132 * dA.schedule_info()["nblocks"]
     elif fmt in ("delta16", "window16"):
         Bk = B - (2 * A.nnz + 3 * A.nrows + 4)
     else:
@@ -520,8 +521,8 @@ def main():
         out["roofline"]["traffic_frac"] = round(traffic / avg / 1e9 / HBM_PEAK_GBS, 4)
     out["roofline"]["basis"] = (
         "frac: algorithmic bytes of int32 CSR (12 nnz + 4 (n+1) + 8 ncols + 8 nrows) over the "
-        "kernel time; the kernel reads one pattern id byte per row (and a 256-byte pattern "
-        "table per tile) instead of int32 columns and rowptr, so it moves fewer "
+        "kernel time; the kernel reads a 4-bit pattern id per row (and a table index per "
+        "tile) instead of int32 columns and rowptr, so it moves fewer "
         "bytes (kernel_min_bytes; measured: traffic) and frac can exceed what HBM alone "
         "allows; traffic_frac is the measured bytes over the same time")
     y_gpu = dy.cpu().numpy()

@@ -102,6 +102,7 @@ constexpr int kXwRec = 2 * kXwWin + 2;
 constexpr int kXwPat = 16;
 constexpr int kXwPatBias = 256;
 static_assert(kXwSlots + kXwPatBias <= 4096 && kSellW < 16, "length field of a row pattern");
+static_assert(kXwPat <= 16, "row pattern ids are 4 bits");
 // Row patterns of the int16 column offsets (num_csr_attach_rel16): a row's
 // offsets column - row (at most 8, zeros past its length, as uint16) and its
 // length per pair of schedule tiles, at most kRelPat distinct patterns per

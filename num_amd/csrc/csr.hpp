@@ -71,9 +71,13 @@ struct num_csr {
     uint8_t* sell_lens = nullptr;
     int32_t sell_w = 0;                 // slices per tile (tile b starts at b * sell_w)
     // Row-pattern form of the window slots (when every tile holds at most
-    // kXwPat distinct patterns), owned: per tile 2 * kXwPat uint64, then
-    // one pattern id per row.
+    // kXwPat distinct patterns), owned, one allocation: the distinct tables
+    // (2 * kXwPat uint64 each; tiles with equal tables share one), each
+    // tile's table index (sell_tix), and each row's 4-bit pattern id,
+    // kSpmvBS / 2 bytes per tile (sell_pid: lane i of tile b in byte
+    // b * kSpmvBS / 2 + i / 2, bits 4 * (i & 1)).
     uint64_t* sell_pat = nullptr;
+    int32_t* sell_tix = nullptr;
     uint8_t* sell_pid = nullptr;
     int32_t pat_mode = -1;              // row patterns: 0 off (the slots), else when they fit
     int32_t xw_mode = -1;               // 1 on, 0 off, -1 from kXwAutoRows rows

@@ -599,9 +599,10 @@ __global__ __launch_bounds__(kSpmvBS) void xw_tile_kernel(const int32_t* __restr
 // per row; one workgroup per tile, one thread per row slot.  With `pat`:
 // also the row-pattern form (kXwPat): each row's slots minus its lane plus
 // kXwPatBias (0 past its length), its length in bits 12-15 of the first
-// (a biased slot is < kXwSlots + kXwPatBias < 4096), are compared in rounds, the tile's
-// distinct patterns numbered in order of their first row, then the rows'
-// id bytes and the tile's table are written; a tile with more than kXwPat
+// (a biased slot is < kXwSlots + kXwPatBias < 4096), are compared in
+// rounds, the tile's distinct patterns numbered in order of their first
+// row, then the rows' 4-bit ids (two lanes to a byte, kSpmvBS / 2 bytes per
+// tile) and the tile's table are written; a tile with more than kXwPat
 // patterns sets *pfail.
 __global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restrict__ rowptr,
                                                         const int32_t* __restrict__ colind,
@@ -672,7 +673,10 @@ __global__ __launch_bounds__(kSpmvBS) void xw_code_kernel(const int32_t* __restr
         if (i == 0) atomicOr(pfail, 1);
         return;
     }
-    if (active) pid[r] = (uint8_t)myid;
+    // lanes i and i + 1 share a wave: the even lane writes both ids
+    const int idv = active ? myid : 0;
+    const int hi = __shfl_down(idv, 1, kWave);
+    if ((i & 1) == 0) pid[(size_t)b * (kSpmvBS / 2) + (i >> 1)] = (uint8_t)(idv | (hi << 4));
     if (i < 2 * kXwPat) pat[(size_t)b * 2 * kXwPat + i] = i < 2 * np ? s_tab[i] : 0ull;
 }
 
@@ -720,10 +724,61 @@ static int build_sell32(num_csr* A, hipStream_t st)
 
 static void free_patterns(num_csr* A)
 {
-    // sell_pid lives in the sell_pat allocation
+    // sell_tix and sell_pid live in the sell_pat allocation
     if (A->sell_pat) (void)hipFree(A->sell_pat);
     A->sell_pat = nullptr;
+    A->sell_tix = nullptr;
     A->sell_pid = nullptr;
+}
+
+// Keeps one copy of each distinct tile table: `tabs` holds every tile's
+// table (2 * kXwPat words), `pid` the 4-bit ids; the final allocation is
+// the distinct tables, each tile's index into them, and the ids (moved
+// over).  Interior tiles of a stencil share a few dozen tables, which then
+// stay in the L2 instead of streaming 256 bytes per tile.  Synchronises.
+static int share_tables(num_csr* A, const uint64_t* tabs, const uint8_t* pid, hipStream_t st)
+{
+    const int32_t nb = A->sched.nblocks;
+    constexpr int TW = 2 * kXwPat;
+    std::vector<uint64_t> h((size_t)nb * TW);
+    NUM_HIP(hipMemcpyAsync(h.data(), tabs, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           st));
+    NUM_HIP(hipStreamSynchronize(st));
+    std::map<std::vector<uint64_t>, int32_t> seen;
+    std::vector<uint64_t> uniq;
+    std::vector<int32_t> tix(nb);
+    for (int32_t b = 0; b < nb; ++b) {
+        std::vector<uint64_t> key(h.begin() + (size_t)b * TW, h.begin() + (size_t)(b + 1) * TW);
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            it = seen.emplace(std::move(key), (int32_t)(uniq.size() / TW)).first;
+            uniq.insert(uniq.end(), h.begin() + (size_t)b * TW, h.begin() + (size_t)(b + 1) * TW);
+        }
+        tix[b] = it->second;
+    }
+    const size_t tab_bytes = (uniq.size() * sizeof(uint64_t) + 255) & ~size_t(255);
+    const size_t tix_bytes = ((size_t)nb * sizeof(int32_t) + 255) & ~size_t(255);
+    const size_t pid_bytes = (size_t)nb * (kSpmvBS / 2);
+    char* buf = nullptr;
+    NUM_HIP(hipMalloc(&buf, tab_bytes + tix_bytes + pid_bytes));
+    hipError_t e = hipMemcpyAsync(buf, uniq.data(), uniq.size() * sizeof(uint64_t),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf + tab_bytes, tix.data(), (size_t)nb * sizeof(int32_t),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf + tab_bytes + tix_bytes, pid, pid_bytes, hipMemcpyDeviceToDevice,
+                           st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipFree(buf);
+        return fail(NUM_EHIP, "row pattern tables failed: %s", hipGetErrorString(e));
+    }
+    free_patterns(A);
+    A->sell_pat = reinterpret_cast<uint64_t*>(buf);
+    A->sell_tix = reinterpret_cast<int32_t*>(buf + tab_bytes);
+    A->sell_pid = reinterpret_cast<uint8_t*>(buf + tab_bytes + tix_bytes);
+    return NUM_OK;
 }
 
 void free_windows(num_csr* A)
@@ -779,7 +834,9 @@ static int build_xw(num_csr* A, hipStream_t st)
     if (e == hipSuccess) e = hipMemsetAsync(d_fail, 0, 2 * sizeof(int32_t), st);
     if (e == hipSuccess)
         e = hipMalloc(&A->sell_vals, nval * (sizeof(double) + sizeof(uint16_t)) + A->nrows);
-    if (e == hipSuccess) e = hipMalloc(&A->sell_pat, npat * sizeof(uint64_t) + A->nrows);
+    // every tile's table, then the ids (share_tables keeps the distinct tables)
+    if (e == hipSuccess)
+        e = hipMalloc(&A->sell_pat, npat * sizeof(uint64_t) + (size_t)s.nblocks * (kSpmvBS / 2));
     if (e == hipSuccess) {
         A->sell_codes = reinterpret_cast<uint16_t*>(A->sell_vals + nval);
         A->sell_lens = reinterpret_cast<uint8_t*>(A->sell_codes + nval);
@@ -802,7 +859,15 @@ static int build_xw(num_csr* A, hipStream_t st)
             return fail(NUM_EHIP, "windowed layout build failed: %s", hipGetErrorString(e));
         return NUM_OK;  // a tile does not fit: no windows
     }
-    if (h_fail[1] || A->pat_mode == 0) free_patterns(A);  // the slots serve instead
+    if (h_fail[1] || A->pat_mode == 0) {
+        free_patterns(A);  // the slots serve instead
+    } else {
+        const int rc = share_tables(A, A->sell_pat, A->sell_pid, st);
+        if (rc) {
+            free_windows(A);
+            return rc;
+        }
+    }
     A->sell_w = W;
     return build_sell32(A, st);
 }

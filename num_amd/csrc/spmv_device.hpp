@@ -676,9 +676,9 @@ __device__ __forceinline__ int sell_slot(int k, uint64_t q0, uint64_t q1)
 // per entry.  Entries are summed in index order from 0.0 with separately
 // rounded products: the bits of spmv_tile_run and spmv_d16_run.
 // With PAT the row's slots and length come from its tile's pattern table
-// (one id byte per row, kXwPat) instead of 2 bytes per entry and a length
-// byte: slot = pattern value - kXwPatBias + lane (low 12 bits), length =
-// bits 12-15 of the first.
+// (a 4-bit id per row, kXwPat; `pid` is the tile's kSpmvBS / 2 id bytes)
+// instead of 2 bytes per entry and a length byte: slot = pattern value -
+// kXwPatBias + lane (low 12 bits), length = bits 12-15 of the first.
 template <int BS, bool NT, bool PAT, class Epi, class Gather>
 __device__ __forceinline__ void spmv_sell_run(
     const TileEntry e0, const TileEntry e1, const int32_t* __restrict__ rec,
@@ -712,7 +712,7 @@ __device__ __forceinline__ void spmv_sell_run(
     if constexpr (PAT) {
         // the row's pattern id now; the tile's table goes to LDS with the
         // windows and is read after the barrier (no dependent global load)
-        id = active ? (int)ld_mat<NT>(pid + r0 + t) : 0;
+        id = active ? ((int)ld_mat<NT>(pid + (t >> 1)) >> (4 * (t & 1))) & 15 : 0;
         if (t < 2 * kXwPat) lpat[t] = pat[t];
     } else {
         sell_slots<NT, BS>(code + vb, width, t, q0, q1);
@@ -835,8 +835,9 @@ template <int BS, bool NT, bool PAT, class Epi, class Gather>
 __global__ __launch_bounds__(BS) void spmv_sell_kernel(
     const int32_t* __restrict__ rec, const uint8_t* __restrict__ lens,
     const uint16_t* __restrict__ code, const uint64_t* __restrict__ pat,
-    const uint8_t* __restrict__ pid, const double* __restrict__ vals, Gather x,
-    const TileEntry* __restrict__ sched, int ntiles, int chunk, Epi epi)
+    const int32_t* __restrict__ tix, const uint8_t* __restrict__ pid,
+    const double* __restrict__ vals, Gather x, const TileEntry* __restrict__ sched, int ntiles,
+    int chunk, Epi epi)
 {
     __shared__ double xw[kXwSlots];
     __shared__ double red[2 * BS / kWave];
@@ -846,9 +847,10 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(
     if (epi.skip()) return;
     x.prepare();
     double ca, cb;
-    spmv_sell_run<BS, NT, PAT, Epi, Gather>(sched[tile], sched[tile + 1], rec + tile * kXwRec,
-                                            lens, code, pat + (PAT ? tile * 2 * kXwPat : 0),
-                                            pid, vals, x, epi, xw, lpat, ca, cb);
+    spmv_sell_run<BS, NT, PAT, Epi, Gather>(
+        sched[tile], sched[tile + 1], rec + tile * kXwRec, lens, code,
+        PAT ? pat + (size_t)tix[tile] * 2 * kXwPat : pat, PAT ? pid + tile * (BS / 2) : pid, vals,
+        x, epi, xw, lpat, ca, cb);
     if constexpr (Epi::kRed == 1) {
         const double s = block_sum<BS>(ca, red);
         if (threadIdx.x == 0) epi.partial_a[tile] = s;

@@ -65,12 +65,12 @@ inline bool launch_sell(const num_csr* A, const Schedule& s, const T* vals, cons
             if (A->sell_pat)
                 spmv_sell_kernel<kSpmvBS, NT, true, Epi, Gather>
                     <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
-                        A->xw, A->sell_lens, A->sell_codes, A->sell_pat, A->sell_pid,
-                        A->sell_vals, x, tiles, s.nblocks, s.chunk, epi);
+                        A->xw, A->sell_lens, A->sell_codes, A->sell_pat, A->sell_tix,
+                        A->sell_pid, A->sell_vals, x, tiles, s.nblocks, s.chunk, epi);
             else
                 spmv_sell_kernel<kSpmvBS, NT, false, Epi, Gather>
                     <<<chunk_grid(s.nblocks, s.chunk), kSpmvBS, 0, stream>>>(
-                        A->xw, A->sell_lens, A->sell_codes, A->sell_pat, A->sell_pid,
+                        A->xw, A->sell_lens, A->sell_codes, nullptr, nullptr, nullptr,
                         A->sell_vals, x, tiles, s.nblocks, s.chunk, epi);
             return true;
         }
