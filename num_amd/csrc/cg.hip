@@ -798,121 +798,235 @@ __global__ __launch_bounds__(kSpmvBS, mix_waves<NPT>()) void mix_pair_kernel(
     if (threadIdx.x == 0) a.pa[p] = s;
 }
 
+// A tile's window record as scalars (the compiler keeps named fields in
+// registers; a record indexed inside loops went to LDS).
+struct XwWin {
+    int l0, l1, l2, l3, e0, e1, e2, e3;
+};
+__device__ __forceinline__ XwWin xw_win(const int32_t* rec)
+{
+    return {rec[0], rec[1], rec[2], rec[3], rec[4], rec[5], rec[6], rec[7]};
+}
+__device__ __forceinline__ int xw_win_col(int s, const XwWin& w)
+{
+    return xw_col(s, w.l0, w.l1, w.l2, w.l3, w.e0, w.e1, w.e2);
+}
+
+// One tile's share of mix_sell2_kernel's FP32 step for thread t: its row's
+// FP32 values and pattern (or slots), and its share of the tile's r32 and
+// p_old windows, all issued before the fold's barrier; the window values
+// are parked in LDS before the fold (park), so that their registers are free
+// while both tiles' values wait for the step.
+template <bool NT32, bool PAT>
+struct Sell2Tile {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    static constexpr int NXW = (kXwSlots + kSpmvBS - 1) / kSpmvBS;
+    XwWin w;
+    int64_t vb;
+    int width, total, r0, nr, id, len;
+    uint64_t q0, q1;
+    f2v v[kSellW / 2];
+    float vl;
+    float wr[NXW], wp[NXW];
+    // wr, wp to LDS: slots [0, total) of the tile's r32 and p_old windows
+    __device__ __forceinline__ void park(float* lr, float* lp) const
+    {
+#pragma unroll
+        for (int u = 0; u < NXW; ++u) {
+            const int s = (int)threadIdx.x + u * kSpmvBS;
+            if (s < total) {
+                lr[s] = wr[u];
+                lp[s] = wp[u];
+            }
+        }
+    }
+    __device__ __forceinline__ void issue(int tile, const int32_t* __restrict__ recs,
+                                          const uint8_t* __restrict__ lens,
+                                          const uint16_t* __restrict__ code,
+                                          const uint8_t* __restrict__ pid,
+                                          const float* __restrict__ v32,
+                                          const TileEntry* __restrict__ sched,
+                                          const float* __restrict__ r32,
+                                          const float* __restrict__ p_old)
+    {
+        constexpr int BS = kSpmvBS;
+        const int t = threadIdx.x;
+        const int32_t* rec = recs + tile * kXwRec;
+        w = xw_win(rec);
+        vb = (int64_t)rec[kXwBase] * BS;
+        width = rec[kXwWidth];
+        total = w.e3;
+        r0 = sched[tile].row;
+        nr = sched[tile + 1].row - r0;
+        const bool active = t < nr;
+        const int n2 = width >> 1;
+        const f2v* __restrict__ vp = reinterpret_cast<const f2v*>(v32 + vb);
+#pragma unroll
+        for (int h = 0; h < kSellW / 2; ++h)
+            if (h < n2) v[h] = ld_mat<NT32>(vp + h * BS + t);
+        vl = (width & 1) ? ld_mat<NT32>(v32 + vb + 2 * n2 * BS + t) : 0.0f;
+        q0 = q1 = 0;
+        id = len = 0;
+        if constexpr (PAT) {
+            id = active ? ((int)ld_mat<NT32>(pid + tile * (BS / 2) + (t >> 1)) >> (4 * (t & 1))) &
+                              15
+                        : 0;
+        } else {
+            len = active ? (int)lens[r0 + t] : 0;
+            sell_slots<NT32, BS>(code + vb, width, t, q0, q1);
+        }
+        const int sm = max(total - 1, 0);
+#pragma unroll
+        for (int u = 0; u < NXW; ++u) {
+            const int col = xw_win_col(min(t + u * BS, sm), w);
+            wr[u] = r32[col];
+            wp[u] = p_old[col];
+        }
+    }
+    // after the barrier: the row's slots and length from the staged table
+    __device__ __forceinline__ void resolve(const uint64_t* lpat)
+    {
+        if constexpr (PAT) {
+            q0 = lpat[2 * id];
+            q1 = lpat[2 * id + 1];
+            len = (int)threadIdx.x < nr ? (int)((q0 >> 12) & 15) : 0;
+        }
+    }
+    __device__ __forceinline__ int slot(int k) const
+    {
+        if constexpr (PAT) return (sell_slot(k, q0, q1) & 0xfff) - kXwPatBias + (int)threadIdx.x;
+        else return sell_slot(k, q0, q1);
+    }
+    __device__ __forceinline__ float value(int k) const
+    {
+        return k < 2 * (width >> 1) ? ((k & 1) ? v[k / 2].y : v[k / 2].x) : vl;
+    }
+};
+
 // SpMV kernel of the mixed slot over the windowed sliced layout (the FP32
-// and FP64 values in sliced order, one window-slot code per entry shared by
-// both; see spmv_sell_run): one workgroup per schedule tile, partials one
-// per tile.  Every load is issued before the update kernel's partials are
-// summed: the FP32 values and slots of the thread's row, r32 and p_old over
-// the tile's windows (p_old's buffer does not depend on the sum) and the
-// partials; after the fold, p_new = r + beta*p_old goes into the windows in
-// LDS and each row is summed from registers and LDS (same values, same
-// order as the staged kernels: the same bits per row).  A refinement
-// (ru) walks the same slots with the FP64 values and x64 + x32 windows.
-template <bool NT32, bool NT64>
-__global__ __launch_bounds__(kSpmvBS, 8) void mix_sell_kernel(
+// and FP64 values in sliced order; the row patterns, or one window-slot
+// code per entry, shared by both; see spmv_sell_run), for a PAIR of
+// schedule tiles per workgroup (thread t takes row t of each).  FP32 step:
+// everything that does not depend on the fold is issued before its barrier
+// -- both tiles' FP32 values and patterns (or slots), and r32 and p_old
+// over their windows (parked in LDS) -- and the update kernel's partials
+// are folded once per pair; after it, p_new = r + beta*p_old is formed in
+// place over the windows and each row is summed from registers and LDS in
+// index order (the same bits per row as the staged kernels), p_new of the
+// own row taken from the windows.  One tile per workgroup instead ran 2-10
+// % slower on every grid measured (its step is bound by its latency chain,
+// not its bytes: 78 us per 192^3 step with 2-byte slots or row patterns
+// alike).  A refinement (ru) walks the two tiles in turn with the FP64
+// values over x64 + x32 windows.  Partials: one per pair.
+template <bool NT32, bool NT64, bool PAT>
+__global__ __launch_bounds__(kSpmvBS, 7) void mix_sell2_kernel(
     const int32_t* __restrict__ recs, const uint8_t* __restrict__ lens,
-    const uint16_t* __restrict__ code, const float* __restrict__ v32,
-    const double* __restrict__ v64, const TileEntry* __restrict__ sched, int ntiles, int chunk,
-    MixArgs a)
+    const uint16_t* __restrict__ code, const uint64_t* __restrict__ pat,
+    const int32_t* __restrict__ tix, const uint8_t* __restrict__ pid,
+    const float* __restrict__ v32, const double* __restrict__ v64,
+    const TileEntry* __restrict__ sched, int ntiles, int npairs, int chunk, MixArgs a)
 {
     constexpr int BS = kSpmvBS;
-    constexpr int NXW = (kXwSlots + BS - 1) / BS;
-    __shared__ double xw64[kXwSlots];
+    constexpr int NXW = Sell2Tile<NT32, PAT>::NXW;
+    constexpr int K = kXwSlots;
+    // FP32 step: [r32 | p_old] windows of tile A, then of tile B, as floats
+    // (the r32 part becomes p_new); a refinement: one tile's FP64 windows
+    __shared__ double xw64[2 * K];
     __shared__ double red[2 * BS / kWave];
+    __shared__ uint64_t lpat[PAT ? 4 * kXwPat : 1];
     float* xw32 = reinterpret_cast<float*>(xw64);
-    const int tile = chunk_tile(blockIdx.x, ntiles, chunk);
-    if (tile < 0) return;
+    const int p = chunk_tile(blockIdx.x, npairs, chunk);
+    if (p < 0) return;
     const int t = threadIdx.x;
-    const int32_t* rec = recs + tile * kXwRec;
-    NUMAMD_XW_LOAD(rec);
-    const int64_t vb = (int64_t)rec[kXwBase] * BS;
-    const int width = rec[kXwWidth];
-    const int total = we3;
-    const int r0 = sched[tile].row, nr = sched[tile + 1].row - r0;
-    const bool active = t < nr;
-    const int len = active ? (int)lens[r0 + t] : 0;
-    // the row's FP32 values, packed in pairs (one 8-byte load each, then a
-    // single slice for an odd width), and its slots, packed; uniform
-    // branches on the tile's width
-    typedef float f2v __attribute__((ext_vector_type(2)));
-    const int n2 = width >> 1;
-    const f2v* __restrict__ vp = reinterpret_cast<const f2v*>(v32 + vb);
-    f2v v[kSellW / 2];
-#pragma unroll
-    for (int h = 0; h < kSellW / 2; ++h)
-        if (h < n2) v[h] = ld_mat<NT32>(vp + h * BS + t);
-    const float vl = (width & 1) ? ld_mat<NT32>(v32 + vb + 2 * n2 * BS + t) : 0.0f;
-    uint64_t q0, q1;
-    sell_slots<NT32, BS>(code + vb, width, t, q0, q1);
+    const int ta = 2 * p;
+    const bool two = ta + 1 < ntiles;  // uniform: the last pair may hold one tile
     const float* __restrict__ p_old = mix_next_psel(*a.sa) ? a.p1 : a.p0;
-    const int sm = max(total - 1, 0);
-    float wr[NXW], wp[NXW];
-#pragma unroll
-    for (int u = 0; u < NXW; ++u) {
-        const int col = xw_col(min(t + u * BS, sm), wl0, wl1, wl2, wl3, we0, we1, we2);
-        wr[u] = a.r32[col];
-        wp[u] = p_old[col];
+    Sell2Tile<NT32, PAT> A, B;
+    A.issue(ta, recs, lens, code, pid, v32, sched, a.r32, p_old);
+    if (two) B.issue(ta + 1, recs, lens, code, pid, v32, sched, a.r32, p_old);
+    if constexpr (PAT) {
+        if (t < 2 * kXwPat) {
+            lpat[t] = pat[(size_t)tix[ta] * 2 * kXwPat + t];
+            if (two) lpat[2 * kXwPat + t] = pat[(size_t)tix[ta + 1] * 2 * kXwPat + t];
+        }
     }
     GatherMix g{a.r32, a.p0, a.p1, a.sa, a.sb, a.pb, a.npb};
     g.issue();
+    A.park(xw32, xw32 + K);
+    if (two) B.park(xw32 + 2 * K, xw32 + 3 * K);
     g.resolve();  // a barrier; workgroup 0 publishes the folded state
     if (g.done) return;
-    const int row = r0 + t;
+    A.resolve(lpat);
+    if (two) B.resolve(lpat + 2 * kXwPat);
     if (g.ru) {
+        // FP64 refinement, the tiles in turn over the FP64 windows
         const int xs = g.xsel;
         GatherXsum gx{xs ? a.x64b : a.x64a, xs ? a.x32b : a.x32a};
         EpiRefine epi{a.b, xs ? a.x64a : a.x64b, xs ? a.x32a : a.x32b, a.r32, nullptr, nullptr};
-        double w64[kSellW];
-#pragma unroll
-        for (int k = 0; k < kSellW; ++k)
-            if (k < width) w64[k] = ld_mat<NT64>(v64 + vb + k * BS + t);
-#pragma unroll
-        for (int u = 0; u < NXW; ++u) {
-            const int s = t + u * BS;
-            if (s < total) xw64[s] = gx(xw_col(s, wl0, wl1, wl2, wl3, we0, we1, we2));
-        }
-        __syncthreads();
         double ca = 0.0, cb = 0.0;
-        if (active) {
-            double acc = 0.0;
+        for (int h = 0; h < (two ? 2 : 1); ++h) {
+            const Sell2Tile<NT32, PAT>& T = h ? B : A;
+            double w64[kSellW];
 #pragma unroll
             for (int k = 0; k < kSellW; ++k)
-                if (k < len) acc += w64[k] * xw64[sell_slot(k, q0, q1)];
-            epi(row, acc, ca, cb, gx);
+                if (k < T.width) w64[k] = ld_mat<NT64>(v64 + T.vb + k * BS + t);
+            __syncthreads();  // the LDS is filled again for this tile
+#pragma unroll
+            for (int u = 0; u < NXW; ++u) {
+                const int s = t + u * BS;
+                if (s < T.total) xw64[s] = gx(xw_win_col(s, T.w));
+            }
+            __syncthreads();
+            if (t < T.nr) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < kSellW; ++k)
+                    if (k < T.len) acc += w64[k] * xw64[T.slot(k)];
+                double xa, xb;
+                epi(T.r0 + t, acc, xa, xb, gx);
+                ca += xa;
+                cb += xb;
+            }
         }
         block_sum2<BS>(ca, cb, red);
         if (t == 0) {
-            a.pa[tile] = ca;
-            a.pc[tile] = cb;
+            a.pa[p] = ca;
+            a.pc[p] = cb;
         }
         return;
     }
     const float beta = g.beta;
+    // p_new = r + beta * p_old over each window slot, in place of r (each
+    // thread the slots it parked)
 #pragma unroll
     for (int u = 0; u < NXW; ++u) {
         const int s = t + u * BS;
-        if (s < total) xw32[s] = wr[u] + beta * wp[u];
+        if (s < A.total) xw32[s] = xw32[s] + beta * xw32[K + s];
+        if (two && s < B.total) xw32[2 * K + s] = xw32[2 * K + s] + beta * xw32[3 * K + s];
     }
     __syncthreads();
     double ca = 0.0;
-    if (active) {
-        float acc = 0.0f;
+    for (int h = 0; h < (two ? 2 : 1); ++h) {
+        const Sell2Tile<NT32, PAT>& T = h ? B : A;
+        const float* xwt = xw32 + 2 * h * K;
+        if (t < T.nr) {
+            float acc = 0.0f;
 #pragma unroll
-        for (int k = 0; k < kSellW; ++k)
-            if (k < len) {
-                const float vk = k < 2 * n2 ? ((k & 1) ? v[k / 2].y : v[k / 2].x) : vl;
-                acc += vk * xw32[sell_slot(k, q0, q1)];
-            }
-        // p_new of the own row: a window slot (the own rows are a window
-        // range of every tile of a square matrix)
-        const int os = xw_slot(row, wl0, wl1, wl2, wl3, we0, we1, we2, we3);
-        const float pn = os >= 0 ? xw32[os] : g(row);
-        __builtin_nontemporal_store(pn, (g.psel ? a.p0 : a.p1) + row);
-        __builtin_nontemporal_store(acc, a.q + row);
-        ca = (double)pn * (double)acc;
+            for (int k = 0; k < kSellW; ++k)
+                if (k < T.len) acc += T.value(k) * xwt[T.slot(k)];
+            // p_new of the own row: a window slot (the own rows are a
+            // window range of every tile of a square matrix)
+            const int row = T.r0 + t;
+            const int os = xw_slot(row, T.w.l0, T.w.l1, T.w.l2, T.w.l3, T.w.e0, T.w.e1, T.w.e2,
+                                   T.w.e3);
+            const float pn = os >= 0 ? xwt[os] : g(row);
+            __builtin_nontemporal_store(pn, (g.psel ? a.p0 : a.p1) + row);
+            __builtin_nontemporal_store(acc, a.q + row);
+            ca += (double)pn * (double)acc;
+        }
     }
-    const double s = block_sum<BS>(ca, red);
-    if (t == 0) a.pa[tile] = s;
+    const double sum = block_sum<BS>(ca, red);
+    if (t == 0) a.pa[p] = sum;
 }
 
 __global__ void mix_init(const double* __restrict__ bb, MixState* st, double tol, double delta,
@@ -1146,14 +1260,12 @@ static bool mix_paired(const num_csr* A)
 
 // The windowed sliced path applies (by default) when the layout carries the
 // FP32 values, the FP32 matrix streams from HBM and its dominant long-range
-// column offset is at least kMixSellOffset rows.  Per FP32 step, windowed
-// against the staged pair kernel: 192^3 (offset 36,864) 127 against 152 us,
-// 96x96x1200 (9,216) 200 against 191 us, 2-D 4000^2 (4,000) 262 against 235
-// us.  An FP32 matrix that stays in the Infinity Cache runs the pair kernel:
-// two tiles per workgroup halve the per-workgroup fold of the update
-// kernel's partials, which a cache-resident step pays relatively more for
-// (128^3 diffusion: 91.8 ms against 97.9 ms).  num_csr_set_mixed_path
-// overrides the choice.
+// column offset is at least kMixSellOffset rows.  Per FP32 step (both with
+// row patterns), windowed against the staged pair kernel: 192^3 (offset
+// 36,864) 116 against 144 us, 96x96x1200 (9,216) 180 against 157 us, 2-D
+// 4000^2 (4,000) 232 against 206 us.  An FP32 matrix that stays in the
+// Infinity Cache runs the staged pair kernel (128^3 diffusion: 35.0 against
+// 39.4 us).  num_csr_set_mixed_path overrides the choice.
 constexpr int32_t kMixSellOffset = 16384;
 static bool mix_sliced(const num_csr* A)
 {
@@ -1162,22 +1274,47 @@ static bool mix_sliced(const num_csr* A)
     return A->sched.nt32 != 0 && A->offset_d >= kMixSellOffset;
 }
 
+// SpMV-kernel partials the update kernel sums: one per pair for the paired
+// kernels (windowed or staged), else one per tile.
+static int mix_partials(const num_csr* A)
+{
+    const int nb = A->sched.nblocks;
+    return mix_sliced(A) || mix_paired(A) ? (nb + 1) / 2 : nb;
+}
+
 static void launch_mix_spmv(const num_csr* A, const MixArgs& a, hipStream_t st)
 {
     if (mix_sliced(A)) {
         const Schedule& s = A->sched;
-        const int grid = chunk_grid(s.nblocks, s.chunk);
+        const int npairs = (s.nblocks + 1) / 2;
+        // plane-aligned pair chunks, as the staged pair kernel
+        const int pc = plane_chunk(A, 2.0 * A->nrows / std::max(1, s.nblocks));
+        const int chunk = pc >= 2 && npairs >= 8 * pc ? pc : tile_chunk(npairs);
+        const int grid = chunk_grid(npairs, chunk);
         const TileEntry* tiles = static_cast<const TileEntry*>(s.d_tiles);
         auto go = [&](auto k) {
-            k<<<grid, kSpmvBS, 0, st>>>(A->xw, A->sell_lens, A->sell_codes, A->sell_vals32,
-                                        A->sell_vals, tiles, s.nblocks, s.chunk, a);
+            k<<<grid, kSpmvBS, 0, st>>>(A->xw, A->sell_lens, A->sell_codes, A->sell_pat,
+                                        A->sell_tix, A->sell_pid, A->sell_vals32, A->sell_vals,
+                                        tiles, s.nblocks, npairs, chunk, a);
         };
-        if (s.nt32) {
-            if (s.nt64) go(mix_sell_kernel<true, true>);
-            else go(mix_sell_kernel<true, false>);
+        // the row patterns of the FP64 layout serve the FP32 values too
+        // (same tiles, same slots)
+        if (A->sell_pat) {
+            if (s.nt32) {
+                if (s.nt64) go(mix_sell2_kernel<true, true, true>);
+                else go(mix_sell2_kernel<true, false, true>);
+            } else {
+                if (s.nt64) go(mix_sell2_kernel<false, true, true>);
+                else go(mix_sell2_kernel<false, false, true>);
+            }
         } else {
-            if (s.nt64) go(mix_sell_kernel<false, true>);
-            else go(mix_sell_kernel<false, false>);
+            if (s.nt32) {
+                if (s.nt64) go(mix_sell2_kernel<true, true, false>);
+                else go(mix_sell2_kernel<true, false, false>);
+            } else {
+                if (s.nt64) go(mix_sell2_kernel<false, true, false>);
+                else go(mix_sell2_kernel<false, false, false>);
+            }
         }
         return;
     }
@@ -1329,8 +1466,7 @@ int num_cg_solve_mixed(const num_csr* A, const double* d_b, double* d_x, double 
     if (rc) return rc;
     mix_init<<<1, 64, 0, st>>>(bb, a.sa, tol, delta, max_iter);
     NUM_LAUNCH_CHECK();
-    const int np_spmv =
-        !mix_sliced(A) && mix_paired(A) ? (A->sched.nblocks + 1) / 2 : A->sched.nblocks;
+    const int np_spmv = mix_partials(A);
     auto slot = [&]() -> int {
         launch_mix_spmv(A, a, st);
         mix_kc<<<gk, kFin, 0, st>>>(n, np_spmv, a);

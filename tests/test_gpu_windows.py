@@ -281,6 +281,16 @@ def test_mixed_windows_streamed(ks, orc, name):
         assert true < 1e-10 and abs(true - r["residual"]) <= 1e-12 * r["residual"] + 1e-16
         assert _rel(xh, x64) < 1e-7
         steps[nt32] = r["inner_total"]
+        if nt32:
+            # the windowed step over the row patterns and over a slot per
+            # entry: the same bits
+            assert dA.patterns["windows"]
+            dA.set_windows(True, patterns=False)
+            dA.set_schedule(nt32=1)
+            assert not dA.patterns["windows"]
+            x2 = _dev(np.zeros(A.nrows))
+            r2 = ks.cg_solve_mixed(dA, _dev(b), x2, tol=1e-10)
+            assert r2 == r and np.array_equal(x2.cpu().numpy(), xh)
     assert abs(steps[1] - steps[0]) <= max(2, 0.02 * steps[0]), steps
 
 
