@@ -139,8 +139,9 @@ int num_csr_index_info(const num_csr* A, int32_t* delta16);
  * order -- straight into registers.  Same entries in the same order:
  * bit-identical results.  Where every tile holds at most 16 distinct row
  * patterns (a row's slots minus its position in the tile: a stencil's
- * interior and boundary rows), each row keeps one pattern id byte into its
- * tile's table instead of a 2-byte slot per entry.  mode 1 builds it (when
+ * interior and boundary rows), each row keeps a 4-bit pattern id into its
+ * tile's table (tiles with equal tables share one) instead of a 2-byte slot
+ * per entry and a length byte.  mode 1 builds it (when
  * the matrix fits), 2 builds it with a slot per entry only, 0 drops
  * it, -1 (the default, applied at num_csr_attach_d16) builds it for
  * matrices of at least 65,536 rows (below that an SpMV is launch-latency

@@ -96,9 +96,10 @@ constexpr int kXwRec = 2 * kXwWin + 2;
 // in the tile (plus kXwPatBias, as uint16; 0 past the row's length) are the
 // same for every row of a tile that reads the same column offsets (a
 // stencil's interior rows), so a tile keeps a table of at most kXwPat
-// distinct patterns, two uint64 of four 16-bit slots each, and one id byte
-// per row replaces the row's 2-byte slots and its length byte: the length
-// sits in bits 12-15 of the first slot (biased slots stay below 4096).
+// distinct patterns, two uint64 of four 16-bit slots each (tiles with equal
+// tables share one), and a 4-bit id per row replaces the row's 2-byte slots
+// and its length byte: the length sits in bits 12-15 of the first slot
+// (biased slots stay below 4096).
 constexpr int kXwPat = 16;
 constexpr int kXwPatBias = 256;
 static_assert(kXwSlots + kXwPatBias <= 4096 && kSellW < 16, "length field of a row pattern");

@@ -176,8 +176,9 @@ class CrsMatrix:
     def index_format(self) -> str:
         """What the FP64 SpMV / CG kernels read for the columns.
         'windowpat': the windowed sliced layout with row patterns (every
-        tile reads x from a few column ranges fetched into LDS; each row one
-        id byte into its tile's table of slot patterns; rows <= 8 entries).
+        tile reads x from a few column ranges fetched into LDS; each row a
+        4-bit id into its tile's table of slot patterns, tables shared by
+        the tiles that have equal ones; rows <= 8 entries).
         'window16': the same layout with a 16-bit slot per entry (a tile with
         more than 16 distinct row patterns, or set_windows(patterns=False)).
         'delta16': the delta-coded column ids
