@@ -165,6 +165,12 @@ def test_pipelined_launch_equals_step(ks, mixed):
         out.append([(eng.result(s)["iters"], eng.solution(s).clone()) for s in slots])
     for (i0, x0), (i1, x1) in zip(*out):
         assert i0 == i1 and torch.equal(x0, x1)
+    # every solution, tiny grids included (1-D and single-row systems read
+    # the neighbour-pattern table's short patterns), against the stencil's
+    # closed form, independent of the library
+    from _helpers import stencil_residual
+    for i, ((d, sh), (_, x)) in enumerate(zip(grids, out[0])):
+        assert stencil_residual(x, d, sh, 10 + i) < 1e-10, d
 
 
 @pytest.mark.parametrize("n,seed", [(0, 1), (1, 7), (1000, 12345), (300_001, 2**40 + 3)])
