@@ -153,11 +153,13 @@ int num_csr_set_windows(num_csr* A, int32_t mode, void* stream);
  * a slot per entry, 0 if not built. */
 int num_csr_window_info(const num_csr* A, int32_t* windowed);
 /* Which kernel takes the FP32 steps of num_cg_solve_mixed (no reference
- * counterpart): mode 1 the windowed sliced step (when the windowed layout
- * and the FP32 values are attached), 0 the staged row-tile / pair kernels
- * (the pair kernel reading the int16 offsets' row patterns where they were
- * built: one id byte per row into a table of at most 16 offset patterns per
- * pair of tiles), 2 the staged kernels with 2 offset bytes per entry, -1
+ * counterpart): mode 1 the windowed sliced step, two tiles per workgroup
+ * over the windowed layout's row patterns or slots (when the windowed
+ * layout and the FP32 values are attached), 0 the staged row-tile / pair
+ * kernels (the pair kernel reading the int16 offsets' row patterns where
+ * they were built: one id byte per row into a table of at most 16 offset
+ * patterns and lengths per pair of tiles, no rowptr), 2 the staged kernels
+ * with 2 offset bytes per entry, -1
  * (the default) the windowed step when the FP32 copy streams from HBM and
  * the matrix's dominant long-range column offset is at least 16,384 (a 3-D
  * grid plane of that many rows; on 2-D and short-plane grids the staged
