@@ -294,6 +294,33 @@ def test_mixed_windows_streamed(ks, orc, name):
     assert abs(steps[1] - steps[0]) <= max(2, 0.02 * steps[0]), steps
 
 
+@pytest.mark.parametrize("rpb", [100, 256])
+def test_mixed_windows_odd_tiles(ks, orc, rpb):
+    # the windowed FP32 step takes two tiles per workgroup: tiles of 100
+    # rows (the pattern ids of lanes past a tile's rows unused) and a tile
+    # count that leaves the last pair one tile, with patterns and with slots
+    A = g.stencil((31, 29, 23), 0.01)
+    b = np.random.default_rng(23).uniform(-1, 1, A.nrows)
+    x64, _ = orc.cg_solve_local(A.rowptr, A.colind, A.values, b, tol=1e-12, max_iter=20000)
+    out = []
+    for pat in (True, False):
+        dA = _up(ks, A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        dA.ensure_f32()
+        dA.set_windows(True, patterns=pat)
+        dA.set_schedule(rows_per_block=rpb, nt32=1)
+        dA.set_mixed_path(1)
+        nb = dA.schedule_info()["nblocks"]
+        assert dA.windowed and dA.patterns["windows"] == pat and nb % 2 == 1, nb
+        x = _dev(np.zeros(A.nrows))
+        r = ks.cg_solve_mixed(dA, _dev(b), x, tol=1e-10)
+        xh = x.cpu().numpy()
+        assert r["converged"]
+        assert np.linalg.norm(b - A.to_scipy() @ xh) / np.linalg.norm(b) < 1e-10
+        assert _rel(xh, x64) < 1e-7
+        out.append((r, xh))
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+
+
 def test_windows_default_from_65536_rows(ks):
     # built by default for matrices of at least 65,536 rows only
     small = g.laplacian_3d(40)   # 64,000 rows
