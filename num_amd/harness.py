@@ -5,6 +5,7 @@
                                    [--backend scipy num_amd num_amd_custom] [--no-csv]
     python -m num_amd.harness cg   [--N 100000] [--matrix laplacian_3d]
                                    [--tol 1e-8] [--maxiter 2000] [--no-csv] [--mixed]
+    python -m num_amd.harness verify [--ranks 4]
 
 They mirror python/benchmark_spmv.py:294-377 and python/benchmark_cg.py:
 278-363: the same matrix families (generators.get_matrix), the same x
@@ -17,6 +18,16 @@ GPU times are the median of HIP-event pairs on the stream the kernels run on
 scipy_cpu / scipy_cg (CPU), num_amd_1gpu and num_amd_custom_1gpu (spmv with
 kokkoskernels True / False: the same kernel), num_amd_batch_k<K>,
 num_amd_cg_1gpu, num_amd_cg_mixed_1gpu, spmv_only_1gpu.
+
+``verify`` runs the four check groups of python/verify_accuracy.py:67-216
+against SciPy with the same matrices, sizes, seeds and bars: single SpMV
+(N = 50k, four families, < 1e-10), batch SpMV (N = 20k, k = 64, < 1e-10),
+local CG (N = 10k, b = 1, tol 1e-10, 2000 iterations, solution within 1e-6
+of SciPy's CG at rtol 1e-12) and the distributed SpMV (N = 40k, < 1e-10).
+The reference runs the last group under MPI with two or more ranks; on one
+GPU every rank's share runs here in turn -- its halo packed by its
+neighbours' shares and scattered into its ghost block -- and the gathered
+rows are checked.  Exit status 0 when every group passes.
 """
 from __future__ import annotations
 
@@ -197,6 +208,100 @@ def run_cg(args) -> list:
     return rows
 
 
+def _rel_err(a, b) -> float:
+    """||a - b|| / ||b|| (||a|| when b is zero), python/verify_accuracy.py:45-50."""
+    d = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) if d == 0 else float(np.linalg.norm(a - b) / d)
+
+
+def _verdict(label: str, err: float, tol: float = 1e-10) -> bool:
+    ok = err < tol
+    print(f"  {'PASS' if ok else 'FAIL'}  {label:56s}  rel_err = {err:.3e}"
+          + ("" if ok else f"  (tol={tol:.0e})"))
+    return ok
+
+
+def run_verify(args) -> bool:
+    """The check groups of python/verify_accuracy.py on the HIP path."""
+    import scipy.sparse.linalg as spla
+    import torch
+    from . import generators as g
+    from . import kokkos_spmv as ks
+    ks.init(0)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    groups = []
+
+    print("\n[1] Single-GPU SpMV")
+    ok = True
+    for name in ("laplacian_2d", "laplacian_3d", "random_sparse", "lattice_gauge"):
+        A = g.get_matrix(name, 50_000)
+        x = np.random.default_rng(0).standard_normal(A.ncols)
+        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        y = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+        ks.spmv(dA, dev(x), y)
+        ok &= _verdict(f"spmv  {name:20s} N={A.nrows:>7,}",
+                       _rel_err(y.cpu().numpy(), A.to_scipy() @ x))
+    groups.append(ok)
+
+    print("\n[2] Batch SpMV")
+    ok = True
+    for name in ("laplacian_2d", "random_sparse"):
+        A = g.get_matrix(name, 20_000)
+        X = np.random.default_rng(1).standard_normal((A.ncols, 64))
+        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        Y = torch.zeros((A.nrows, 64), dtype=torch.float64, device="cuda")
+        ks.batch_spmv(dA, dev(X), Y)
+        ok &= _verdict(f"batch_spmv k= 64  {name:20s} N={A.nrows:>7,}",
+                       _rel_err(Y.cpu().numpy(), A.to_scipy() @ X))
+    groups.append(ok)
+
+    print("\n[3] Local CG solver")
+    ok = True
+    for name in ("laplacian_3d", "random_sparse"):
+        A = g.get_matrix(name, 10_000)
+        b = np.ones(A.nrows)
+        x_ref, info = spla.cg(A.to_scipy(), b, rtol=1e-12, maxiter=2000)
+        if info != 0:
+            print(f"  [WARN] SciPy CG failed (info={info}) for {name}")
+        dA = ks.upload_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols)
+        x = torch.zeros(A.nrows, dtype=torch.float64, device="cuda")
+        r = ks.cg_solve_local(dA, dev(b), x, tol=1e-10, max_iter=2000)
+        ok &= _verdict(f"cg_local  {name:20s} N={A.nrows:>7,}  iters={r['iters']:4d}  "
+                       f"converged={r['converged']}", _rel_err(x.cpu().numpy(), x_ref), 1e-6)
+    groups.append(ok)
+
+    print("\n[4] Distributed SpMV")
+    ok = True
+    size = int(args.ranks)
+    for name in ("laplacian_2d", "laplacian_3d"):
+        A = g.get_matrix(name, 40_000)
+        x = np.random.default_rng(2).standard_normal(A.ncols)
+        D = [ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=r, size=size)
+             for r in range(size)]
+        bufs = [ks.halo_pack(D[r], dev(x[D[r].local_row_start:D[r].local_row_end]))
+                for r in range(size)]
+        y = np.empty(A.nrows)
+        for q in range(size):
+            xg = torch.zeros(D[q].n_ghost, dtype=torch.float64, device="cuda")
+            for src, _, _ in D[q].recv:
+                off, cnt = next((o, c) for dst, o, c in ks.send_ranges(D[src]) if dst == q)
+                ks.halo_scatter(D[q], bufs[src][off:off + cnt].contiguous(), xg, src_rank=src)
+            s, e = D[q].local_row_start, D[q].local_row_end
+            yl = torch.empty(e - s, dtype=torch.float64, device="cuda")
+            ks.dist_spmv(D[q], dev(x[s:e]), yl, xg)
+            y[s:e] = yl.cpu().numpy()
+        ok &= _verdict(f"dist_spmv  {name:20s} N={A.nrows:>7,}  np={size}",
+                       _rel_err(y, A.to_scipy() @ x))
+    groups.append(ok)
+
+    n_pass = sum(groups)
+    print("\n" + "=" * 72)
+    print(f"  ALL {len(groups)} test groups PASSED" if all(groups)
+          else f"  {n_pass}/{len(groups)} test groups passed - see failures above")
+    print("=" * 72)
+    return all(groups)
+
+
 def main(argv=None) -> int:
     from .generators import MATRIX_FAMILIES
     ap = argparse.ArgumentParser(description="num_amd benchmark harness")
@@ -217,7 +322,12 @@ def main(argv=None) -> int:
     cg_.add_argument("--mixed", action="store_true", help="also run the mixed refined solver")
     cg_.add_argument("--no-scipy", action="store_true")
     cg_.add_argument("--no-csv", action="store_true")
+    vf_ = sub.add_parser("verify", help="accuracy checks (python/verify_accuracy.py)")
+    vf_.add_argument("--ranks", type=int, default=4,
+                     help="shares of the distributed SpMV check, run in turn on one GPU")
     args = ap.parse_args(argv)
+    if args.cmd == "verify":
+        return 0 if run_verify(args) else 1
     rows = run_spmv(args) if args.cmd == "spmv" else run_cg(args)
     if not args.no_csv and rows:
         print(f"\n  Results saved to {_write_csv(args.cmd, rows)}")

@@ -34,3 +34,13 @@ def test_harness_cg(ks, tmp_path, monkeypatch, matrix):
     # same CG sequence as scipy's unpreconditioned CG: iteration counts close
     assert abs(int(rows["num_amd_cg_1gpu"]["iters"]) - int(rows["scipy_cg"]["iters"])) <= \
         max(2, 0.05 * int(rows["scipy_cg"]["iters"]))
+
+
+def test_harness_verify(ks, capsys):
+    # the check groups of python/verify_accuracy.py (single SpMV, batch SpMV,
+    # local CG against SciPy, the distributed SpMV over 3 shares run in turn)
+    from num_amd import harness
+    assert harness.main(["verify", "--ranks", "3"]) == 0
+    out = capsys.readouterr().out
+    assert "ALL 4 test groups PASSED" in out and "FAIL" not in out
+    assert out.count("PASS") == 4 + 2 + 2 + 2 + 1
