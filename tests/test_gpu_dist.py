@@ -164,6 +164,55 @@ def test_halo_scatter_checks(ks):
                         torch.zeros(3, dtype=torch.float64, device="cuda"), src_rank=0)
 
 
+def _shares_spmv(ks, D, v, out, xg):
+    """out[q] = share q's rows of A v on one GPU: every share packs its halo
+    (halo_pack), each neighbour's part is scattered into share q's ghost
+    block (halo_scatter), then share q's SpMV (dist_spmv); the transport
+    between pack and scatter is the part a second GPU would carry."""
+    size = len(D)
+    bufs = [ks.halo_pack(D[s], v[s]) for s in range(size)]
+    for q in range(size):
+        for src, _, _ in D[q].recv:
+            off, cnt = next((o, c) for dst, o, c in ks.send_ranges(D[src]) if dst == q)
+            ks.halo_scatter(D[q], bufs[src][off:off + cnt].contiguous(), xg[q], src_rank=src)
+        ks.dist_spmv(D[q], v[q], out[q], xg[q])
+
+
+@pytest.mark.parametrize("size", [2, 3, 4])
+def test_shares_repeated_zero_and_basis(ks, size):
+    """tests/test_halo_exchange.py:263-400 on one GPU: repeated distributed
+    SpMVs are bit-identical, x = 0 gives y = 0 exactly, and x = e_j gives
+    column j of the 1-D Laplacian for the columns at the partition
+    boundaries."""
+    def setup(N):
+        A = g.laplacian_1d(N)
+        D = [ks.distribute_csr(A.rowptr, A.colind, A.values, N, N, rank=r, size=size)
+             for r in range(size)]
+        xg = [torch.zeros(d.n_ghost, dtype=torch.float64, device="cuda") for d in D]
+        return A, D, xg
+
+    def run(D, xg, xglob):
+        v = [_dev(xglob[d.local_row_start:d.local_row_end]) for d in D]
+        out = [torch.full((d.local_row_end - d.local_row_start,), float("nan"),
+                          dtype=torch.float64, device="cuda") for d in D]
+        _shares_spmv(ks, D, v, out, xg)
+        return np.concatenate([o.cpu().numpy() for o in out])
+
+    A, D, xg = setup(100)
+    x = np.random.default_rng(42).standard_normal(100)
+    ys = [run(D, xg, x) for _ in range(5)]
+    assert all(np.array_equal(y, ys[0]) for y in ys[1:])
+    assert np.array_equal(ys[0], A.to_scipy() @ x)
+    A, D, xg = setup(64)
+    assert np.all(run(D, xg, np.zeros(64)) == 0.0)
+    A, D, xg = setup(60)
+    dense = A.to_scipy().toarray()
+    for col in sorted({0, 60 // size - 1, 60 // size, 59}):
+        e = np.zeros(60)
+        e[col] = 1.0
+        assert np.array_equal(run(D, xg, e), dense[:, col]), col
+
+
 @pytest.mark.parametrize("size", [2, 4])
 def test_distributed_cg_over_shares(ks, orc, size):
     """tests/test_cg.cpp [8] on one GPU: the distributed CG of
@@ -186,13 +235,7 @@ def test_distributed_cg_over_shares(ks, orc, size):
     xg = [vec(d.n_ghost) for d in D]
 
     def dist_spmv(v, out):
-        bufs = [ks.halo_pack(D[s], v[s]) for s in range(size)]
-        for q in range(size):
-            for src, _, _ in D[q].recv:
-                off, cnt = next((o, c) for dst, o, c in ks.send_ranges(D[src]) if dst == q)
-                ks.halo_scatter(D[q], bufs[src][off:off + cnt].contiguous(), xg[q],
-                                src_rank=src)
-            ks.dist_spmv(D[q], v[q], out[q], xg[q])
+        _shares_spmv(ks, D, v, out, xg)
 
     gdot = lambda u, v: sum(ks.dot(u[s], v[s]) for s in range(size))
     dist_spmv(x, Ap)
