@@ -117,3 +117,21 @@ def test_distributed_spmv_40k_gathered(ks, orc, name, size):
     y = np.concatenate(parts)
     assert _rel_err(y, A.to_scipy() @ x) < 1e-10
     assert np.array_equal(y, y_or)
+
+
+@pytest.mark.parametrize("size", [2, 4])
+def test_distributed_cg_20k_vs_scipy(ks, size):
+    """tests/test_accuracy.py:265-322 on one GPU: the distributed CG over
+    every share in turn (laplacian_3d, N = 20k, b = 1, tol 1e-10, 5000
+    iterations) against SciPy's CG at rtol 1e-12, solution within 1e-6."""
+    import scipy.sparse.linalg as spla
+    from _helpers import shares_cg
+    A = g.get_matrix("laplacian_3d", 20_000)
+    b = np.ones(A.nrows)
+    x_ref, info = spla.cg(A.to_scipy(), b, rtol=1e-12, maxiter=5000)
+    assert info == 0
+    D = [ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=r, size=size)
+         for r in range(size)]
+    x, iters, conv = shares_cg(ks, D, b, tol=1e-10, max_iter=5000)
+    assert conv and iters > 0
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-6

@@ -12,6 +12,7 @@ import pytest
 import torch
 
 from num_amd import generators as g
+from _helpers import shares_cg, shares_spmv
 
 pytestmark = pytest.mark.gpu
 
@@ -164,20 +165,6 @@ def test_halo_scatter_checks(ks):
                         torch.zeros(3, dtype=torch.float64, device="cuda"), src_rank=0)
 
 
-def _shares_spmv(ks, D, v, out, xg):
-    """out[q] = share q's rows of A v on one GPU: every share packs its halo
-    (halo_pack), each neighbour's part is scattered into share q's ghost
-    block (halo_scatter), then share q's SpMV (dist_spmv); the transport
-    between pack and scatter is the part a second GPU would carry."""
-    size = len(D)
-    bufs = [ks.halo_pack(D[s], v[s]) for s in range(size)]
-    for q in range(size):
-        for src, _, _ in D[q].recv:
-            off, cnt = next((o, c) for dst, o, c in ks.send_ranges(D[src]) if dst == q)
-            ks.halo_scatter(D[q], bufs[src][off:off + cnt].contiguous(), xg[q], src_rank=src)
-        ks.dist_spmv(D[q], v[q], out[q], xg[q])
-
-
 @pytest.mark.parametrize("size", [2, 3, 4])
 def test_shares_repeated_zero_and_basis(ks, size):
     """tests/test_halo_exchange.py:263-400 on one GPU: repeated distributed
@@ -195,7 +182,7 @@ def test_shares_repeated_zero_and_basis(ks, size):
         v = [_dev(xglob[d.local_row_start:d.local_row_end]) for d in D]
         out = [torch.full((d.local_row_end - d.local_row_start,), float("nan"),
                           dtype=torch.float64, device="cuda") for d in D]
-        _shares_spmv(ks, D, v, out, xg)
+        shares_spmv(ks, D, v, out, xg)
         return np.concatenate([o.cpu().numpy() for o in out])
 
     A, D, xg = setup(100)
@@ -225,44 +212,9 @@ def test_distributed_cg_over_shares(ks, orc, size):
     A = g.laplacian_1d(100)
     D = [ks.distribute_csr(A.rowptr, A.colind, A.values, A.nrows, A.ncols, rank=r, size=size)
          for r in range(size)]
-    nl = [d.local_row_end - d.local_row_start for d in D]
-    vec = lambda n, v=0.0: torch.full((n,), v, dtype=torch.float64, device="cuda")
-    b = [vec(n, 1.0) for n in nl]
-    x = [vec(n) for n in nl]
-    r = [vec(n) for n in nl]
-    p = [vec(n) for n in nl]
-    Ap = [vec(n) for n in nl]
-    xg = [vec(d.n_ghost) for d in D]
-
-    def dist_spmv(v, out):
-        _shares_spmv(ks, D, v, out, xg)
-
-    gdot = lambda u, v: sum(ks.dot(u[s], v[s]) for s in range(size))
-    dist_spmv(x, Ap)
-    for s in range(size):
-        ks.axpy(-1.0, Ap[s], b[s], r[s])
-        ks.copy(r[s], p[s])
-    bnorm = float(np.sqrt(gdot(b, b))) or 1.0
-    rr = gdot(r, r)
-    iters, conv = 0, False
-    for it in range(1000):
-        dist_spmv(p, Ap)
-        pAp = gdot(p, Ap)
-        assert abs(pAp) >= 1e-300
-        alpha = rr / pAp
-        for s in range(size):
-            ks.axpy(alpha, p[s], x[s], x[s])
-            ks.axpy(-alpha, Ap[s], r[s], r[s])
-        rr_new = gdot(r, r)
-        if np.sqrt(rr_new) / bnorm < 1e-10:
-            iters, conv = it + 1, True
-            break
-        beta, rr = rr_new / rr, rr_new
-        for s in range(size):
-            ks.axpy2(1.0, r[s], beta, p[s], p[s])
-    assert conv
-    xh = np.concatenate([v.cpu().numpy() for v in x])
     ones = np.ones(A.nrows)
+    xh, iters, conv = shares_cg(ks, D, ones, tol=1e-10, max_iter=1000)
+    assert conv
     assert np.linalg.norm(ones - A.to_scipy() @ xh) / np.linalg.norm(ones) < 1e-8
     _, ro = orc.cg_solve_local(A.rowptr, A.colind, A.values, ones, tol=1e-10, max_iter=1000)
     assert ro["converged"] and abs(iters - ro["iters"]) <= max(1, int(0.02 * ro["iters"]))
